@@ -1,0 +1,161 @@
+/*
+ * fdtdpic.h -- C ABI of libfdtdpic.so, the MI355X (gfx950) FDTD(+PIC) step loop.
+ *
+ * The reference drives its loop from Python: Solver_Engine.Integrator*1D
+ * (Solver_Engine.py:142-371) calls ~8 numba kernels of BaseFDTD11.py per
+ * step on numpy float64 arrays owned by the Variables / CPML_Variables
+ * jitclasses (MasterController.py:145-440).  This library keeps the whole
+ * step loop on the GPU: the Python drop-in modules (Solver_Engine.py,
+ * BaseFDTD11.py, ...) build the same coefficient arrays the reference
+ * builds, hand them over once through fdtd_add_sim, advance with fdtd_run,
+ * and read fields / probes back with fdtd_read_*.
+ *
+ * Conventions
+ *   - Every function returns 0 on success or a negative FDTD_E_* code;
+ *     fdtd_last_error() returns the thread-local message of the last failure.
+ *     The library never calls exit(); the reference's sys.exit() paths
+ *     (BaseFDTD11.py:36-66, Environment_Setup.py:59-155) are raised by the
+ *     Python layer.
+ *   - Host arrays are plain pointers with explicit lengths; values are FP64 on
+ *     the host side whatever the device precision.
+ *   - A context owns one HIP stream; fdtd_run is asynchronous, fdtd_sync
+ *     waits.  Contexts are not thread-safe; use one per host thread.
+ */
+#ifndef FDTDPIC_H
+#define FDTDPIC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FDTD_OK 0
+#define FDTD_E_ARG (-1)      /* invalid argument / shape */
+#define FDTD_E_HIP (-2)      /* HIP runtime error */
+#define FDTD_E_STATE (-3)    /* call not valid in the current state */
+#define FDTD_E_NOMEM (-4)    /* device allocation failed or budget exceeded */
+
+#define FDTD_F64 0
+#define FDTD_F32 1
+
+/* slab_kind: how E is formed in the slab [F,R) */
+#define FDTD_SLAB_NONE 0     /* no slab */
+#define FDTD_SLAB_DIEL 1     /* E += keS*(dH-J), H += khS*dE  (IntegratorFreeSpace1D + UpdateCoef) */
+#define FDTD_SLAB_LINEAR 2   /* D += kD*(dH-jD*J); E = (D-P)/e          (ADE_DxUpdate + ADE_ExCreate) */
+#define FDTD_SLAB_KERR_REF 3 /* D as above; E = D/(e0*lin + e0*chi3*A), A = Re roots[0] of the x=|E|^2 cubic
+                                (AcubicFinder + NonLinExUpdate, BaseFDTD11.py:803-877) */
+#define FDTD_SLAB_KERR_E 4   /* D as above; E = real root of chi3 E^3 + e E = D - P (physical model) */
+
+#define FDTD_MAX_PROBES 16
+
+/* Fields addressable by fdtd_read_field / fdtd_write_field */
+#define FDTD_FIELD_E 0       /* Ex, length N+1 */
+#define FDTD_FIELD_H 1       /* Hy, length N+1 */
+#define FDTD_FIELD_D 2       /* Dx over the slab, length R-F */
+#define FDTD_FIELD_P 3       /* polarisationCurr over the slab */
+#define FDTD_FIELD_PM 4      /* previous polarisation P^{n-1} (tempVarPol) over the slab */
+#define FDTD_FIELD_PSIE 5    /* psi_Ex, length N+1 (zero outside the strips) */
+#define FDTD_FIELD_PSIH 6    /* psi_Hy, length N+1 */
+#define FDTD_FIELD_J 7       /* Jx as used by the E update, length N+1 */
+
+typedef struct fdtd_ctx fdtd_ctx;
+
+/*
+ * One simulation.  Cells are 0..N.  E is updated as
+ *     E[n] += ke(n)*(H[n]-H[n-1]-J[n]);  psi_E = bE psi_E + cE dH;  E -= CbE psi_E
+ * and H as
+ *     H[n] += kh(n)*(E[n+1]-E[n]);       psi_H = bH psi_H + cH dE;  H += C2H psi_H
+ * (ADE_ExUpdate BaseFDTD11.py:663-669, CPML_Psi_e_Update :364-376,
+ *  ADE_HyUpdate :640-656, CPML_Psi_m_Update :381-393).
+ * Cells outside the two strips [0,Ls) and [Rs,N] use ke0/kh0 (or keS/khS in a
+ * DIEL slab); strip cells carry their own coefficients (ke = UpExMat*den_Exdz
+ * etc.); a cell that is never updated has ke or kh = 0 there.
+ */
+typedef struct {
+    int32_t N;                 /* cells 0..N */
+    int32_t steps;             /* steps this simulation runs (reference P.timeSteps) */
+    double ke0, kh0;           /* interior Yee coefficients (UpExMat, UpHyMat) */
+    int32_t Ls, Rs;            /* strips [0,Ls) and [Rs,N] */
+    /* strip coefficients, length Ls + (N+1-Rs), left strip then right strip */
+    const double *s_ke, *s_kh, *s_bE, *s_cE, *s_CbE, *s_bH, *s_cH, *s_C2H;
+    /* slab [F,R) */
+    int32_t F, R, slab_kind, ade;
+    double keS, khS;           /* DIEL slab coefficients */
+    double kD, jD;             /* D += kD*(dH - jD*J) */
+    const double *sl_A, *sl_B, *sl_C;  /* ADE P' = A P + B Pm + C E (ADE_PolarisationCurrent_Ex :620-631) */
+    const double *sl_e;        /* LINEAR: divisor e (eps0); KERR_E: eps_inf per cell */
+    double kerr_a, kerr_b, kerr_c;     /* KERR_REF cubic a x^3 + b x^2 + c x + d, d = -(D/e0)^2 */
+    double kerr_e0, kerr_lin, kerr_chi3; /* KERR_REF: e0, sqrt(1.2), chi3Stat; KERR_E: chi3 in kerr_chi3 */
+    /* source: E[src_cell] += srcE[t]; if tfsf: H[src_cell-1] -= srcH[t] (Solver_Engine.py:307-310) */
+    int32_t src_cell, tfsf;
+    const double *srcE, *srcH; /* length steps (srcH may be NULL when tfsf == 0) */
+    /* probes: record E[probe_cell] after step t for t in [probe_t0, probe_t1] (probeSim :16-54) */
+    int32_t nprobe;
+    int32_t probe_cell[FDTD_MAX_PROBES];
+    int32_t probe_t0[FDTD_MAX_PROBES], probe_t1[FDTD_MAX_PROBES];
+    /* snapshots: row t/interval <- E after step t when t>0, t%interval==0, row<rows (vidMake :57-68) */
+    int32_t snap_interval, snap_rows;
+    /* PIC: 1 if a current array is attached (J is then re-deposited every step) */
+    int32_t has_current;
+} fdtd_sim_desc_t;
+
+/* ---- context ---------------------------------------------------------- */
+/* precision: FDTD_F64 or FDTD_F32 (FP32 fields, Kahan-compensated D, P and
+ * fields, FP64 Kerr root).  Replaces the per-call numba dispatch of
+ * Solver_Engine.py:294-316 with one device-resident loop. */
+int fdtd_ctx_create(int precision, fdtd_ctx **out);
+int fdtd_ctx_destroy(fdtd_ctx *ctx);
+/* Temporal tiling: K cells per lane (1,2,4,8), S steps per launch (<= 32K-1).
+ * S = 0 picks the library default for K. */
+int fdtd_set_tiling(fdtd_ctx *ctx, int K, int S);
+
+/* ---- simulations ------------------------------------------------------ */
+/* Uploads coefficients, allocates zeroed state (FieldInit + CPML_FieldInit,
+ * BaseFDTD11.py:36-82,195-219) and makes the simulation active from the next
+ * fdtd_run.  The descriptor's host arrays may be freed on return. */
+int fdtd_add_sim(fdtd_ctx *ctx, const fdtd_sim_desc_t *desc, int *sim_id);
+/* Frees a simulation's device memory (it stops being advanced). */
+int fdtd_remove_sim(fdtd_ctx *ctx, int sim_id);
+/* Zeroes a simulation's state (fields, polarisation, CPML psi, probes) and
+ * restarts it at step 0 from the next fdtd_run (a fresh FieldInit without
+ * re-uploading coefficients).  Asynchronous on the context stream. */
+int fdtd_reset_sim(fdtd_ctx *ctx, int sim_id);
+/* Steps a simulation has completed. */
+int fdtd_sim_progress(fdtd_ctx *ctx, int sim_id, int64_t *steps_done);
+
+/* ---- stepping --------------------------------------------------------- */
+/* Advances every active, unfinished simulation by up to nsteps steps
+ * (the `for counts in range(P.timeSteps)` bodies of Solver_Engine.py:167,236,294).
+ * Asynchronous on the context stream. */
+int fdtd_run(fdtd_ctx *ctx, int64_t nsteps);
+/* Advances until every active simulation has run all its steps. */
+int fdtd_run_all(fdtd_ctx *ctx);
+int fdtd_sync(fdtd_ctx *ctx);
+
+/* ---- data movement (host arrays FP64) --------------------------------- */
+int fdtd_read_field(fdtd_ctx *ctx, int sim_id, int field, double *out, int64_t len);
+int fdtd_write_field(fdtd_ctx *ctx, int sim_id, int field, const double *in, int64_t len);
+/* probes as nprobe x steps, row k holds probe k (x1ColBe/x1ColAf/Port1/Port2) */
+int fdtd_read_probes(fdtd_ctx *ctx, int sim_id, double *out, int64_t len);
+/* snapshots as snap_rows x (N+1) (Ex_History, MasterController.py:160) */
+int fdtd_read_snapshots(fdtd_ctx *ctx, int sim_id, double *out, int64_t len);
+
+/* ---- diagnostics ------------------------------------------------------ */
+const char *fdtd_last_error(void);
+/* Device time (ms) spent in stencil launches since the last reset, measured
+ * with HIP events on the context stream; also the launch count. */
+int fdtd_timing(fdtd_ctx *ctx, double *ms, int64_t *launches, int reset);
+/* Profiling: when on, every stencil launch is bracketed by HIP events on the
+ * context stream so fdtd_timing reports the summed kernel durations and the
+ * launch count (the average launch duration used for the roofline). */
+int fdtd_set_profile(fdtd_ctx *ctx, int on);
+/* Bytes of device memory the context currently holds. */
+int fdtd_device_bytes(fdtd_ctx *ctx, int64_t *bytes);
+/* Library version string. */
+const char *fdtd_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FDTDPIC_H */

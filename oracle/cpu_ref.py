@@ -1,0 +1,112 @@
+"""Oracle: ctypes loader for fdtd_cpu.c.  TEST INFRASTRUCTURE ONLY.
+
+`build_lib()` compiles oracle/fdtd_cpu.c with gcc -O3 -march=native
+-ffp-contract=off -fopenmp into oracle/libfdtd_cpu.so; `run_model()` runs a
+model dict from oracle.physical.build on it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libfdtd_cpu.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class PhysT(C.Structure):
+    _fields_ = [("N", C.c_int32), ("steps", C.c_int32), ("Sc", C.c_double)] + \
+        [(n, C.c_int32) for n in ("e_l0", "e_l1", "e_r0", "e_r1", "h_l0", "h_l1", "h_r0", "h_r1")] + \
+        [(n, _dp) for n in ("bE", "cE", "bH", "cH")] + \
+        [("F", C.c_int32), ("R", C.c_int32)] + \
+        [(n, _dp) for n in ("A", "B", "C", "eps")] + \
+        [("chi3", C.c_double), ("src_cell", C.c_int32), ("src", _dp), ("nprobe", C.c_int32),
+         ("probes", _ip), ("probe_out", _dp), ("J", _dp)]
+
+
+class StateT(C.Structure):
+    _fields_ = [(n, _dp) for n in ("E", "H", "psiE", "psiH", "D", "P", "Pm")]
+
+
+def build_lib(force=False):
+    src = os.path.join(HERE, "fdtd_cpu.c")
+    if force or not os.path.exists(LIB) or os.path.getmtime(LIB) < os.path.getmtime(src):
+        subprocess.check_call(["gcc", "-O3", "-march=native", "-ffp-contract=off", "-fopenmp",
+                               "-fPIC", "-shared", "-o", LIB, src, "-lm"])
+    return LIB
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build_lib()
+        _lib = C.CDLL(LIB)
+        _lib.cpu_phys_run.restype = C.c_double
+        _lib.cpu_phys_run.argtypes = [C.POINTER(PhysT), C.POINTER(StateT), C.c_int, C.c_int, _dp, C.c_int]
+        _lib.cpu_phys_run_many.restype = C.c_double
+        _lib.cpu_max_threads.restype = C.c_int
+    return _lib
+
+
+def _p(a, t=_dp):
+    return a.ctypes.data_as(t)
+
+
+def _ranges(mask, N):
+    idx = np.nonzero(mask)[0]
+    left = idx[idx < N // 2]
+    right = idx[idx >= N // 2]
+    lr = (int(left[0]), int(left[-1]) + 1) if len(left) else (0, 0)
+    rr = (int(right[0]), int(right[-1]) + 1) if len(right) else (0, 0)
+    return lr, rr
+
+
+class CpuSim:
+    """Holds the ctypes structs and numpy buffers for one model."""
+
+    def __init__(self, m, J=None):
+        N = m["N"]
+        self.m = m
+        self.keep = []
+        pe = m["pmlE"].copy()
+        pe[0] = pe[N] = False
+        ph = m["pmlH"].copy()
+        ph[N] = False
+        (el0, el1), (er0, er1) = _ranges(pe, N)
+        (hl0, hl1), (hr0, hr1) = _ranges(ph, N)
+        arr = lambda a: self._keep(np.ascontiguousarray(a, dtype=np.float64))
+        self.probes = np.ascontiguousarray(m["probes"], dtype=np.int32)
+        self.probe_out = np.zeros((len(self.probes), m["steps"]))
+        self.J = None if J is None else np.ascontiguousarray(J, dtype=np.float64)
+        self.p = PhysT(N, m["steps"], m["Sc"], el0, el1, er0, er1, hl0, hl1, hr0, hr1,
+                       _p(arr(m["bE"])), _p(arr(m["cE"])), _p(arr(m["bH"])), _p(arr(m["cH"])),
+                       m["F"], m["R"], _p(arr(m["A"])), _p(arr(m["B"])), _p(arr(m["C"])),
+                       _p(arr(m["eps"])), m["chi3"], m["src_cell"], _p(arr(m["src"])),
+                       len(self.probes), _p(self.probes, _ip), _p(self.probe_out),
+                       _p(self.J) if self.J is not None else _dp())
+        ns = m["R"] - m["F"]
+        self.E, self.H, self.psiE, self.psiH = (np.zeros(N + 1) for _ in range(4))
+        self.D, self.P, self.Pm = (np.zeros(max(ns, 1)) for _ in range(3))
+        self.s = StateT(*(_p(a) for a in (self.E, self.H, self.psiE, self.psiH, self.D, self.P, self.Pm)))
+        self.scratch = np.zeros(N + 1)
+
+    def _keep(self, a):
+        self.keep.append(a)
+        return a
+
+    def run(self, t0=0, t1=None, threads=1):
+        t1 = self.m["steps"] if t1 is None else t1
+        return lib().cpu_phys_run(C.byref(self.p), C.byref(self.s), t0, t1, _p(self.scratch), threads)
+
+
+def run_model(m, threads=1, steps=None):
+    sim = CpuSim(m)
+    secs = sim.run(0, m["steps"] if steps is None else steps, threads)
+    return sim, secs

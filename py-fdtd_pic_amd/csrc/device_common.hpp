@@ -1,0 +1,127 @@
+// Device-side types and helpers shared by the stencil and PIC kernels (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fdtd {
+
+constexpr int kMaxProbes = 16;
+
+// Per-simulation descriptor as the kernels see it.  One per simulation,
+// read by the tile kernel with wave-uniform (scalar) loads.  State arrays
+// are ping-ponged: a launch reads [par] and writes [par^1].
+template <typename R>
+struct SimDev {
+    int32_t N, T, Ls, Rs, F, Rr;
+    int32_t slab_kind, ade, src_cell, tfsf;
+    int32_t nprobe, snap_interval, snap_rows, has_current;
+    int64_t launch_begin;     // global launch index of this simulation's first launch
+    int64_t t_begin;          // global step at which the simulation's step 0 runs
+    int32_t probe_cell[kMaxProbes], probe_t0[kMaxProbes], probe_t1[kMaxProbes];
+    R ke0, kh0, keS, khS, kD, jD;
+    double kerr_a, kerr_b, kerr_c, kerr_e0, kerr_lin, kerr_chi3;
+    R *E[2], *H[2], *cE[2], *cH[2];             // N+1 cells
+    R *D[2], *P[2], *Pm[2], *cD[2], *cP[2];     // R-F slab cells (Pm holds V in FP32)
+    R *psiE[2], *psiH[2];                       // strip cells
+    const R *s_ke, *s_kh, *s_bE, *s_cE, *s_CbE, *s_bH, *s_cH, *s_C2H;
+    const R *sl_A, *sl_B, *sl_C, *sl_e;         // FP64: A,B,C ; FP32: -B, 1-A-B, C
+    const R *srcE, *srcH;
+    R *probe_out;                               // nprobe x T
+    R *snap;                                    // snap_rows x (N+1)
+    const R *J;                                 // N+1 (PIC current, read every step)
+};
+
+enum SlabKind { SLAB_NONE = 0, SLAB_DIEL = 1, SLAB_LINEAR = 2, SLAB_KERR_REF = 3, SLAB_KERR_E = 4 };
+
+// ---- cross-lane shifts inside one wavefront (DPP, GFX9 wave_shr/wave_shl) ----
+// shr1: lane i receives lane i-1 (lane 0 receives 0); shl1: lane i receives lane i+1.
+__device__ __forceinline__ int dpp_shr1_i(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false); }
+__device__ __forceinline__ int dpp_shl1_i(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, false); }
+__device__ __forceinline__ float shr1(float v) { return __int_as_float(dpp_shr1_i(__float_as_int(v))); }
+__device__ __forceinline__ float shl1(float v) { return __int_as_float(dpp_shl1_i(__float_as_int(v))); }
+__device__ __forceinline__ double shr1(double v) {
+    long long b = __double_as_longlong(v);
+    int lo = dpp_shr1_i((int)(b & 0xffffffffll)), hi = dpp_shr1_i((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double shl1(double v) {
+    long long b = __double_as_longlong(v);
+    int lo = dpp_shl1_i((int)(b & 0xffffffffll)), hi = dpp_shl1_i((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Kahan-compensated s += inc; (s - c) is the compensated value.
+template <typename R>
+__device__ __forceinline__ void kahan_add(R& s, R& c, R inc) {
+    R y = inc - c;
+    R t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+// ---- Kerr, reference path: Re(roots[0]) of a x^3 + b x^2 + c x + d (FP64) ----
+// Restates the 1728.org closed form vendored at CubicEquationSolver.py:29-105
+// (branch order and tests kept), as consumed by Nonlin_Cubic_Solver
+// (BaseFDTD11.py:834-838): only roots[0] is used, |d| <= 1e-8 gives 0.
+__device__ inline double cubic_root0_1728(double a, double b, double c, double d) {
+    if (!(fabs(d) > 1e-8)) return 0.0;
+    if (a == 0.0 && b == 0.0) return (-d * 1.0) / c;
+    if (a == 0.0) {
+        double disc = c * c - 4.0 * b * d;
+        if (disc >= 0.0) return (-c + sqrt(disc)) / (2.0 * b);
+        return -c / (2.0 * b);               // real part of the complex pair
+    }
+    const double f = ((3.0 * c / a) - (pow(b, 2.0) / pow(a, 2.0))) / 3.0;
+    const double g = (((2.0 * pow(b, 3.0)) / pow(a, 3.0)) - ((9.0 * b * c) / pow(a, 2.0)) + (27.0 * d / a)) / 27.0;
+    const double h = pow(g, 2.0) / 4.0 + pow(f, 3.0) / 27.0;
+    if (f == 0.0 && g == 0.0 && h == 0.0) {
+        if ((d / a) >= 0.0) return pow(d / (1.0 * a), 1.0 / 3.0) * -1.0;
+        return pow(-d / (1.0 * a), 1.0 / 3.0);
+    }
+    if (h <= 0.0) {
+        const double i = sqrt((pow(g, 2.0) / 4.0) - h);
+        const double j = pow(i, 1.0 / 3.0);
+        const double k = acos(-(g / (2.0 * i)));
+        return 2.0 * j * cos(k / 3.0) - (b / (3.0 * a));
+    }
+    const double sh = sqrt(h);
+    const double Rv = -(g / 2.0) + sh;
+    const double S = Rv >= 0.0 ? pow(Rv, 1.0 / 3.0) : pow(-Rv, 1.0 / 3.0) * -1.0;
+    const double Tv = -(g / 2.0) - sh;
+    const double U = Tv >= 0.0 ? pow(Tv, 1.0 / 3.0) : pow(-Tv, 1.0 / 3.0) * -1.0;
+    return (S + U) - (b / (3.0 * a));
+}
+
+// ---- Kerr, physical path: real root of chi3 E^3 + eps E = u (chi3 >= 0), FP64 ----
+// With x = u/eps and z = chi3 x^2/eps (27 z / 4 is the ratio q^2/(4 p^3/27) that
+// sets the sign of the depressed cubic's discriminant), E = x*y where
+// y = 1 - z y^3.  For |z| < 0.01 y is the alternating Fuss-Catalan series
+// sum_k (-1)^k C(3k,k)/(2k+1) z^k (14 terms, error < 1e-16); otherwise the
+// hyperbolic closed form E = -2 s sinh(asinh(3q/(2p s))/3), s = sqrt(p/3).
+__device__ inline double kerr_root_e(double u, double eps, double chi3) {
+    const double x = u / eps;
+    if (chi3 == 0.0) return x;
+    const double z = chi3 * x * x / eps;
+    if (z < 0.01) {
+        double y = 300830572.0;
+        y = fma(y, -z, 50067108.0);
+        y = fma(y, -z, 8414640.0);
+        y = fma(y, -z, 1430715.0);
+        y = fma(y, -z, 246675.0);
+        y = fma(y, -z, 43263.0);
+        y = fma(y, -z, 7752.0);
+        y = fma(y, -z, 1428.0);
+        y = fma(y, -z, 273.0);
+        y = fma(y, -z, 55.0);
+        y = fma(y, -z, 12.0);
+        y = fma(y, -z, 3.0);
+        y = fma(y, -z, 1.0);
+        y = fma(y, -z, 1.0);
+        return x * y;
+    }
+    const double p = eps / chi3, q = -u / chi3;
+    const double s = sqrt(p / 3.0);
+    return -2.0 * s * sinh(asinh((1.5 * q / p) / s) / 3.0);
+}
+
+}  // namespace fdtd

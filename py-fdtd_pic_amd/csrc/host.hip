@@ -1,0 +1,535 @@
+// C-ABI of libfdtdpic.so: contexts, per-simulation device state, tile lists
+// and the launch loop around the stencil kernel.  See include/fdtdpic.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/fdtdpic.h"
+#include "device_common.hpp"
+#include "host_internal.hpp"
+
+namespace fdtd {
+template <typename R, bool KAHAN>
+hipError_t launch_stencil(const SimDev<R>* sims, const int2* tiles, int ntiles, long long launch_idx,
+                          long long gstep0, int nsteps, int K, int S, hipStream_t st);
+}
+
+using namespace fdtd;
+
+static thread_local std::string g_err;
+
+int fdtd_fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(x)                                                                              \
+    do {                                                                                       \
+        hipError_t e_ = (x);                                                                   \
+        if (e_ != hipSuccess) return fdtd_fail(FDTD_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+static size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// ------------------------------------------------------------------------
+// simulation block layout
+// ------------------------------------------------------------------------
+struct Layout {
+    size_t off[64];
+    int n = 0;
+    size_t total = 0;
+    size_t add(size_t bytes) {
+        size_t o = total;
+        off[n++] = o;
+        total = align_up(total + bytes, 256);
+        return o;
+    }
+};
+
+template <typename R>
+static void fill_dev(SimHost& s, SimDev<R>& d) {
+    const fdtd_sim_desc_t& q = s.d;
+    char* blk = (char*)s.block;
+    const int N = q.N, ns = s.ns, nstrip = s.nstrip, T = q.steps;
+    std::memset(&d, 0, sizeof(d));
+    d.N = N; d.T = T; d.Ls = q.Ls; d.Rs = q.Rs; d.F = q.F; d.Rr = q.R;
+    d.slab_kind = q.slab_kind; d.ade = q.ade; d.src_cell = q.src_cell; d.tfsf = q.tfsf;
+    d.nprobe = q.nprobe; d.snap_interval = q.snap_interval; d.snap_rows = q.snap_rows;
+    d.has_current = q.has_current;
+    d.launch_begin = s.launch_begin; d.t_begin = s.t_begin;
+    for (int p = 0; p < FDTD_MAX_PROBES; ++p) {
+        d.probe_cell[p] = q.probe_cell[p]; d.probe_t0[p] = q.probe_t0[p]; d.probe_t1[p] = q.probe_t1[p];
+    }
+    d.ke0 = (R)q.ke0; d.kh0 = (R)q.kh0; d.keS = (R)q.keS; d.khS = (R)q.khS; d.kD = (R)q.kD; d.jD = (R)q.jD;
+    d.kerr_a = q.kerr_a; d.kerr_b = q.kerr_b; d.kerr_c = q.kerr_c;
+    d.kerr_e0 = q.kerr_e0; d.kerr_lin = q.kerr_lin; d.kerr_chi3 = q.kerr_chi3;
+    auto P = [&](int k) { return (R*)(blk + s.lay_off[k]); };
+    int k = 0;
+    const R* cs[8];
+    for (int i = 0; i < 8; ++i) cs[i] = P(k++);
+    d.s_ke = cs[0]; d.s_kh = cs[1]; d.s_bE = cs[2]; d.s_cE = cs[3];
+    d.s_CbE = cs[4]; d.s_bH = cs[5]; d.s_cH = cs[6]; d.s_C2H = cs[7];
+    d.sl_A = P(k++); d.sl_B = P(k++); d.sl_C = P(k++); d.sl_e = P(k++);
+    d.srcE = P(k++); d.srcH = P(k++);
+    for (int b = 0; b < 2; ++b) {
+        d.E[b] = P(k++); d.H[b] = P(k++); d.cE[b] = P(k++); d.cH[b] = P(k++);
+        d.D[b] = P(k++); d.P[b] = P(k++); d.Pm[b] = P(k++); d.cD[b] = P(k++); d.cP[b] = P(k++);
+        d.psiE[b] = P(k++); d.psiH[b] = P(k++);
+    }
+    d.probe_out = P(k++);
+    d.snap = P(k++);
+    d.J = q.has_current ? P(k++) : nullptr;
+    (void)N; (void)ns; (void)nstrip; (void)T;
+}
+
+template <typename R>
+static int build_sim(fdtd_ctx* ctx, SimHost& s, const fdtd_sim_desc_t* q) {
+    const int N = q->N, T = q->steps;
+    const int ns = std::max(q->R - q->F, 0);
+    const int nstrip = q->Ls + (N + 1 - q->Rs);
+    s.ns = ns; s.nstrip = nstrip;
+    Layout L;
+    const size_t rs = sizeof(R);
+    for (int i = 0; i < 8; ++i) L.add(rs * std::max(nstrip, 1));      // strip coefficients
+    for (int i = 0; i < 4; ++i) L.add(rs * std::max(ns, 1));          // slab coefficients
+    L.add(rs * T); L.add(rs * T);                                      // srcE, srcH
+    const size_t coef_end = L.total;
+    for (int b = 0; b < 2; ++b) {
+        for (int i = 0; i < 4; ++i) L.add(rs * (N + 1));              // E H cE cH
+        for (int i = 0; i < 5; ++i) L.add(rs * std::max(ns, 1));      // D P Pm cD cP
+        for (int i = 0; i < 2; ++i) L.add(rs * std::max(nstrip, 1));  // psiE psiH
+    }
+    L.add(rs * (size_t)std::max(q->nprobe, 1) * T);                   // probes
+    L.add(rs * (size_t)std::max(q->snap_rows, 1) * (N + 1));           // snapshots
+    if (q->has_current) L.add(rs * (N + 1));                           // J
+    for (int i = 0; i < L.n; ++i) s.lay_off[i] = L.off[i];
+    s.bytes = L.total;
+    if (hipMalloc(&s.block, s.bytes) != hipSuccess) {
+        s.block = nullptr;
+        return fdtd_fail(FDTD_E_NOMEM, "hipMalloc of " + std::to_string(s.bytes) + " bytes failed");
+    }
+    // coefficient image (host) -> one copy; state region zeroed.
+    std::vector<R> img(coef_end / rs, R(0));
+    auto put = [&](int k, const double* src, int n) {
+        if (!src) return;
+        R* dst = img.data() + L.off[k] / rs;
+        for (int i = 0; i < n; ++i) dst[i] = (R)src[i];
+    };
+    const double* sc[8] = {q->s_ke, q->s_kh, q->s_bE, q->s_cE, q->s_CbE, q->s_bH, q->s_cH, q->s_C2H};
+    for (int i = 0; i < 8; ++i) put(i, sc[i], nstrip);
+    if (ns > 0 && q->slab_kind >= FDTD_SLAB_LINEAR) {
+        R* A = img.data() + L.off[8] / rs;
+        R* B = img.data() + L.off[9] / rs;
+        R* C = img.data() + L.off[10] / rs;
+        for (int i = 0; i < ns; ++i) {
+            const double a = q->ade ? q->sl_A[i] : 0.0, b = q->ade ? q->sl_B[i] : 0.0, c = q->ade ? q->sl_C[i] : 0.0;
+            if (sizeof(R) == 4) {   // velocity form coefficients: -B, 1-A-B, C
+                A[i] = (R)(-b); B[i] = (R)(1.0 - a - b); C[i] = (R)c;
+            } else {
+                A[i] = (R)a; B[i] = (R)b; C[i] = (R)c;
+            }
+        }
+        put(11, q->sl_e, ns);
+    }
+    put(12, q->srcE, T);
+    put(13, q->srcH, T);
+    HIPCHK(hipMemsetAsync(s.block, 0, s.bytes, ctx->st));
+    HIPCHK(hipMemcpyAsync(s.block, img.data(), coef_end, hipMemcpyHostToDevice, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return FDTD_OK;
+}
+
+static int validate(const fdtd_sim_desc_t* q, int prec) {
+    if (!q) return fdtd_fail(FDTD_E_ARG, "null descriptor");
+    if (q->N < 2 || q->steps < 1) return fdtd_fail(FDTD_E_ARG, "N must be >= 2 and steps >= 1");
+    if (q->Ls < 0 || q->Rs > q->N + 1 || q->Ls > q->Rs)
+        return fdtd_fail(FDTD_E_ARG, "strips must satisfy 0 <= Ls <= Rs <= N+1");
+    if (q->slab_kind < 0 || q->slab_kind > FDTD_SLAB_KERR_E) return fdtd_fail(FDTD_E_ARG, "bad slab_kind");
+    if (q->slab_kind != FDTD_SLAB_NONE && (q->F < 1 || q->R > q->N + 1 || q->F >= q->R))
+        return fdtd_fail(FDTD_E_ARG, "slab must satisfy 1 <= F < R <= N+1");
+    if (q->slab_kind >= FDTD_SLAB_LINEAR && (!q->sl_e || (q->ade && (!q->sl_A || !q->sl_B || !q->sl_C))))
+        return fdtd_fail(FDTD_E_ARG, "slab coefficient arrays missing");
+    if (q->slab_kind == FDTD_SLAB_KERR_REF && prec != FDTD_F64)
+        return fdtd_fail(FDTD_E_ARG, "the reference Kerr path runs in FP64 only");
+    if (q->slab_kind == FDTD_SLAB_KERR_E && q->kerr_chi3 < 0)
+        return fdtd_fail(FDTD_E_ARG, "KERR_E needs chi3 >= 0");
+    if (q->src_cell < 1 || q->src_cell > q->N) return fdtd_fail(FDTD_E_ARG, "source cell out of range");
+    if (!q->srcE) return fdtd_fail(FDTD_E_ARG, "srcE missing");
+    if (q->tfsf && !q->srcH) return fdtd_fail(FDTD_E_ARG, "srcH missing with tfsf");
+    if (q->slab_kind >= FDTD_SLAB_LINEAR && q->src_cell >= q->F - (q->tfsf ? 1 : 0) && q->src_cell < q->R + (q->tfsf ? 1 : 0))
+        return fdtd_fail(FDTD_E_ARG, "source inside (or, with TFSF, adjacent to) the slab");
+    const int nstrip = q->Ls + (q->N + 1 - q->Rs);
+    if (nstrip > 0 && (!q->s_ke || !q->s_kh || !q->s_bE || !q->s_cE || !q->s_CbE || !q->s_bH || !q->s_cH || !q->s_C2H))
+        return fdtd_fail(FDTD_E_ARG, "strip coefficient arrays missing");
+    if (q->nprobe < 0 || q->nprobe > FDTD_MAX_PROBES) return fdtd_fail(FDTD_E_ARG, "nprobe out of range");
+    for (int p = 0; p < q->nprobe; ++p)
+        if (q->probe_cell[p] < 0 || q->probe_cell[p] > q->N) return fdtd_fail(FDTD_E_ARG, "probe cell out of range");
+    if (q->snap_interval < 0 || q->snap_rows < 0) return fdtd_fail(FDTD_E_ARG, "bad snapshot settings");
+    return FDTD_OK;
+}
+
+// ------------------------------------------------------------------------
+// C ABI
+// ------------------------------------------------------------------------
+extern "C" {
+
+const char* fdtd_last_error(void) { return g_err.c_str(); }
+const char* fdtd_version(void) { return "fdtdpic-gfx950 0.1"; }
+
+int fdtd_ctx_create(int precision, fdtd_ctx** out) {
+    if (!out) return fdtd_fail(FDTD_E_ARG, "null out");
+    if (precision != FDTD_F64 && precision != FDTD_F32) return fdtd_fail(FDTD_E_ARG, "bad precision");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fdtd_fail(FDTD_E_HIP, "no HIP device visible");
+    fdtd_ctx* c = new fdtd_ctx();
+    c->prec = precision;
+    c->K = precision == FDTD_F64 ? 2 : 4;
+    c->S = default_S(c->K);
+    if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fdtd_fail(FDTD_E_HIP, "hipStreamCreate failed");
+    }
+    *out = c;
+    return FDTD_OK;
+}
+
+int fdtd_ctx_destroy(fdtd_ctx* c) {
+    if (!c) return FDTD_OK;
+    hipStreamSynchronize(c->st);
+    for (SimHost* s : c->sims)
+        if (s) { if (s->block) hipFree(s->block); delete s; }
+    if (c->d_sims) hipFree(c->d_sims);
+    if (c->d_tiles) hipFree(c->d_tiles);
+    for (auto& e : c->events) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
+    hipStreamDestroy(c->st);
+    delete c;
+    return FDTD_OK;
+}
+
+int fdtd_set_tiling(fdtd_ctx* c, int K, int S) {
+    if (!c) return fdtd_fail(FDTD_E_ARG, "null ctx");
+    if (K != 1 && K != 2 && K != 4 && K != 8) return fdtd_fail(FDTD_E_ARG, "K must be 1, 2, 4 or 8");
+    if (S == 0) S = default_S(K);
+    if (S < 1 || 64 * K - 2 * S < 1) return fdtd_fail(FDTD_E_ARG, "S must satisfy 1 <= S <= 32K-1");
+    c->K = K; c->S = S; c->dirty = true;
+    return FDTD_OK;
+}
+
+int fdtd_add_sim(fdtd_ctx* c, const fdtd_sim_desc_t* q, int* sim_id) {
+    if (!c || !sim_id) return fdtd_fail(FDTD_E_ARG, "null argument");
+    int rc = validate(q, c->prec);
+    if (rc) return rc;
+    SimHost* s = new SimHost();
+    s->d = *q;
+    s->launch_begin = c->launch_idx;
+    s->t_begin = c->gstep;
+    rc = c->prec == FDTD_F64 ? build_sim<double>(c, *s, q) : build_sim<float>(c, *s, q);
+    if (rc) { delete s; return rc; }
+    // host pointers are not kept
+    s->d.s_ke = s->d.s_kh = s->d.s_bE = s->d.s_cE = s->d.s_CbE = s->d.s_bH = s->d.s_cH = s->d.s_C2H = nullptr;
+    s->d.sl_A = s->d.sl_B = s->d.sl_C = s->d.sl_e = s->d.srcE = s->d.srcH = nullptr;
+    c->dev_bytes += s->bytes;
+    c->sims.push_back(s);
+    *sim_id = (int)c->sims.size() - 1;
+    c->dirty = true;
+    return FDTD_OK;
+}
+
+int fdtd_remove_sim(fdtd_ctx* c, int id) {
+    if (!c || id < 0 || id >= (int)c->sims.size() || !c->sims[id]) return fdtd_fail(FDTD_E_ARG, "bad sim id");
+    HIPCHK(hipStreamSynchronize(c->st));
+    SimHost* s = c->sims[id];
+    if (s->block) hipFree(s->block);
+    c->dev_bytes -= s->bytes;
+    delete s;
+    c->sims[id] = nullptr;
+    c->dirty = true;
+    return FDTD_OK;
+}
+
+int fdtd_reset_sim(fdtd_ctx* c, int id) {
+    if (!c || id < 0 || id >= (int)c->sims.size() || !c->sims[id]) return fdtd_fail(FDTD_E_ARG, "bad sim id");
+    SimHost* s = c->sims[id];
+    const size_t state0 = s->lay_off[14];   // first state sub-array (after coefficients and sources)
+    HIPCHK(hipMemsetAsync((char*)s->block + state0, 0, s->bytes - state0, c->st));
+    s->steps_done = 0;
+    s->launches = 0;
+    s->launch_begin = c->launch_idx;
+    s->t_begin = c->gstep;
+    c->dirty = true;
+    return FDTD_OK;
+}
+
+int fdtd_sim_progress(fdtd_ctx* c, int id, int64_t* done) {
+    if (!c || !done || id < 0 || id >= (int)c->sims.size() || !c->sims[id]) return fdtd_fail(FDTD_E_ARG, "bad sim id");
+    *done = c->sims[id]->steps_done;
+    return FDTD_OK;
+}
+
+}  // extern "C"
+
+// Rebuild the device SimDev array and the tile list for active, unfinished sims.
+template <typename R>
+static int rebuild_t(fdtd_ctx* c) {
+    const int nsim = (int)c->sims.size();
+    std::vector<SimDev<R>> devs(std::max(nsim, 1));
+    std::vector<int2> tiles;
+    const int U = 64 * c->K - 2 * c->S;
+    c->tile_sims.clear();
+    for (int i = 0; i < nsim; ++i) {
+        SimHost* s = c->sims[i];
+        if (!s) continue;
+        fill_dev<R>(*s, devs[i]);
+        if (s->steps_done >= s->d.steps) continue;
+        c->tile_sims.push_back(i);
+        for (int o = 0; o <= s->d.N; o += U) tiles.push_back(make_int2(i, o));
+    }
+    const size_t need = sizeof(SimDev<R>) * devs.size();
+    if (need > c->d_sims_cap) {
+        if (c->d_sims) HIPCHK(hipFree(c->d_sims));
+        HIPCHK(hipMalloc(&c->d_sims, need * 2));
+        c->d_sims_cap = need * 2;
+    }
+    HIPCHK(hipMemcpyAsync(c->d_sims, devs.data(), need, hipMemcpyHostToDevice, c->st));
+    if (tiles.size() > c->d_tiles_cap) {
+        if (c->d_tiles) HIPCHK(hipFree(c->d_tiles));
+        HIPCHK(hipMalloc(&c->d_tiles, sizeof(int2) * tiles.size() * 2));
+        c->d_tiles_cap = tiles.size() * 2;
+    }
+    if (!tiles.empty())
+        HIPCHK(hipMemcpyAsync(c->d_tiles, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice, c->st));
+    c->ntiles = (int)tiles.size();
+    c->dirty = false;
+    return FDTD_OK;
+}
+
+static int launch_once(fdtd_ctx* c, int n) {
+    hipError_t e;
+    if (c->prec == FDTD_F64)
+        e = launch_stencil<double, false>((const SimDev<double>*)c->d_sims, c->d_tiles, c->ntiles, c->launch_idx,
+                                          c->gstep, n, c->K, c->S, c->st);
+    else
+        e = launch_stencil<float, true>((const SimDev<float>*)c->d_sims, c->d_tiles, c->ntiles, c->launch_idx,
+                                        c->gstep, n, c->K, c->S, c->st);
+    if (e != hipSuccess) return fdtd_fail(FDTD_E_HIP, std::string("stencil launch: ") + hipGetErrorString(e));
+    return FDTD_OK;
+}
+
+extern "C" {
+
+int fdtd_run(fdtd_ctx* c, int64_t nsteps) {
+    if (!c) return fdtd_fail(FDTD_E_ARG, "null ctx");
+    if (nsteps < 0) return fdtd_fail(FDTD_E_ARG, "nsteps < 0");
+    int64_t remaining = nsteps;
+    while (remaining > 0) {
+        if (c->dirty) {
+            int rc = c->prec == FDTD_F64 ? rebuild_t<double>(c) : rebuild_t<float>(c);
+            if (rc) return rc;
+        }
+        if (c->ntiles == 0) break;
+        // steps this launch: S, bounded by the remaining request and by PIC coupling
+        int n = (int)std::min<int64_t>(remaining, c->S);
+        std::pair<hipEvent_t, hipEvent_t>* ev = nullptr;
+        if (c->profile) {
+            if (c->ev_used == c->events.size()) {
+                hipEvent_t a, b;
+                HIPCHK(hipEventCreate(&a));
+                HIPCHK(hipEventCreate(&b));
+                c->events.emplace_back(a, b);
+            }
+            ev = &c->events[c->ev_used++];
+            HIPCHK(hipEventRecord(ev->first, c->st));
+        }
+        int rc = launch_once(c, n);
+        if (rc) return rc;
+        if (ev) HIPCHK(hipEventRecord(ev->second, c->st));
+        c->launches++;
+        bool finished = false;
+        for (int i : c->tile_sims) {
+            SimHost* s = c->sims[i];
+            s->launches++;
+            const int64_t done = std::min<int64_t>(s->d.steps, c->gstep + n - s->t_begin);
+            s->steps_done = std::max<int64_t>(s->steps_done, done);
+            if (s->steps_done >= s->d.steps) finished = true;
+        }
+        c->launch_idx++;
+        c->gstep += n;
+        remaining -= n;
+        if (finished) c->dirty = true;
+    }
+    return FDTD_OK;
+}
+
+int fdtd_run_all(fdtd_ctx* c) {
+    if (!c) return fdtd_fail(FDTD_E_ARG, "null ctx");
+    int64_t most = 0;
+    for (SimHost* s : c->sims)
+        if (s) most = std::max<int64_t>(most, s->d.steps - s->steps_done + (s->t_begin - c->gstep > 0 ? s->t_begin - c->gstep : 0));
+    return fdtd_run(c, most);
+}
+
+int fdtd_sync(fdtd_ctx* c) {
+    if (!c) return fdtd_fail(FDTD_E_ARG, "null ctx");
+    HIPCHK(hipStreamSynchronize(c->st));
+    return FDTD_OK;
+}
+
+int fdtd_timing(fdtd_ctx* c, double* ms, int64_t* launches, int reset) {
+    if (!c) return fdtd_fail(FDTD_E_ARG, "null ctx");
+    HIPCHK(hipStreamSynchronize(c->st));
+    double tot = 0;
+    for (size_t i = 0; i < c->ev_used; ++i) {
+        float m = 0;
+        HIPCHK(hipEventElapsedTime(&m, c->events[i].first, c->events[i].second));
+        tot += m;
+    }
+    if (ms) *ms = tot;
+    if (launches) *launches = (int64_t)c->ev_used;
+    if (reset) c->ev_used = 0;
+    return FDTD_OK;
+}
+
+int fdtd_set_profile(fdtd_ctx* c, int on) {
+    if (!c) return fdtd_fail(FDTD_E_ARG, "null ctx");
+    c->profile = on != 0;
+    c->ev_used = 0;
+    return FDTD_OK;
+}
+
+int fdtd_device_bytes(fdtd_ctx* c, int64_t* bytes) {
+    if (!c || !bytes) return fdtd_fail(FDTD_E_ARG, "null argument");
+    *bytes = c->dev_bytes;
+    return FDTD_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------
+// reads / writes
+// ------------------------------------------------------------------------
+template <typename R>
+static int xfer_field(fdtd_ctx* c, SimHost& s, int field, double* out, const double* in, int64_t len) {
+    SimDev<R> d;
+    fill_dev<R>(s, d);
+    const int par = (int)(s.launches & 1);
+    const int N = s.d.N;
+    const R* src = nullptr;
+    int64_t n = 0;
+    bool strip = false;
+    switch (field) {
+        case FDTD_FIELD_E: src = d.E[par]; n = N + 1; break;
+        case FDTD_FIELD_H: src = d.H[par]; n = N + 1; break;
+        case FDTD_FIELD_D: src = d.D[par]; n = s.ns; break;
+        case FDTD_FIELD_P: src = d.P[par]; n = s.ns; break;
+        case FDTD_FIELD_PM:
+            if (sizeof(R) == 4 && out) {   // FP32 keeps the velocity V = P^n - P^{n-1}
+                std::vector<double> p(s.ns), v(s.ns);
+                int rc = xfer_field<R>(c, s, FDTD_FIELD_P, p.data(), nullptr, s.ns);
+                if (rc) return rc;
+                if (len != s.ns) return fdtd_fail(FDTD_E_ARG, "length mismatch");
+                std::vector<R> hv(s.ns);
+                if (s.ns) HIPCHK(hipMemcpy(hv.data(), d.Pm[par], sizeof(R) * s.ns, hipMemcpyDeviceToHost));
+                for (int i = 0; i < s.ns; ++i) out[i] = p[i] - (double)hv[i];
+                return FDTD_OK;
+            }
+            if (sizeof(R) == 4) return fdtd_fail(FDTD_E_ARG, "PM is derived in FP32 and cannot be written");
+            src = d.Pm[par]; n = s.ns; break;
+        case FDTD_FIELD_PSIE: src = d.psiE[par]; n = s.nstrip; strip = true; break;
+        case FDTD_FIELD_PSIH: src = d.psiH[par]; n = s.nstrip; strip = true; break;
+        case FDTD_FIELD_J: src = d.J; n = d.J ? N + 1 : 0; break;
+        default: return fdtd_fail(FDTD_E_ARG, "bad field id");
+    }
+    const int64_t want = strip ? (N + 1) : n;
+    if (len != want) return fdtd_fail(FDTD_E_ARG, "length mismatch: expected " + std::to_string(want));
+    if (n == 0) return FDTD_OK;
+    std::vector<R> h(n);
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (out) {
+        HIPCHK(hipMemcpy(h.data(), src, sizeof(R) * n, hipMemcpyDeviceToHost));
+        if (!strip) {
+            for (int64_t i = 0; i < n; ++i) out[i] = (double)h[i];
+            // FP32 Kahan state: report the compensated value for E/H/D/P
+            if (sizeof(R) == 4 && (field <= FDTD_FIELD_P)) {
+                const R* comp = field == FDTD_FIELD_E ? d.cE[par] : field == FDTD_FIELD_H ? d.cH[par]
+                              : field == FDTD_FIELD_D ? d.cD[par] : d.cP[par];
+                std::vector<R> cc(n);
+                HIPCHK(hipMemcpy(cc.data(), comp, sizeof(R) * n, hipMemcpyDeviceToHost));
+                for (int64_t i = 0; i < n; ++i) out[i] = (double)h[i] - (double)cc[i];
+            }
+        } else {
+            std::fill(out, out + N + 1, 0.0);
+            for (int i = 0; i < s.d.Ls; ++i) out[i] = (double)h[i];
+            for (int i = s.d.Rs; i <= N; ++i) out[i] = (double)h[s.d.Ls + (i - s.d.Rs)];
+        }
+    } else {
+        if (!strip) {
+            for (int64_t i = 0; i < n; ++i) h[i] = (R)in[i];
+        } else {
+            for (int i = 0; i < s.d.Ls; ++i) h[i] = (R)in[i];
+            for (int i = s.d.Rs; i <= N; ++i) h[s.d.Ls + (i - s.d.Rs)] = (R)in[i];
+        }
+        HIPCHK(hipMemcpy((void*)src, h.data(), sizeof(R) * n, hipMemcpyHostToDevice));
+        if (sizeof(R) == 4 && field <= FDTD_FIELD_P) {
+            R* comp = field == FDTD_FIELD_E ? d.cE[par] : field == FDTD_FIELD_H ? d.cH[par]
+                    : field == FDTD_FIELD_D ? d.cD[par] : d.cP[par];
+            HIPCHK(hipMemset(comp, 0, sizeof(R) * n));
+        }
+    }
+    return FDTD_OK;
+}
+
+template <typename R>
+static int read_plain(fdtd_ctx* c, const R* src, int64_t n, double* out) {
+    std::vector<R> h(n);
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (n) HIPCHK(hipMemcpy(h.data(), src, sizeof(R) * n, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < n; ++i) out[i] = (double)h[i];
+    return FDTD_OK;
+}
+
+static SimHost* get_sim(fdtd_ctx* c, int id) {
+    if (!c || id < 0 || id >= (int)c->sims.size()) return nullptr;
+    return c->sims[id];
+}
+
+extern "C" {
+
+int fdtd_read_field(fdtd_ctx* c, int id, int field, double* out, int64_t len) {
+    SimHost* s = get_sim(c, id);
+    if (!s || !out) return fdtd_fail(FDTD_E_ARG, "bad sim id or null out");
+    return c->prec == FDTD_F64 ? xfer_field<double>(c, *s, field, out, nullptr, len)
+                               : xfer_field<float>(c, *s, field, out, nullptr, len);
+}
+
+int fdtd_write_field(fdtd_ctx* c, int id, int field, const double* in, int64_t len) {
+    SimHost* s = get_sim(c, id);
+    if (!s || !in) return fdtd_fail(FDTD_E_ARG, "bad sim id or null input");
+    if (field == FDTD_FIELD_J) return fdtd_fail(FDTD_E_ARG, "J is written by the deposit");
+    return c->prec == FDTD_F64 ? xfer_field<double>(c, *s, field, nullptr, in, len)
+                               : xfer_field<float>(c, *s, field, nullptr, in, len);
+}
+
+int fdtd_read_probes(fdtd_ctx* c, int id, double* out, int64_t len) {
+    SimHost* s = get_sim(c, id);
+    if (!s || !out) return fdtd_fail(FDTD_E_ARG, "bad sim id or null out");
+    const int64_t n = (int64_t)s->d.nprobe * s->d.steps;
+    if (len != n) return fdtd_fail(FDTD_E_ARG, "length mismatch: expected " + std::to_string(n));
+    if (c->prec == FDTD_F64) { SimDev<double> d; fill_dev<double>(*s, d); return read_plain<double>(c, d.probe_out, n, out); }
+    SimDev<float> d; fill_dev<float>(*s, d); return read_plain<float>(c, d.probe_out, n, out);
+}
+
+int fdtd_read_snapshots(fdtd_ctx* c, int id, double* out, int64_t len) {
+    SimHost* s = get_sim(c, id);
+    if (!s || !out) return fdtd_fail(FDTD_E_ARG, "bad sim id or null out");
+    const int64_t n = (int64_t)s->d.snap_rows * (s->d.N + 1);
+    if (len != n) return fdtd_fail(FDTD_E_ARG, "length mismatch: expected " + std::to_string(n));
+    if (c->prec == FDTD_F64) { SimDev<double> d; fill_dev<double>(*s, d); return read_plain<double>(c, d.snap, n, out); }
+    SimDev<float> d; fill_dev<float>(*s, d); return read_plain<float>(c, d.snap, n, out);
+}
+
+}  // extern "C"

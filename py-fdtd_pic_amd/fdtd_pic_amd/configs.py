@@ -1,0 +1,106 @@
+"""Seeded synthetic workloads (physical model parameter dicts).
+
+lambda0 = 1.5 um, dz = lambda0/40 = 37.5 nm, Sc = 0.5, CPML width 32 on
+both ends.  Every random quantity is drawn from numpy.random.default_rng
+with the config number as seed, in FP64, so the oracle and the device see
+identical inputs.
+
+config1  N=2000, 10,000 steps, Lorentz slab [1000,1400) (eps_inf 1.5,
+         d_eps 2, w0 2pi 150 THz, gamma 2pi 5 THz), soft Gaussian source
+         (200 THz, tau 30 fs) at cell 100, probes at 60 and N-42.
+config2  N=65,536, 50,000 steps, slab [20000,40000) Lorentz + Kerr
+         chi3 = 1e-19 m^2/V^2, eps_inf ~ U[1,3], w0 jittered +-5% per cell,
+         source amplitude 1.8e8 V/m (peak index change ~1e-3).
+config3  N=65,536, 20,000 steps, config-1 slab, 1e6-particle electron beam
+         over cells [30000,40000), v = 0.1c z + N(0,(1e-3 c)^2).
+config4  4,096 instances, Nz log-uniform [1e3, 65,536], source centre
+         U[100,300] THz, d_eps U[0.5,3], chi3 U[0,2e-19], 10,000 steps; slab
+         20% of Nz from 0.5 Nz; 1,024 instances with 1e4-1e5 particles.
+config5  Poisson arrivals (200/s) of config-3-style requests; see serve.py.
+"""
+import math
+
+import numpy as np
+
+C0 = 299792458.0
+LAMBDA0 = 1.5e-6
+DZ = LAMBDA0 / 40
+SC = 0.5
+DT = SC * DZ / C0
+W = 32
+TWO_PI = 2 * math.pi
+
+
+def base(N, steps, **kw):
+    p = dict(N=int(N), steps=int(steps), Sc=SC, dz=DZ, pml_width=W, pml_order=3, pml_R0=1e-8,
+             slab_start=int(N) // 2, slab_stop=int(N) // 2, eps_inf=1.5, d_eps=2.0,
+             omega0=TWO_PI * 150e12, gamma=TWO_PI * 5e12, chi3=0.0, ade=True,
+             src_cell=100, src_f0=200e12, src_tau=30e-15, src_t0=120e-15, src_amp=1.0,
+             probes=[60, int(N) - W - 10])
+    p.update(kw)
+    return p
+
+
+def config1():
+    return base(2000, 10000, slab_start=1000, slab_stop=1400)
+
+
+def config2(seed=2, steps=50000, N=65536):
+    rng = np.random.default_rng(seed)
+    F, R = 20000 * N // 65536, 40000 * N // 65536
+    ns = R - F
+    eps_inf = rng.uniform(1.0, 3.0, ns)
+    omega0 = TWO_PI * 150e12 * (1.0 + rng.uniform(-0.05, 0.05, ns))
+    return base(N, steps, slab_start=F, slab_stop=R, eps_inf=eps_inf, omega0=omega0, chi3=1e-19,
+                src_amp=1.8e8)
+
+
+def config3(seed=3, steps=20000, N=65536, n_particles=1_000_000):
+    p = base(N, steps, slab_start=1000, slab_stop=1400)
+    p["beam"] = dict(seed=seed, n=int(n_particles), cell_lo=30000 * N // 65536, cell_hi=40000 * N // 65536,
+                     v_drift=0.1 * C0, v_th=1e-3 * C0, density=1e16)
+    return p
+
+
+def config4_instances(seed=4, count=4096, steps=10000):
+    rng = np.random.default_rng(seed)
+    Nz = np.round(np.exp(rng.uniform(math.log(1e3), math.log(65536), count))).astype(int)
+    fc = rng.uniform(100e12, 300e12, count)
+    d_eps = rng.uniform(0.5, 3.0, count)
+    chi3 = rng.uniform(0.0, 2e-19, count)
+    beam_ids = set(rng.permutation(count)[:count // 4].tolist())
+    nparts = rng.integers(10_000, 100_001, count)
+    out = []
+    for i in range(count):
+        N = int(Nz[i])
+        p = base(N, steps, slab_start=N // 2, slab_stop=N // 2 + N // 5, d_eps=float(d_eps[i]),
+                 chi3=float(chi3[i]), src_f0=float(fc[i]), src_amp=1.8e8 if chi3[i] > 0 else 1.0,
+                 src_cell=min(100, N // 10))
+        p["probes"] = [max(W + 4, p["src_cell"] - 40), N - W - 10]
+        if i in beam_ids:
+            lo = N // 8 * 5
+            p["beam"] = dict(seed=seed * 100003 + i, n=int(nparts[i]), cell_lo=lo,
+                             cell_hi=min(lo + N // 8, N - W - 20), v_drift=0.1 * C0, v_th=1e-3 * C0,
+                             density=1e16)
+        out.append(p)
+    return out
+
+
+def bytes_per_cell_update(p, precision="f32"):
+    """Algorithmic HBM bytes per cell-update of the one-step streaming schedule.
+
+    vacuum: E, H read+write; CPML adds psi_E, psi_H; a Lorentz slab cell adds
+    D, P, P_prev (and in FP32 the two Kahan words), plus per-cell eps_inf and
+    w0 reads when they vary per cell.  Averaged over the grid.
+    """
+    w = 8 if precision == "f64" else 4
+    N = p["N"]
+    pml = 2 * p["pml_width"]
+    ns = max(p["slab_stop"] - p["slab_start"], 0)
+    vac = N - pml - ns
+    b_vac = 4 * w
+    b_pml = 8 * w
+    b_slab = 10 * w + (0 if precision == "f64" else 4 * w)
+    if np.ndim(p["eps_inf"]) or np.ndim(p["omega0"]):
+        b_slab += 2 * w
+    return (vac * b_vac + pml * b_pml + ns * b_slab) / N

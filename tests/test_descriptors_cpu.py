@@ -1,0 +1,91 @@
+"""CPU checks of the host side: the drop-in modules and descriptor builders.
+
+The drop-in integrators are run end to end with the device call replaced by
+tests/emulator.py (the kernel's per-cell semantics in NumPy), and compared
+with the oracles.  This pins the Python host code (parameter objects,
+set-up builders, reference_spec/physical_spec, read-back) independently of
+the GPU; the `gpu` tests then pin the kernel itself.
+"""
+import numpy as np
+import pytest
+
+import emulator
+from refcases import small_setup, oracle_objects, product_objects, rel_l2
+
+FREQ = 9e9
+
+
+@pytest.fixture
+def emulated(monkeypatch):
+    from fdtd_pic_amd import reference_loop as RL
+    monkeypatch.setattr(RL, "run_pass", lambda spec, precision="f64": emulator.run(spec))
+    return RL
+
+
+def _both(setup, flags, tweak=None):
+    from oracle import faithful as FO
+    import MasterController as MC
+    Vo, Po, Co, CPo = oracle_objects(setup, FREQ, **flags)
+    Vp, Pp, Cp, CPp = product_objects(setup, FREQ, **flags)
+    if tweak:
+        tweak(Po)
+        tweak(Pp)
+    FO.controller(Vo, Po, Co, CPo)
+    MC.Controller(Vp, Pp, Cp, CPp)
+    return (Vo, Po, Co), (Vp, Pp, Cp)
+
+
+def _check(o, p, names, tol=1e-12):
+    for i, name in names:
+        err = rel_l2(getattr(p[i], name), getattr(o[i], name))
+        assert err <= tol, (name, err)
+
+
+def test_linlor_host_chain(emulated):
+    o, p = _both(small_setup(Nz=300, T=400, F=200, R=299, nzsrc=50),
+                 dict(LorentzMed=True, SineCont=True, TFSF=True))
+    _check(o, p, [(0, "Ex"), (0, "Hy"), (0, "Dx"), (0, "polarisationCurr"), (0, "tempVarPol"),
+                  (2, "psi_Ex"), (2, "psi_Hy"), (0, "x1ColBe"), (0, "x1ColAf"), (0, "Ex_History")])
+    assert np.abs(p[0].polarisationCurr).max() > 0
+
+
+def test_nl_host_chain(emulated):
+    def tw(P):
+        P.FreeSpace = False
+        P.Periods = 1000
+    o, p = _both(small_setup(Nz=200, T=150, F=140, R=199, nzsrc=40),
+                 dict(nonLinMed=True, SineCont=True, TFSF=True), tw)
+    _check(o, p, [(0, "Ex"), (0, "Hy"), (0, "Dx"), (2, "psi_Ex"), (2, "psi_Hy"), (0, "Port1"),
+                  (0, "Port2")])
+
+
+def test_free_space_host_chain(emulated):
+    def tw(P):
+        P.epsRe = 2.25
+        P.atten = True
+    o, p = _both(small_setup(Nz=300, T=300, F=180, R=260, nzsrc=50), dict(SineCont=True, TFSF=True), tw)
+    _check(o, p, [(0, "Ex"), (0, "Hy"), (2, "psi_Ex"), (2, "psi_Hy"), (0, "x1ColBe"), (0, "x1ColAf"),
+                  (0, "x1Atten")])
+
+
+def test_physical_spec_matches_oracle_model():
+    from oracle import physical as ph
+    from fdtd_pic_amd import engine
+    spec = ph.default_spec(N=500, steps=700, slab_start=250, slab_stop=330, chi3=1e-3, src_amp=2.0,
+                           src_cell=50, probes=[30, 460], src_t0=60e-15, src_tau=15e-15)
+    s_or, pr = ph.run(ph.build(spec))
+    out = emulator.run(engine.physical_spec(spec))
+    assert rel_l2(out["E"], s_or.E) <= 1e-13
+    assert rel_l2(out["H"], s_or.H) <= 1e-13
+    assert rel_l2(out["D"], s_or.D) <= 1e-13
+    assert rel_l2(out["probes"], pr) <= 1e-13
+    assert rel_l2(out["PSIE"], s_or.psiE) <= 1e-13
+
+
+def test_strip_split_is_minimal():
+    from oracle import physical as ph
+    from fdtd_pic_amd import engine
+    spec = ph.default_spec(N=400, steps=10)
+    s = engine.physical_spec(spec)
+    W = spec["pml_width"]
+    assert s.Ls == W and s.Rs == spec["N"] - W

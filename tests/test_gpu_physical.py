@@ -1,0 +1,106 @@
+"""GPU parity of the physical model (configs 1-3 shapes) against the FP64 oracle.
+
+FP64 device runs must match oracle/physical.py to relative L2 <= 1e-10; FP32
+runs (Kahan-compensated fields, FP64 Kerr root) to <= 1e-5, and the
+reflection/transmission spectra to <= 1e-4 where the incident spectrum is
+above 1% of its peak.  Different temporal tilings (K, S) must give bitwise
+identical FP64 results up to FMA contraction order (the compiler may fuse a
+different product in different K instantiations): relative L2 <= 1e-13.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from refcases import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _spec(**kw):
+    from oracle import physical as ph
+    base = dict(N=700, steps=1800, slab_start=330, slab_stop=470, src_cell=60, probes=[40, 650],
+                src_t0=60e-15, src_tau=15e-15)
+    base.update(kw)
+    return ph.default_spec(**base)
+
+
+def _device(spec, prec, K=None, S=0, with_slab=True):
+    from fdtd_pic_amd import engine
+    with engine.Context(prec) as ctx:
+        if K:
+            ctx.set_tiling(K, S)
+        sim = ctx.add(engine.physical_spec(spec, with_slab=with_slab))
+        ctx.run_all()
+        out = {k: sim.read(k) for k in ("E", "H", "PSIE", "PSIH")}
+        if with_slab:
+            out.update({k: sim.read(k) for k in ("D", "P")})
+        out["probes"] = sim.probes()
+    return out
+
+
+def _oracle(spec, with_slab=True):
+    from oracle import physical as ph
+    m = ph.build(spec, with_slab=with_slab)
+    s, probes = ph.run(m)
+    return s, probes
+
+
+@pytest.mark.parametrize("case", ["linear", "kerr_series", "kerr_closed", "no_ade"])
+def test_fp64_matches_oracle(case):
+    kw = dict(linear=dict(chi3=0.0), kerr_series=dict(chi3=1e-3, src_amp=2.0),
+              kerr_closed=dict(chi3=3e-2, src_amp=2.0), no_ade=dict(ade=False, chi3=0.0))[case]
+    spec = _spec(**kw)
+    print("case", case, kw, flush=True)
+    s, pr = _oracle(spec)
+    d = _device(spec, "f64")
+    errs = dict(E=rel_l2(d["E"], s.E), H=rel_l2(d["H"], s.H), D=rel_l2(d["D"], s.D),
+                P=rel_l2(d["P"], s.P) if case != "no_ade" else 0.0, probes=rel_l2(d["probes"], pr),
+                psiE=rel_l2(d["PSIE"], s.psiE), psiH=rel_l2(d["PSIH"], s.psiH))
+    print(errs, flush=True)
+    for k, v in errs.items():
+        assert v <= 1e-10, (k, v)
+
+
+def test_fp32_within_tolerance():
+    spec = _spec(chi3=1e-3, src_amp=2.0)
+    s, pr = _oracle(spec)
+    d = _device(spec, "f32")
+    errs = dict(E=rel_l2(d["E"], s.E), H=rel_l2(d["H"], s.H), D=rel_l2(d["D"], s.D),
+                P=rel_l2(d["P"], s.P), probes=rel_l2(d["probes"], pr))
+    print(errs, flush=True)
+    for k, v in errs.items():
+        assert v <= 1e-5, (k, v)
+
+
+@pytest.mark.parametrize("K,S", [(1, 3), (1, 31), (2, 16), (4, 8), (4, 100), (8, 64), (8, 255)])
+def test_tilings_agree(K, S):
+    spec = _spec(chi3=1e-3, src_amp=2.0, steps=900)
+    base = _device(spec, "f64", K=2, S=1)
+    d = _device(spec, "f64", K=K, S=S)
+    errs = {k: rel_l2(d[k], base[k]) for k in ("E", "H", "D", "P", "PSIE", "PSIH", "probes")}
+    print(K, S, {k: f"{v:.1e}" for k, v in errs.items()}, flush=True)
+    for k, v in errs.items():
+        assert v <= 1e-13, (K, S, k, v)
+
+
+def test_spectra_fp32_vs_fp64():
+    """Two-pass protocol: R(f), T(f) from FP32 device runs vs the FP64 oracle."""
+    from oracle import physical as ph
+    from fdtd_pic_amd import engine
+    spec = ph.default_spec(N=2000, steps=9000)
+    freqs = np.linspace(100e12, 300e12, 256)
+    f, R64, T64, _, p0, p1 = ph.spectra(spec, freqs)
+    d0 = _device(spec, "f32", with_slab=False)["probes"]
+    d1 = _device(spec, "f32")["probes"]
+    dt = spec["Sc"] * spec["dz"] / 299792458.0
+    X0 = engine.dft(d0, dt, freqs)
+    X1 = engine.dft(d1, dt, freqs)
+    R32 = np.abs(X1[0] - X0[0]) / np.abs(X0[0])
+    T32 = np.abs(X1[1]) / np.abs(X0[1])
+    band = np.abs(X0[0]) >= 0.01 * np.abs(X0[0]).max()
+    assert band.sum() > 20
+    eR = np.max(np.abs(R32 - R64)[band])
+    eT = np.max(np.abs(T32 - T64)[band])
+    print("spectra err", eR, eT, "R range", R64[band].min(), R64[band].max(), flush=True)
+    assert eR <= 1e-4 and eT <= 1e-4
