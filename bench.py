@@ -43,7 +43,10 @@ def parse():
 def cpu_baseline(p, target_s):
     """C restatement (FP64, OpenMP) on the first steps of the same workload."""
     from oracle import cpu_ref, physical as ph
-    cpu_ref.build_lib()
+    try:   # tune for the host that runs the baseline; fall back to the portable in-tree build
+        cpu_ref.use(cpu_ref.build_lib(path="/tmp/fdtd_cpu_native.so", native=True))
+    except Exception:
+        cpu_ref.use(cpu_ref.build_lib())
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or cpu_ref.lib().cpu_max_threads()
     m = ph.build(p)
     sim = cpu_ref.CpuSim(m)

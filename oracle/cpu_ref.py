@@ -31,23 +31,37 @@ class StateT(C.Structure):
     _fields_ = [(n, _dp) for n in ("E", "H", "psiE", "psiH", "D", "P", "Pm")]
 
 
-def build_lib(force=False):
+def build_lib(force=False, path=None, native=False):
+    """Compile fdtd_cpu.c.  The in-tree copy is portable (x86-64-v3); a
+    `native=True` copy tuned for the host that runs the baseline can be built
+    elsewhere (bench.py builds one in /tmp on the GPU box)."""
     src = os.path.join(HERE, "fdtd_cpu.c")
-    if force or not os.path.exists(LIB) or os.path.getmtime(LIB) < os.path.getmtime(src):
-        subprocess.check_call(["gcc", "-O3", "-march=native", "-ffp-contract=off", "-fopenmp",
-                               "-fPIC", "-shared", "-o", LIB, src, "-lm"])
-    return LIB
+    out = path or LIB
+    if force or not os.path.exists(out) or os.path.getmtime(out) < os.path.getmtime(src):
+        arch = "-march=native" if native else "-march=x86-64-v3"
+        subprocess.check_call(["gcc", "-O3", arch, "-ffp-contract=off", "-fopenmp",
+                               "-fPIC", "-shared", "-o", out + ".tmp", src, "-lm"])
+        os.replace(out + ".tmp", out)
+    return out
 
 
 _lib = None
 
 
-def lib():
+def use(path):
+    """Load the oracle library from `path` instead of the in-tree copy."""
+    global _lib
+    _lib = None
+    return lib(path)
+
+
+def lib(path=None):
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB):
+        p = path or LIB
+        if not os.path.exists(p):
             build_lib()
-        _lib = C.CDLL(LIB)
+        _lib = C.CDLL(p)
         _lib.cpu_phys_run.restype = C.c_double
         _lib.cpu_phys_run.argtypes = [C.POINTER(PhysT), C.POINTER(StateT), C.c_int, C.c_int, _dp, C.c_int]
         _lib.cpu_phys_run_many.restype = C.c_double

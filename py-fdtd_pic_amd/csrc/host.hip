@@ -15,7 +15,7 @@
 namespace fdtd {
 template <typename R, bool KAHAN>
 hipError_t launch_stencil(const SimDev<R>* sims, const int2* tiles, int ntiles, long long launch_idx,
-                          long long gstep0, int nsteps, int K, int S, hipStream_t st);
+                          long long gstep0, int nsteps, int K, int S, int kind, hipStream_t st);
 }
 
 using namespace fdtd;
@@ -133,7 +133,9 @@ static int build_sim(fdtd_ctx* ctx, SimHost& s, const fdtd_sim_desc_t* q) {
                 A[i] = (R)a; B[i] = (R)b; C[i] = (R)c;
             }
         }
-        put(11, q->sl_e, ns);
+        // the kernels multiply by 1/e (LINEAR: 1/eps0, KERR_E: 1/eps_inf)
+        R* ie = img.data() + L.off[11] / rs;
+        for (int i = 0; i < ns; ++i) ie[i] = (R)(1.0 / q->sl_e[i]);
     }
     put(12, q->srcE, T);
     put(13, q->srcH, T);
@@ -200,13 +202,14 @@ int fdtd_ctx_create(int precision, fdtd_ctx** out) {
 
 int fdtd_ctx_destroy(fdtd_ctx* c) {
     if (!c) return FDTD_OK;
-    hipStreamSynchronize(c->st);
+    // best effort: teardown errors are not actionable here
+    (void)hipStreamSynchronize(c->st);
     for (SimHost* s : c->sims)
-        if (s) { if (s->block) hipFree(s->block); delete s; }
-    if (c->d_sims) hipFree(c->d_sims);
-    if (c->d_tiles) hipFree(c->d_tiles);
-    for (auto& e : c->events) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
-    hipStreamDestroy(c->st);
+        if (s) { if (s->block) (void)hipFree(s->block); delete s; }
+    if (c->d_sims) (void)hipFree(c->d_sims);
+    if (c->d_tiles) (void)hipFree(c->d_tiles);
+    for (auto& e : c->events) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+    (void)hipStreamDestroy(c->st);
     delete c;
     return FDTD_OK;
 }
@@ -244,7 +247,7 @@ int fdtd_remove_sim(fdtd_ctx* c, int id) {
     if (!c || id < 0 || id >= (int)c->sims.size() || !c->sims[id]) return fdtd_fail(FDTD_E_ARG, "bad sim id");
     HIPCHK(hipStreamSynchronize(c->st));
     SimHost* s = c->sims[id];
-    if (s->block) hipFree(s->block);
+    if (s->block) (void)hipFree(s->block);
     c->dev_bytes -= s->bytes;
     delete s;
     c->sims[id] = nullptr;
@@ -278,7 +281,7 @@ template <typename R>
 static int rebuild_t(fdtd_ctx* c) {
     const int nsim = (int)c->sims.size();
     std::vector<SimDev<R>> devs(std::max(nsim, 1));
-    std::vector<int2> tiles;
+    std::vector<int2> by_kind[5];
     const int U = 64 * c->K - 2 * c->S;
     c->tile_sims.clear();
     for (int i = 0; i < nsim; ++i) {
@@ -287,7 +290,15 @@ static int rebuild_t(fdtd_ctx* c) {
         fill_dev<R>(*s, devs[i]);
         if (s->steps_done >= s->d.steps) continue;
         c->tile_sims.push_back(i);
-        for (int o = 0; o <= s->d.N; o += U) tiles.push_back(make_int2(i, o));
+        const int kind = s->d.slab_kind == FDTD_SLAB_NONE ? FDTD_SLAB_DIEL : s->d.slab_kind;
+        for (int o = 0; o <= s->d.N; o += U) by_kind[kind].push_back(make_int2(i, o));
+    }
+    std::vector<int2> tiles;
+    c->groups.clear();
+    for (int k = 0; k < 5; ++k) {
+        if (by_kind[k].empty()) continue;
+        c->groups.push_back({k, (int)tiles.size(), (int)by_kind[k].size()});
+        tiles.insert(tiles.end(), by_kind[k].begin(), by_kind[k].end());
     }
     const size_t need = sizeof(SimDev<R>) * devs.size();
     if (need > c->d_sims_cap) {
@@ -309,14 +320,16 @@ static int rebuild_t(fdtd_ctx* c) {
 }
 
 static int launch_once(fdtd_ctx* c, int n) {
-    hipError_t e;
-    if (c->prec == FDTD_F64)
-        e = launch_stencil<double, false>((const SimDev<double>*)c->d_sims, c->d_tiles, c->ntiles, c->launch_idx,
-                                          c->gstep, n, c->K, c->S, c->st);
-    else
-        e = launch_stencil<float, true>((const SimDev<float>*)c->d_sims, c->d_tiles, c->ntiles, c->launch_idx,
-                                        c->gstep, n, c->K, c->S, c->st);
-    if (e != hipSuccess) return fdtd_fail(FDTD_E_HIP, std::string("stencil launch: ") + hipGetErrorString(e));
+    for (const TileGroup& g : c->groups) {
+        hipError_t e;
+        if (c->prec == FDTD_F64)
+            e = launch_stencil<double, false>((const SimDev<double>*)c->d_sims, c->d_tiles + g.offset, g.count,
+                                              c->launch_idx, c->gstep, n, c->K, c->S, g.kind, c->st);
+        else
+            e = launch_stencil<float, true>((const SimDev<float>*)c->d_sims, c->d_tiles + g.offset, g.count,
+                                            c->launch_idx, c->gstep, n, c->K, c->S, g.kind, c->st);
+        if (e != hipSuccess) return fdtd_fail(FDTD_E_HIP, std::string("stencil launch: ") + hipGetErrorString(e));
+    }
     return FDTD_OK;
 }
 

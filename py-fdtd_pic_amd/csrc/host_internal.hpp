@@ -20,6 +20,10 @@ struct SimHost {
     long long launches = 0;     // launches this simulation took part in
 };
 
+struct TileGroup {
+    int kind, offset, count;    // tiles [offset, offset+count) run the kernel for this slab kind
+};
+
 struct fdtd_ctx {
     int prec = FDTD_F64;
     hipStream_t st = nullptr;
@@ -31,6 +35,7 @@ struct fdtd_ctx {
     size_t d_tiles_cap = 0;
     int ntiles = 0;
     std::vector<int> tile_sims;  // simulations covered by the current tile list
+    std::vector<TileGroup> groups;
     long long launch_idx = 0, gstep = 0;
     bool dirty = true;
     bool profile = false;
