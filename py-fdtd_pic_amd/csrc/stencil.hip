@@ -136,6 +136,237 @@ __device__ __forceinline__ void kerr_fixup(R (&E)[K], const double (&x)[K], cons
     }
 }
 
+// Register-resident state of one lane's K cells.  Kept as separate arrays
+// passed by reference (one struct of ~25 arrays is not scalar-replaced and
+// would live in scratch memory).
+#define TILE_PARAMS                                                                                      \
+    R(&E)[K], R(&H)[K], R(&cE)[K], R(&cH)[K], R(&ke)[K], R(&kh)[K], R(&D)[K], R(&P)[K], R(&Pm)[K],  \
+        R(&cD)[K], R(&cP)[K], R(&sA)[K], R(&sB)[K], R(&sC)[K], R(&ie)[K], double(&ied)[K],            \
+        double(&kz)[K], R(&psE)[K], R(&psH)[K], R(&bE)[K], R(&cEc)[K], R(&CbE)[K], R(&bH)[K],           \
+        R(&cHc)[K], R(&C2H)[K], const unsigned slab_mask, const unsigned interior_mask
+#define TILE_ARGS                                                                                       \
+    E, H, cE, cH, ke, kh, D, P, Pm, cD, cP, sA, sB, sC, ie, ied, kz, psE, psH, bE, cEc, CbE, bH, cHc, C2H, \
+        slab_mask, interior_mask
+
+// Uniform per-launch values of one tile.
+template <typename R>
+struct TileCtx {
+    int lane, t0, base, N, s_lo, s_hi, t_first, T;
+    long long gstep0, t_begin;
+    int src_lane, src_j, tf_lane, tf_j;
+    int npr, pr_rel[2], pr_t0[2], pr_t1[2], pr_id[2];
+    unsigned probe_bits;
+    int nprobe, snap_interval, snap_rows;
+    R kD, jD;
+    double chi3;
+    const R *srcE, *srcH, *Jc;
+    R *probe_out, *snap;
+};
+
+// What a tile's step loop has to handle (compile-time, one loop per class).
+enum : int {
+    M_PURE_SLAB = 1,   // every cell is a slab cell
+    M_MIXED = 2,       // slab and interior cells: selects per cell
+    M_STRIP = 4,       // CPML psi (coefficients are zero off the strips)
+    M_SRC = 8,         // soft source / TFSF cell in the tile
+    M_PROBE = 16,      // up to two probes in the tile's interior
+    M_GENERAL = 32,    // current J, snapshots, more than two probes
+};
+
+// Slab E half for all K cells of a pure-slab lane (straight-line: the K FP64
+// chains interleave).
+template <typename R, int K, bool KAHAN, int KIND>
+__device__ __forceinline__ void e_pure_slab(TILE_PARAMS, const TileCtx<R>& c, const SimDev<R>& sd,
+                                            const R (&dH)[K], const R (&Jv)[K]) {
+    double x[K], z[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        const double u = slab_u<R, KAHAN>(c.kD, c.jD, dH[j], Jv[j], E[j], D[j], cD[j], P[j], Pm[j],
+                                          cP[j], sA[j], sB[j], sC[j]);
+        if constexpr (KIND == SLAB_KERR_E) {
+            x[j] = u * ied[j];
+            z[j] = kz[j] * x[j] * x[j];
+            E[j] = (R)(x[j] * kerr_series<R>(z[j]));
+        } else {
+            E[j] = slab_e_simple<R, KIND>(sd, u, D[j], ie[j]);
+        }
+    }
+    if constexpr (KIND == SLAB_KERR_E) kerr_fixup<R, K>(E, x, z, ied, c.chi3);
+}
+
+// Mixed tile: every cell takes the interior update (+ CPML psi), slab cells
+// then take E from D - P through selects.
+template <typename R, int K, bool KAHAN, int KIND, bool STRIP>
+__device__ __forceinline__ void e_mixed(TILE_PARAMS, const TileCtx<R>& c, const SimDev<R>& sd,
+                                        const R (&dH)[K], const R (&Jv)[K]) {
+    R Ev[K], cEv[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        Ev[j] = E[j]; cEv[j] = cE[j];
+        const R inc = (dH[j] - Jv[j]) * ke[j];
+        if (KAHAN) kahan_add(Ev[j], cEv[j], inc); else Ev[j] = Ev[j] + inc;
+        if (STRIP) {
+            psE[j] = bE[j] * psE[j] + cEc[j] * dH[j];
+            Ev[j] = Ev[j] - CbE[j] * psE[j];
+        }
+    }
+    double x[K], z[K];
+    R Es[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        const bool sl = (slab_mask >> j) & 1u;
+        R Dn = D[j], cDn = cD[j], Pn = P[j], Pmn = Pm[j], cPn = cP[j];
+        const double u = slab_u<R, KAHAN>(c.kD, c.jD, dH[j], Jv[j], E[j], Dn, cDn, Pn, Pmn, cPn, sA[j],
+                                          sB[j], sC[j]);
+        if constexpr (KIND == SLAB_KERR_E) {
+            x[j] = u * ied[j];
+            z[j] = sl ? kz[j] * x[j] * x[j] : 0.0;
+            Es[j] = (R)(x[j] * kerr_series<R>(z[j]));
+        } else {
+            x[j] = z[j] = 0.0;
+            Es[j] = slab_e_simple<R, KIND>(sd, u, Dn, ie[j]);
+        }
+        D[j] = sl ? Dn : D[j]; cD[j] = sl ? cDn : cD[j];
+        P[j] = sl ? Pn : P[j]; Pm[j] = sl ? Pmn : Pm[j]; cP[j] = sl ? cPn : cP[j];
+    }
+    if constexpr (KIND == SLAB_KERR_E) kerr_fixup<R, K>(Es, x, z, ied, c.chi3);
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        const bool sl = (slab_mask >> j) & 1u;
+        E[j] = sl ? Es[j] : Ev[j];
+        cE[j] = sl ? cE[j] : cEv[j];
+    }
+}
+
+template <typename R, int K, bool KAHAN, int KIND, int MODE>
+__device__ __forceinline__ void run_steps(TILE_PARAMS, const TileCtx<R>& c, const SimDev<R>& sd) {
+    constexpr bool SLAB_CODE = (MODE & (M_PURE_SLAB | M_MIXED)) != 0 && KIND >= SLAB_LINEAR;
+    constexpr bool STRIP = (MODE & M_STRIP) != 0;
+    constexpr bool SRC = (MODE & M_SRC) != 0;
+    constexpr bool PROBE = (MODE & M_PROBE) != 0;
+    constexpr bool GEN = (MODE & M_GENERAL) != 0;
+    // source values of this launch, one step per lane (read back with
+    // v_readlane), and probe samples collected with lane selects and stored
+    // once after the loop: no memory access on the per-step path
+    // (chunk q of 64 steps sits in srcEc[0] while it runs; the arrays rotate
+    // at chunk ends so every index is a compile-time constant)
+    R srcEc[kChunks], srcHc[kChunks], pacc[2];
+    if constexpr (SRC) {
+#pragma unroll
+        for (int q = 0; q < kChunks; ++q) {
+            const int ix = q * 64 + c.lane;
+            const bool in = ix < (c.s_hi - c.s_lo);
+            srcEc[q] = (in && c.src_lane >= 0) ? c.srcE[c.t_first + ix] : R(0);
+            srcHc[q] = (in && c.tf_lane >= 0) ? c.srcH[c.t_first + ix] : R(0);
+        }
+    }
+    if constexpr (PROBE) {
+        pacc[0] = pacc[1] = R(0);
+    }
+    for (int s = c.s_lo; s < c.s_hi; ++s) {
+        const int ix = s - c.s_lo;
+        R Jv[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) Jv[j] = R(0);
+        if constexpr (GEN) {
+            if (c.Jc) {
+#pragma unroll
+                for (int j = 0; j < K; ++j) {
+                    const int n = c.base + j;
+                    if (n >= 0 && n <= c.N) Jv[j] = c.Jc[n];
+                }
+            }
+        }
+        // ---------------- E half ----------------
+        const R Hl = shr1(H[K - 1]);
+        R dH[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) dH[j] = H[j] - ((j == 0) ? Hl : H[j - 1]);
+        if constexpr (SLAB_CODE && (MODE & M_PURE_SLAB) && !STRIP) {
+            e_pure_slab<R, K, KAHAN, KIND>(TILE_ARGS, c, sd, dH, Jv);
+        } else if constexpr (SLAB_CODE) {
+            e_mixed<R, K, KAHAN, KIND, STRIP>(TILE_ARGS, c, sd, dH, Jv);
+        } else {
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                const R inc = (dH[j] - Jv[j]) * ke[j];
+                if (KAHAN) kahan_add(E[j], cE[j], inc); else E[j] = E[j] + inc;
+                if (STRIP) {
+                    psE[j] = bE[j] * psE[j] + cEc[j] * dH[j];
+                    E[j] = E[j] - CbE[j] * psE[j];
+                }
+            }
+        }
+        if constexpr (SRC) {
+            if (c.src_lane >= 0) {
+                const R sv = lane_read(srcEc[0], ix & 63);
+                if (c.lane == c.src_lane) add_at<R, K>(E, c.src_j, sv);
+            }
+            // ---------------- H half (TFSF correction first) ----------------
+            if (c.tf_lane >= 0) {
+                const R hv = lane_read(srcHc[0], ix & 63);
+                if (c.lane == c.tf_lane) add_at<R, K>(H, c.tf_j, -hv);
+            }
+        }
+        const R Er = shl1(E[0]);
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const R dE = ((j == K - 1) ? Er : E[j + 1]) - E[j];
+            const R inc = dE * kh[j];
+            if (KAHAN) kahan_add(H[j], cH[j], inc); else H[j] = H[j] + inc;
+            if (STRIP) {
+                psH[j] = bH[j] * psH[j] + cHc[j] * dE;
+                H[j] = H[j] + C2H[j] * psH[j];
+            }
+        }
+        // ---------------- probes / snapshots ----------------
+        if constexpr (PROBE && !GEN) {
+#pragma unroll
+            for (int k = 0; k < 2; ++k)
+                if (k < c.npr) {
+                    const R v = lane_read(pick<R, K>(E, c.pr_rel[k] % K), c.pr_rel[k] / K);
+                    pacc[k] = lane_write(v, ix & 63, pacc[k]);
+                }
+        }
+        // end of a 64-step chunk: flush probe samples, rotate source chunks
+        if constexpr (PROBE || SRC) {
+            if (((ix & 63) == 63) || s == c.s_hi - 1) {
+                if constexpr (PROBE && !GEN) {
+                    const int tt = c.t_first + (ix & ~63) + c.lane;
+#pragma unroll
+                    for (int k = 0; k < 2; ++k)
+                        if (k < c.npr && c.lane <= (ix & 63) && tt >= c.pr_t0[k] && tt <= c.pr_t1[k])
+                            c.probe_out[(long long)c.pr_id[k] * c.T + tt] = pacc[k];
+                }
+                if constexpr (SRC) {
+#pragma unroll
+                    for (int q = 0; q + 1 < kChunks; ++q) { srcEc[q] = srcEc[q + 1]; srcHc[q] = srcHc[q + 1]; }
+                }
+            }
+        }
+        if constexpr (GEN) {
+            const int t = c.t_first + ix;
+            if (c.probe_bits) {
+                for (int p = 0; p < c.nprobe; ++p) {
+                    if (!((c.probe_bits >> p) & 1u)) continue;
+                    if (t < sd.probe_t0[p] || t > sd.probe_t1[p]) continue;
+                    const int rel = sd.probe_cell[p] - c.t0;
+                    if (c.lane == rel / K) c.probe_out[(long long)p * c.T + t] = pick<R, K>(E, rel % K);
+                }
+            }
+            if (c.snap_interval > 0 && t > 0 && (t % c.snap_interval) == 0) {
+                const int row = t / c.snap_interval;
+                if (row < c.snap_rows) {
+#pragma unroll
+                    for (int j = 0; j < K; ++j)
+                        if ((interior_mask >> j) & 1u)
+                            c.snap[(long long)row * (c.N + 1) + c.base + j] = E[j];
+                }
+            }
+        }
+    }
+}
+
 template <typename R, int K, bool KAHAN, int KIND>
 __global__ __launch_bounds__(64) void stencil_tiles(const SimDev<R>* __restrict__ sims,
                                                     const int2* __restrict__ tiles, int ntiles,
@@ -151,35 +382,39 @@ __global__ __launch_bounds__(64) void stencil_tiles(const SimDev<R>* __restrict_
     const int lane = threadIdx.x;
     const int U = 64 * K - 2 * S;
     const int out0 = tl.y;
-    const int t0 = out0 - S;                 // first cell of the tile
-    const int base = t0 + lane * K;          // my first cell
     const int N = sd.N;
     const int par = (int)((launch_idx - sd.launch_begin) & 1);
     const int F = sd.F, Rr = sd.Rr, Ls = sd.Ls, Rs = sd.Rs;
+
     // Everything the step loop reads from the descriptor is copied into
-    // registers here: the loop's global stores (probes, snapshots) could
-    // alias the descriptor, so the compiler would otherwise reload it per step.
-    const double chi3 = sd.kerr_chi3;
-    const R kD = sd.kD, jD = sd.jD;
-    const long long t_begin = sd.t_begin;
-    const int T = sd.T;
-    const int nprobe = sd.nprobe, snap_interval = sd.snap_interval, snap_rows = sd.snap_rows;
-    const R* __restrict__ srcE = sd.srcE;
-    const R* __restrict__ srcH = sd.srcH;
-    const R* __restrict__ Jc = sd.J;
-    R* __restrict__ probe_out = sd.probe_out;
-    R* __restrict__ snap = sd.snap;
+    // registers: the loop's global stores (probes, snapshots) could alias the
+    // descriptor, so the compiler would otherwise reload it every step.
+    TileCtx<R> c;
+    c.lane = lane;
+    c.t0 = out0 - S;
+    c.base = c.t0 + lane * K;
+    c.N = N;
+    c.T = sd.T;
+    c.gstep0 = gstep0;
+    c.t_begin = sd.t_begin;
+    c.kD = sd.kD; c.jD = sd.jD; c.chi3 = sd.kerr_chi3;
+    c.nprobe = sd.nprobe; c.snap_interval = sd.snap_interval; c.snap_rows = sd.snap_rows;
+    c.srcE = sd.srcE; c.srcH = sd.srcH; c.Jc = sd.J; c.probe_out = sd.probe_out; c.snap = sd.snap;
+    // valid local steps of this launch: t = gstep0 + s - t_begin in [0, T)
+    c.s_lo = (int)max(0LL, c.t_begin - gstep0);
+    c.s_hi = (int)min((long long)nsteps, (long long)c.T + c.t_begin - gstep0);
+    c.t_first = (int)(gstep0 + c.s_lo - c.t_begin);
 
     R E[K], H[K], cE[K], cH[K], ke[K], kh[K];
     R D[K], P[K], Pm[K], cD[K], cP[K], sA[K], sB[K], sC[K], ie[K];
+    double ied[K], kz[K];
     R psE[K], psH[K], bE[K], cEc[K], CbE[K], bH[K], cHc[K], C2H[K];
     unsigned slab_mask = 0, strip_mask = 0, interior_mask = 0;
-
     const R* Ein = sd.E[par];
     const R* Hin = sd.H[par];
 #pragma unroll
     for (int j = 0; j < K; ++j) {
-        const int n = base + j;
+        const int n = c.base + j;
         const bool valid = (n >= 0) && (n <= N);
         E[j] = valid ? Ein[n] : R(0);
         H[j] = valid ? Hin[n] : R(0);
@@ -208,273 +443,61 @@ __global__ __launch_bounds__(64) void stencil_tiles(const SimDev<R>* __restrict_
             psE[j] = sd.psiE[par][i]; psH[j] = sd.psiH[par][i];
         }
         if (valid && n >= out0 && n < out0 + U) interior_mask |= 1u << j;
+        ied[j] = (double)ie[j];
+        kz[j] = c.chi3 * ied[j];
     }
-    double ied[K], kz[K];      // 1/e and chi3/e per cell, FP64
-#pragma unroll
-    for (int j = 0; j < K; ++j) { ied[j] = (double)ie[j]; kz[j] = chi3 * ied[j]; }
     constexpr unsigned full = (K >= 32) ? 0xffffffffu : ((1u << K) - 1u);
-    // tile classes (wave-uniform): pure slab / pure interior / mixed
+    // tile classes (wave-uniform)
     const bool tile_strip = __any(strip_mask != 0);
     const bool pure_slab = DFORM && __all(slab_mask == full);
     const bool any_slab = DFORM && __any(slab_mask != 0);
-    const int tile_end = t0 + 64 * K;
+    const int tile_end = c.t0 + 64 * K;
     const int src = sd.src_cell;
-    const bool tile_src = src >= t0 && src < tile_end;
-    const int src_lane = (src - t0) / K, src_j = (src - t0) % K;
-    const bool tile_tf = sd.tfsf && (src - 1) >= t0 && (src - 1) < tile_end;
-    const int tf_lane = (src - 1 - t0) / K, tf_j = (src - 1 - t0) % K;
-    // probes in this tile's interior: at most two kept in registers (cell
-    // offset, window); more than two fall back to the descriptor loop
-    unsigned probe_bits = 0;
-    for (int p = 0; p < nprobe; ++p) {
-        const int c = sd.probe_cell[p];
-        if (c >= out0 && c < out0 + U && c <= N) probe_bits |= 1u << p;
+    const bool tile_src = src >= c.t0 && src < tile_end;
+    c.src_lane = tile_src ? (src - c.t0) / K : -1;
+    c.src_j = tile_src ? (src - c.t0) % K : 0;
+    const bool tile_tf = sd.tfsf && (src - 1) >= c.t0 && (src - 1) < tile_end;
+    c.tf_lane = tile_tf ? (src - 1 - c.t0) / K : -1;
+    c.tf_j = tile_tf ? (src - 1 - c.t0) % K : 0;
+    c.probe_bits = 0;
+    for (int p = 0; p < c.nprobe; ++p) {
+        const int cc = sd.probe_cell[p];
+        if (cc >= out0 && cc < out0 + U && cc <= N) c.probe_bits |= 1u << p;
     }
-    int pr_rel[2] = {0, 0}, pr_t0[2] = {0, 0}, pr_t1[2] = {-1, -1}, pr_id[2] = {0, 0};
-    const int npr_local = __popc(probe_bits);
+    c.npr = __popc(c.probe_bits);
+    // first two probes of the tile (constant indices keep TileCtx in registers)
+    c.pr_rel[0] = c.pr_rel[1] = c.pr_id[0] = c.pr_id[1] = c.pr_t0[0] = c.pr_t0[1] = 0;
+    c.pr_t1[0] = c.pr_t1[1] = -1;
     {
-        int k = 0;
-        for (int p = 0; p < nprobe && k < 2; ++p)
-            if ((probe_bits >> p) & 1u) {
-                pr_rel[k] = sd.probe_cell[p] - t0; pr_t0[k] = sd.probe_t0[p]; pr_t1[k] = sd.probe_t1[p];
-                pr_id[k] = p; ++k;
-            }
-    }
-
-    // valid local steps of this launch: t = gstep0 + s - t_begin in [0, T)
-    const int s_lo = (int)max(0LL, t_begin - gstep0);
-    const int s_hi = (int)min((long long)nsteps, (long long)T + t_begin - gstep0);
-    // fast path: a plain tile (all slab or all interior, no CPML strip, source,
-    // probe, current or snapshot) runs a loop with no per-step uniform checks
-    const bool plain = (pure_slab || !any_slab) && !tile_strip && !tile_src && !tile_tf && npr_local == 0 &&
-                       !Jc && snap_interval == 0;
-    if (plain && pure_slab) {
-        for (int s = s_lo; s < s_hi; ++s) {
-            const R Hl = shr1(H[K - 1]);
-            double x[K], z[K];
-#pragma unroll
-            for (int j = 0; j < K; ++j) {
-                const R dH = H[j] - ((j == 0) ? Hl : H[j - 1]);
-                const double u = slab_u<R, KAHAN>(kD, jD, dH, R(0), E[j], D[j], cD[j], P[j], Pm[j], cP[j],
-                                                  sA[j], sB[j], sC[j]);
-                if constexpr (KIND == SLAB_KERR_E) {
-                    x[j] = u * ied[j];
-                    z[j] = kz[j] * x[j] * x[j];
-                    E[j] = (R)(x[j] * kerr_series<R>(z[j]));
-                } else {
-                    E[j] = slab_e_simple<R, KIND>(sd, u, D[j], ie[j]);
-                }
-            }
-            if constexpr (KIND == SLAB_KERR_E) kerr_fixup<R, K>(E, x, z, ied, chi3);
-            const R Er = shl1(E[0]);
-#pragma unroll
-            for (int j = 0; j < K; ++j) {
-                const R inc = (((j == K - 1) ? Er : E[j + 1]) - E[j]) * kh[j];
-                if (KAHAN) kahan_add(H[j], cH[j], inc); else H[j] = H[j] + inc;
-            }
-        }
-    } else if (plain) {
-        for (int s = s_lo; s < s_hi; ++s) {
-            const R Hl = shr1(H[K - 1]);
-#pragma unroll
-            for (int j = 0; j < K; ++j) {
-                const R inc = (H[j] - ((j == 0) ? Hl : H[j - 1])) * ke[j];
-                if (KAHAN) kahan_add(E[j], cE[j], inc); else E[j] = E[j] + inc;
-            }
-            const R Er = shl1(E[0]);
-#pragma unroll
-            for (int j = 0; j < K; ++j) {
-                const R inc = (((j == K - 1) ? Er : E[j + 1]) - E[j]) * kh[j];
-                if (KAHAN) kahan_add(H[j], cH[j], inc); else H[j] = H[j] + inc;
-            }
-        }
-    } else {
-        // source values of this launch, one step per lane (read back with
-        // v_readlane), and probe samples collected with v_writelane and
-        // stored once after the loop: no memory access on the per-step path
-        const int t_first = (int)(gstep0 + s_lo - t_begin);
-        R srcEc[kChunks], srcHc[kChunks], pacc[2][kChunks];
-#pragma unroll
-        for (int c = 0; c < kChunks; ++c) {
-            const int tt = t_first + c * 64 + lane;
-            const bool in = (c * 64 + lane) < (s_hi - s_lo);
-            srcEc[c] = (tile_src && in) ? srcE[tt] : R(0);
-            srcHc[c] = (tile_tf && in) ? srcH[tt] : R(0);
-            pacc[0][c] = pacc[1][c] = R(0);
-        }
-        for (int s = s_lo; s < s_hi; ++s) {
-            const int t = (int)(gstep0 + s - t_begin);
-            const int ix = s - s_lo;
-            R Jv[K];
-#pragma unroll
-            for (int j = 0; j < K; ++j) Jv[j] = R(0);
-            if (Jc) {
-#pragma unroll
-                for (int j = 0; j < K; ++j) {
-                    const int n = base + j;
-                    if (n >= 0 && n <= N) Jv[j] = Jc[n];
-                }
-            }
-            // ---------------- E half ----------------
-            const R Hl = shr1(H[K - 1]);
-            R dHs[K];
-#pragma unroll
-            for (int j = 0; j < K; ++j) dHs[j] = H[j] - ((j == 0) ? Hl : H[j - 1]);
-
-            if (pure_slab && !tile_strip) {
-                // straight-line over the K cells: the FP64 chains interleave
-                if constexpr (KIND == SLAB_KERR_E) {
-                    double x[K], z[K];
-#pragma unroll
-                    for (int j = 0; j < K; ++j) {
-                        const double u = slab_u<R, KAHAN>(kD, jD, dHs[j], Jv[j], E[j], D[j], cD[j], P[j], Pm[j], cP[j],
-                                                          sA[j], sB[j], sC[j]);
-                        x[j] = u * ied[j];
-                        z[j] = kz[j] * x[j] * x[j];
-                        E[j] = (R)(x[j] * kerr_series<R>(z[j]));
-                    }
-                    kerr_fixup<R, K>(E, x, z, ied, chi3);
-                } else {
-#pragma unroll
-                    for (int j = 0; j < K; ++j) {
-                        const double u = slab_u<R, KAHAN>(kD, jD, dHs[j], Jv[j], E[j], D[j], cD[j], P[j], Pm[j], cP[j],
-                                                          sA[j], sB[j], sC[j]);
-                        E[j] = slab_e_simple<R, KIND>(sd, u, D[j], ie[j]);
-                    }
-                }
-            } else if (!any_slab && !tile_strip) {
-#pragma unroll
-                for (int j = 0; j < K; ++j) {
-                    const R inc = (dHs[j] - Jv[j]) * ke[j];
-                    if (KAHAN) kahan_add(E[j], cE[j], inc); else E[j] = E[j] + inc;
-                }
-            } else {
-                // mixed tile: every cell takes the interior update (CPML psi
-                // coefficients are zero off the strips, so no per-cell test);
-                // slab cells then take E from D - P through selects, and the
-                // Kerr closed form hides behind one wave-level test.
-                R Ev[K], cEv[K];
-#pragma unroll
-                for (int j = 0; j < K; ++j) {
-                    Ev[j] = E[j]; cEv[j] = cE[j];
-                    const R inc = (dHs[j] - Jv[j]) * ke[j];
-                    if (KAHAN) kahan_add(Ev[j], cEv[j], inc); else Ev[j] = Ev[j] + inc;
-                }
-                if (tile_strip) {
-#pragma unroll
-                    for (int j = 0; j < K; ++j) {
-                        psE[j] = bE[j] * psE[j] + cEc[j] * dHs[j];
-                        Ev[j] = Ev[j] - CbE[j] * psE[j];
-                    }
-                }
-                if (DFORM && any_slab) {
-                    double x[K], z[K];
-                    R Es[K];
-#pragma unroll
-                    for (int j = 0; j < K; ++j) {
-                        const bool sl = (slab_mask >> j) & 1u;
-                        R Dn = D[j], cDn = cD[j], Pn = P[j], Pmn = Pm[j], cPn = cP[j];
-                        const double u = slab_u<R, KAHAN>(kD, jD, dHs[j], Jv[j], E[j], Dn, cDn, Pn, Pmn, cPn,
-                                                          sA[j], sB[j], sC[j]);
-                        if constexpr (KIND == SLAB_KERR_E) {
-                            x[j] = u * ied[j];
-                            z[j] = sl ? kz[j] * x[j] * x[j] : 0.0;
-                            Es[j] = (R)(x[j] * kerr_series<R>(z[j]));
-                        } else {
-                            x[j] = z[j] = 0.0;
-                            Es[j] = slab_e_simple<R, KIND>(sd, u, Dn, ie[j]);
-                        }
-                        D[j] = sl ? Dn : D[j]; cD[j] = sl ? cDn : cD[j];
-                        P[j] = sl ? Pn : P[j]; Pm[j] = sl ? Pmn : Pm[j]; cP[j] = sl ? cPn : cP[j];
-                    }
-                    if constexpr (KIND == SLAB_KERR_E) kerr_fixup<R, K>(Es, x, z, ied, chi3);
-#pragma unroll
-                    for (int j = 0; j < K; ++j) {
-                        const bool sl = (slab_mask >> j) & 1u;
-                        E[j] = sl ? Es[j] : Ev[j];
-                        cE[j] = sl ? cE[j] : cEv[j];
-                    }
-                } else {
-#pragma unroll
-                    for (int j = 0; j < K; ++j) { E[j] = Ev[j]; cE[j] = cEv[j]; }
-                }
-            }
-            if (tile_src) {
-                const R sv = lane_read(pick<R, kChunks>(srcEc, ix >> 6), ix & 63);
-                if (lane == src_lane) add_at<R, K>(E, src_j, sv);
-            }
-            // ---------------- H half ----------------
-            if (tile_tf) {
-                const R hv = lane_read(pick<R, kChunks>(srcHc, ix >> 6), ix & 63);
-                if (lane == tf_lane) add_at<R, K>(H, tf_j, -hv);
-            }
-            const R Er = shl1(E[0]);
-            if (!tile_strip) {
-#pragma unroll
-                for (int j = 0; j < K; ++j) {
-                    const R dE = ((j == K - 1) ? Er : E[j + 1]) - E[j];
-                    const R inc = dE * kh[j];
-                    if (KAHAN) kahan_add(H[j], cH[j], inc); else H[j] = H[j] + inc;
-                }
-            } else {
-#pragma unroll
-                for (int j = 0; j < K; ++j) {   // psi coefficients are zero off the strips
-                    const R dE = ((j == K - 1) ? Er : E[j + 1]) - E[j];
-                    const R inc = dE * kh[j];
-                    if (KAHAN) kahan_add(H[j], cH[j], inc); else H[j] = H[j] + inc;
-                    psH[j] = bH[j] * psH[j] + cHc[j] * dE;
-                    H[j] = H[j] + C2H[j] * psH[j];
-                }
-            }
-            // ---------------- probes / snapshots ----------------
-            if (npr_local) {
-#pragma unroll
-                for (int k = 0; k < 2; ++k)
-                    if (k < npr_local) {
-                        const R v = lane_read(pick<R, K>(E, pr_rel[k] % K), pr_rel[k] / K);
-#pragma unroll
-                        for (int c = 0; c < kChunks; ++c)
-                            if (c == (ix >> 6)) pacc[k][c] = lane_write(v, ix & 63, pacc[k][c]);
-                    }
-                if (npr_local > 2) {
-                    for (int p = 0; p < nprobe; ++p) {
-                        if (!((probe_bits >> p) & 1u) || p == pr_id[0] || p == pr_id[1]) continue;
-                        if (t < sd.probe_t0[p] || t > sd.probe_t1[p]) continue;
-                        const int rel = sd.probe_cell[p] - t0;
-                        if (lane == rel / K) probe_out[(long long)p * T + t] = pick<R, K>(E, rel % K);
-                    }
-                }
-            }
-            if (snap_interval > 0 && t > 0 && (t % snap_interval) == 0) {
-                const int row = t / snap_interval;
-                if (row < snap_rows) {
-#pragma unroll
-                    for (int j = 0; j < K; ++j)
-                        if ((interior_mask >> j) & 1u) snap[(long long)row * (N + 1) + base + j] = E[j];
-                }
-            }
-        }
-        if (npr_local) {
-#pragma unroll
-            for (int k = 0; k < 2; ++k)
-                if (k < npr_local) {
-#pragma unroll
-                    for (int c = 0; c < kChunks; ++c) {
-                        const int ix = c * 64 + lane;
-                        const int tt = t_first + ix;
-                        if (ix < s_hi - s_lo && tt >= pr_t0[k] && tt <= pr_t1[k])
-                            probe_out[(long long)pr_id[k] * T + tt] = pacc[k][c];
-                    }
-                }
+        bool have0 = false;
+        for (int p = 0; p < c.nprobe; ++p) {
+            if (!((c.probe_bits >> p) & 1u)) continue;
+            const int rel = sd.probe_cell[p] - c.t0, a = sd.probe_t0[p], z = sd.probe_t1[p];
+            if (!have0) { c.pr_rel[0] = rel; c.pr_t0[0] = a; c.pr_t1[0] = z; c.pr_id[0] = p; have0 = true; }
+            else { c.pr_rel[1] = rel; c.pr_t0[1] = a; c.pr_t1[1] = z; c.pr_id[1] = p; break; }
         }
     }
+
+    // one lean loop per tile class
+    const bool gen = c.Jc != nullptr || c.snap_interval > 0 || c.npr > 2;
+    const bool extras = tile_strip || tile_src || tile_tf || c.npr > 0;
+    if (gen)
+        run_steps<R, K, KAHAN, KIND, M_MIXED | M_STRIP | M_SRC | M_GENERAL>(TILE_ARGS, c, sd);
+    else if (pure_slab && !extras)
+        run_steps<R, K, KAHAN, KIND, M_PURE_SLAB>(TILE_ARGS, c, sd);
+    else if (!any_slab && !extras)
+        run_steps<R, K, KAHAN, KIND, 0>(TILE_ARGS, c, sd);
+    else if (!any_slab)
+        run_steps<R, K, KAHAN, KIND, M_STRIP | M_SRC | M_PROBE>(TILE_ARGS, c, sd);
+    else
+        run_steps<R, K, KAHAN, KIND, M_MIXED | M_STRIP | M_SRC | M_PROBE>(TILE_ARGS, c, sd);
 
     // ---------------- write back the interior ----------------
     const int po = par ^ 1;
 #pragma unroll
     for (int j = 0; j < K; ++j) {
         if (!((interior_mask >> j) & 1u)) continue;
-        const int n = base + j;
+        const int n = c.base + j;
         sd.E[po][n] = E[j];
         sd.H[po][n] = H[j];
         if (KAHAN) { sd.cE[po][n] = cE[j]; sd.cH[po][n] = cH[j]; }
