@@ -15,7 +15,8 @@
 namespace fdtd {
 template <typename R, bool KAHAN>
 hipError_t launch_stencil(const SimDev<R>* sims, const int2* tiles, int ntiles, long long launch_idx,
-                          long long gstep0, int nsteps, int K, int S, int kind, hipStream_t st);
+                          long long gstep0, int nsteps, int K, int S, int kind, hipStream_t st,
+                          hipEvent_t ev0, hipEvent_t ev1);
 }
 
 using namespace fdtd;
@@ -319,15 +320,21 @@ static int rebuild_t(fdtd_ctx* c) {
     return FDTD_OK;
 }
 
-static int launch_once(fdtd_ctx* c, int n) {
-    for (const TileGroup& g : c->groups) {
+// ev (profiling): ev->first is stamped at the start of the first group's
+// kernel, ev->second at the end of the last one.
+static int launch_once(fdtd_ctx* c, int n, std::pair<hipEvent_t, hipEvent_t>* ev) {
+    const size_t ng = c->groups.size();
+    for (size_t gi = 0; gi < ng; ++gi) {
+        const TileGroup& g = c->groups[gi];
+        hipEvent_t e0 = (ev && gi == 0) ? ev->first : nullptr;
+        hipEvent_t e1 = (ev && gi + 1 == ng) ? ev->second : nullptr;
         hipError_t e;
         if (c->prec == FDTD_F64)
             e = launch_stencil<double, false>((const SimDev<double>*)c->d_sims, c->d_tiles + g.offset, g.count,
-                                              c->launch_idx, c->gstep, n, c->K, c->S, g.kind, c->st);
+                                              c->launch_idx, c->gstep, n, c->K, c->S, g.kind, c->st, e0, e1);
         else
             e = launch_stencil<float, true>((const SimDev<float>*)c->d_sims, c->d_tiles + g.offset, g.count,
-                                            c->launch_idx, c->gstep, n, c->K, c->S, g.kind, c->st);
+                                            c->launch_idx, c->gstep, n, c->K, c->S, g.kind, c->st, e0, e1);
         if (e != hipSuccess) return fdtd_fail(FDTD_E_HIP, std::string("stencil launch: ") + hipGetErrorString(e));
     }
     return FDTD_OK;
@@ -356,11 +363,9 @@ int fdtd_run(fdtd_ctx* c, int64_t nsteps) {
                 c->events.emplace_back(a, b);
             }
             ev = &c->events[c->ev_used++];
-            HIPCHK(hipEventRecord(ev->first, c->st));
         }
-        int rc = launch_once(c, n);
+        int rc = launch_once(c, n, ev);
         if (rc) return rc;
-        if (ev) HIPCHK(hipEventRecord(ev->second, c->st));
         c->launches++;
         bool finished = false;
         for (int i : c->tile_sims) {

@@ -19,6 +19,8 @@
 //         AcubicFinder/NonLinExUpdate :803-877)
 //   H update + CPML psi_H (ADE_HyUpdate :640-656, CPML_Psi_m_Update :381-393)
 //   probes / snapshots (probeSim :16-54, vidMake :57-68)
+#include <hip/hip_ext.h>
+
 #include "device_common.hpp"
 
 namespace fdtd {
@@ -514,24 +516,33 @@ __global__ __launch_bounds__(64) void stencil_tiles(const SimDev<R>* __restrict_
 }
 
 // Host-side launcher (called from host.hip).
+// With profiling events the launch goes through hipExtLaunchKernelGGL, which
+// stamps ev0/ev1 from the kernel's own dispatch packet (kernel execution time
+// only, the figure a kernel trace reports).
 template <typename R, int K, bool KAHAN, int KIND>
 static hipError_t launch_k(const SimDev<R>* sims, const int2* tiles, int ntiles, long long launch_idx,
-                           long long gstep0, int nsteps, int S, hipStream_t st) {
+                           long long gstep0, int nsteps, int S, hipStream_t st, hipEvent_t ev0,
+                           hipEvent_t ev1) {
     const int per_xcd = (ntiles + 7) / 8;
     const int grid = per_xcd * 8;
-    hipLaunchKernelGGL((stencil_tiles<R, K, KAHAN, KIND>), dim3(grid), dim3(64), 0, st, sims, tiles, ntiles,
-                       per_xcd, launch_idx, gstep0, nsteps, S);
+    if (ev0 || ev1)
+        hipExtLaunchKernelGGL((stencil_tiles<R, K, KAHAN, KIND>), dim3(grid), dim3(64), 0, st, ev0, ev1, 0,
+                              sims, tiles, ntiles, per_xcd, launch_idx, gstep0, nsteps, S);
+    else
+        hipLaunchKernelGGL((stencil_tiles<R, K, KAHAN, KIND>), dim3(grid), dim3(64), 0, st, sims, tiles,
+                           ntiles, per_xcd, launch_idx, gstep0, nsteps, S);
     return hipGetLastError();
 }
 
 template <typename R, bool KAHAN, int KIND>
 static hipError_t launch_kind(const SimDev<R>* sims, const int2* tiles, int ntiles, long long launch_idx,
-                              long long gstep0, int nsteps, int K, int S, hipStream_t st) {
+                              long long gstep0, int nsteps, int K, int S, hipStream_t st, hipEvent_t ev0,
+                              hipEvent_t ev1) {
     switch (K) {
-        case 1: return launch_k<R, 1, KAHAN, KIND>(sims, tiles, ntiles, launch_idx, gstep0, nsteps, S, st);
-        case 2: return launch_k<R, 2, KAHAN, KIND>(sims, tiles, ntiles, launch_idx, gstep0, nsteps, S, st);
-        case 4: return launch_k<R, 4, KAHAN, KIND>(sims, tiles, ntiles, launch_idx, gstep0, nsteps, S, st);
-        case 8: return launch_k<R, 8, KAHAN, KIND>(sims, tiles, ntiles, launch_idx, gstep0, nsteps, S, st);
+        case 1: return launch_k<R, 1, KAHAN, KIND>(sims, tiles, ntiles, launch_idx, gstep0, nsteps, S, st, ev0, ev1);
+        case 2: return launch_k<R, 2, KAHAN, KIND>(sims, tiles, ntiles, launch_idx, gstep0, nsteps, S, st, ev0, ev1);
+        case 4: return launch_k<R, 4, KAHAN, KIND>(sims, tiles, ntiles, launch_idx, gstep0, nsteps, S, st, ev0, ev1);
+        case 8: return launch_k<R, 8, KAHAN, KIND>(sims, tiles, ntiles, launch_idx, gstep0, nsteps, S, st, ev0, ev1);
         default: return hipErrorInvalidValue;
     }
 }
@@ -540,23 +551,26 @@ static hipError_t launch_kind(const SimDev<R>* sims, const int2* tiles, int ntil
 // SLAB_KERR_REF (FP64 only), SLAB_KERR_E.
 template <typename R, bool KAHAN>
 hipError_t launch_stencil(const SimDev<R>* sims, const int2* tiles, int ntiles, long long launch_idx,
-                          long long gstep0, int nsteps, int K, int S, int kind, hipStream_t st) {
+                          long long gstep0, int nsteps, int K, int S, int kind, hipStream_t st,
+                          hipEvent_t ev0, hipEvent_t ev1) {
     switch (kind) {
         case SLAB_NONE:
-        case SLAB_DIEL: return launch_kind<R, KAHAN, SLAB_DIEL>(sims, tiles, ntiles, launch_idx, gstep0, nsteps, K, S, st);
-        case SLAB_LINEAR: return launch_kind<R, KAHAN, SLAB_LINEAR>(sims, tiles, ntiles, launch_idx, gstep0, nsteps, K, S, st);
-        case SLAB_KERR_E: return launch_kind<R, KAHAN, SLAB_KERR_E>(sims, tiles, ntiles, launch_idx, gstep0, nsteps, K, S, st);
+        case SLAB_DIEL: return launch_kind<R, KAHAN, SLAB_DIEL>(sims, tiles, ntiles, launch_idx, gstep0, nsteps, K, S, st, ev0, ev1);
+        case SLAB_LINEAR: return launch_kind<R, KAHAN, SLAB_LINEAR>(sims, tiles, ntiles, launch_idx, gstep0, nsteps, K, S, st, ev0, ev1);
+        case SLAB_KERR_E: return launch_kind<R, KAHAN, SLAB_KERR_E>(sims, tiles, ntiles, launch_idx, gstep0, nsteps, K, S, st, ev0, ev1);
         case SLAB_KERR_REF:
             if constexpr (sizeof(R) == 8)
-                return launch_kind<R, KAHAN, SLAB_KERR_REF>(sims, tiles, ntiles, launch_idx, gstep0, nsteps, K, S, st);
+                return launch_kind<R, KAHAN, SLAB_KERR_REF>(sims, tiles, ntiles, launch_idx, gstep0, nsteps, K, S, st, ev0, ev1);
             return hipErrorInvalidValue;
         default: return hipErrorInvalidValue;
     }
 }
 
 template hipError_t launch_stencil<double, false>(const SimDev<double>*, const int2*, int, long long,
-                                                  long long, int, int, int, int, hipStream_t);
+                                                  long long, int, int, int, int, hipStream_t, hipEvent_t,
+                                                  hipEvent_t);
 template hipError_t launch_stencil<float, true>(const SimDev<float>*, const int2*, int, long long,
-                                                long long, int, int, int, int, hipStream_t);
+                                                long long, int, int, int, int, hipStream_t, hipEvent_t,
+                                                hipEvent_t);
 
 }  // namespace fdtd
