@@ -133,6 +133,31 @@ int fdtd_run(fdtd_ctx *ctx, int64_t nsteps);
 int fdtd_run_all(fdtd_ctx *ctx);
 int fdtd_sync(fdtd_ctx *ctx);
 
+/* ---- particle beams (PIC) ---------------------------------------------- */
+/* The reference has no particle code: its current sink V.Jx
+ * (MasterController.py:159) enters the grid in ADE_ExUpdate
+ * (BaseFDTD11.py:663-669) as Ex += (dHy - Jx)*UpExMat, in units of J_x*dz.
+ * A beam is advanced every step by a relativistic Boris push with linear
+ * gather of E and the time-averaged H, and deposits J' = Z0*Jx by CIC at the
+ * half-step position; a context holding a beam runs one step per launch. */
+typedef struct {
+    int64_t n;          /* macro-particles */
+    double q, m;        /* charge (C) and mass (kg) of one physical particle */
+    double w;           /* macro-particle weight per unit transverse area (m^-2) */
+    double dt, dz;      /* time step (s) and cell size (m) of the simulation */
+    const double *x;    /* positions in cell units (E node k at x = k), in [2, N-2) */
+    const double *u;    /* gamma*v/c, n x 3 (ux, uy, uz) */
+    int32_t group;      /* cells per sort/deposit wave, 1..61 (0: picked from the density) */
+} fdtd_beam_desc_t;
+
+/* Attaches a beam to a simulation created with has_current = 1 that has not
+ * run a step yet.  Particles are stored sorted by (cell, position, u) so the
+ * run does not depend on the input order. */
+int pic_attach(fdtd_ctx *ctx, int sim_id, const fdtd_beam_desc_t *beam);
+/* Particle state in input order: x (n), u (n x 3), alive (n; may be NULL).
+ * Absorbed particles report alive = 0 and a position in cell 1 or N-2. */
+int pic_read(fdtd_ctx *ctx, int sim_id, double *x, double *u, int8_t *alive, int64_t n);
+
 /* ---- data movement (host arrays FP64) --------------------------------- */
 int fdtd_read_field(fdtd_ctx *ctx, int sim_id, int field, double *out, int64_t len);
 int fdtd_write_field(fdtd_ctx *ctx, int sim_id, int field, const double *in, int64_t len);
@@ -143,12 +168,12 @@ int fdtd_read_snapshots(fdtd_ctx *ctx, int sim_id, double *out, int64_t len);
 
 /* ---- diagnostics ------------------------------------------------------ */
 const char *fdtd_last_error(void);
-/* Device time (ms) spent in stencil launches since the last reset, measured
- * with HIP events on the context stream; also the launch count. */
+/* Device time (ms) spent in stencil launches since the last reset, from HIP
+ * events stamped by the kernels' own dispatch packets; also the launch count. */
 int fdtd_timing(fdtd_ctx *ctx, double *ms, int64_t *launches, int reset);
-/* Profiling: when on, every stencil launch is bracketed by HIP events on the
- * context stream so fdtd_timing reports the summed kernel durations and the
- * launch count (the average launch duration used for the roofline). */
+/* Profiling: when on, every stencil launch carries start/stop HIP events so
+ * fdtd_timing reports the summed kernel durations and the launch count (the
+ * average launch duration used for the roofline). */
 int fdtd_set_profile(fdtd_ctx *ctx, int on);
 /* Bytes of device memory the context currently holds. */
 int fdtd_device_bytes(fdtd_ctx *ctx, int64_t *bytes);

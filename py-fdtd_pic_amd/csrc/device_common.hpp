@@ -33,6 +33,28 @@ struct SimDev {
 
 enum SlabKind { SLAB_NONE = 0, SLAB_DIEL = 1, SLAB_LINEAR = 2, SLAB_KERR_REF = 3, SLAB_KERR_E = 4 };
 
+// Particle beam of one simulation.  Particles are kept sorted by cell
+// (stable), structure of arrays, ping-ponged per step like the fields;
+// start[c] is the first particle of cell c (start[N+1] = np).  Live
+// particles occupy cells [2, N-3]; cells 1 and N-2 park absorbed ones.
+constexpr int kPicGMax = 61;        // cells per sort/deposit wave (LDS tables of kPicGMax+3)
+template <typename R>
+struct BeamDev {
+    int32_t sim, N, np, G;
+    R a;                 // q dt / (2 m c)
+    R jscale;            // Z0 q w c: J' of one particle per unit u_x/gamma
+    R Sc;                // c dt / dz
+    int32_t* cell[2];
+    R *frac[2], *ux[2], *uy[2], *uz[2];
+    int32_t* id[2];
+    int32_t* start[2];   // N+2
+    // per-step records at the old (sorted) index, written by the push
+    int32_t* tcell;
+    R *tfrac, *tux, *tuy, *tuz, *ta0, *ta1;
+    uint8_t* tcode;      // (d+1) | (node offset + 1) << 2, d = new cell - old cell
+    int32_t* err;        // set when an index check fails (never expected)
+};
+
 // ---- cross-lane shifts inside one wavefront (DPP, GFX9 wave_shr/wave_shl) ----
 // shr1: lane i receives lane i-1 (lane 0 receives 0); shl1: lane i receives lane i+1.
 __device__ __forceinline__ int dpp_shr1_i(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false); }

@@ -17,6 +17,9 @@ template <typename R, bool KAHAN>
 hipError_t launch_stencil(const SimDev<R>* sims, const int2* tiles, int ntiles, long long launch_idx,
                           long long gstep0, int nsteps, int K, int S, int kind, hipStream_t st,
                           hipEvent_t ev0, hipEvent_t ev1);
+template <typename R>
+hipError_t launch_pic(const SimDev<R>* sims, const BeamDev<R>* beams, const int2* push_blocks, int npush,
+                      const int2* items, int nitems, long long launch_idx, hipStream_t st);
 }
 
 using namespace fdtd;
@@ -50,6 +53,9 @@ struct Layout {
         return o;
     }
 };
+
+template <typename R>
+static int upload_beam(fdtd_ctx* c, BeamHost& b, int N);
 
 template <typename R>
 static void fill_dev(SimHost& s, SimDev<R>& d) {
@@ -197,6 +203,11 @@ int fdtd_ctx_create(int precision, fdtd_ctx** out) {
         delete c;
         return fdtd_fail(FDTD_E_HIP, "hipStreamCreate failed");
     }
+    if (hipMalloc(&c->d_err, sizeof(int32_t)) != hipSuccess || hipMemset(c->d_err, 0, sizeof(int32_t)) != hipSuccess) {
+        (void)hipStreamDestroy(c->st);
+        delete c;
+        return fdtd_fail(FDTD_E_HIP, "allocation of the error flag failed");
+    }
     *out = c;
     return FDTD_OK;
 }
@@ -207,6 +218,10 @@ int fdtd_ctx_destroy(fdtd_ctx* c) {
     (void)hipStreamSynchronize(c->st);
     for (SimHost* s : c->sims)
         if (s) { if (s->block) (void)hipFree(s->block); delete s; }
+    for (BeamHost* b : c->beams)
+        if (b) { if (b->block) (void)hipFree(b->block); delete b; }
+    for (void* p : {(void*)c->d_beams, (void*)c->d_pblocks, (void*)c->d_pitems, (void*)c->d_err})
+        if (p) (void)hipFree(p);
     if (c->d_sims) (void)hipFree(c->d_sims);
     if (c->d_tiles) (void)hipFree(c->d_tiles);
     for (auto& e : c->events) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
@@ -252,6 +267,13 @@ int fdtd_remove_sim(fdtd_ctx* c, int id) {
     c->dev_bytes -= s->bytes;
     delete s;
     c->sims[id] = nullptr;
+    if (id < (int)c->beams.size() && c->beams[id]) {
+        BeamHost* b = c->beams[id];
+        if (b->block) (void)hipFree(b->block);
+        c->dev_bytes -= b->bytes;
+        delete b;
+        c->beams[id] = nullptr;
+    }
     c->dirty = true;
     return FDTD_OK;
 }
@@ -261,6 +283,11 @@ int fdtd_reset_sim(fdtd_ctx* c, int id) {
     SimHost* s = c->sims[id];
     const size_t state0 = s->lay_off[14];   // first state sub-array (after coefficients and sources)
     HIPCHK(hipMemsetAsync((char*)s->block + state0, 0, s->bytes - state0, c->st));
+    if (id < (int)c->beams.size() && c->beams[id]) {
+        int rc = c->prec == FDTD_F64 ? upload_beam<double>(c, *c->beams[id], s->d.N)
+                                     : upload_beam<float>(c, *c->beams[id], s->d.N);
+        if (rc) return rc;
+    }
     s->steps_done = 0;
     s->launches = 0;
     s->launch_begin = c->launch_idx;
@@ -277,13 +304,165 @@ int fdtd_sim_progress(fdtd_ctx* c, int id, int64_t* done) {
 
 }  // extern "C"
 
+// Grow-only device buffer, then an async upload on the context stream.
+static int upload_grow(fdtd_ctx* c, void** buf, size_t* cap, const void* src, size_t bytes) {
+    if (bytes > *cap) {
+        if (*buf) HIPCHK(hipFree(*buf));
+        HIPCHK(hipMalloc(buf, bytes * 2));
+        *cap = bytes * 2;
+    }
+    if (bytes) HIPCHK(hipMemcpyAsync(*buf, src, bytes, hipMemcpyHostToDevice, c->st));
+    return FDTD_OK;
+}
+
+// ------------------------------------------------------------------------
+// particle beams
+// ------------------------------------------------------------------------
+enum { BO_CELL = 0, BO_FRAC = 2, BO_UX = 4, BO_UY = 6, BO_UZ = 8, BO_ID = 10, BO_START = 12,
+       BO_TCELL = 14, BO_TFRAC, BO_TUX, BO_TUY, BO_TUZ, BO_TA0, BO_TA1, BO_TCODE, BO_END };
+
+template <typename R>
+static BeamDev<R> fill_beam_dev(BeamHost& b, int32_t* err) {
+    BeamDev<R> d;
+    std::memset(&d, 0, sizeof(d));
+    char* blk = (char*)b.block;
+    d.sim = b.sim; d.N = b.N; d.np = (int32_t)b.np; d.G = b.G;
+    d.a = (R)b.a; d.jscale = (R)b.jscale; d.Sc = (R)b.Sc;
+    for (int k = 0; k < 2; ++k) {
+        d.cell[k] = (int32_t*)(blk + b.off[BO_CELL + k]);
+        d.frac[k] = (R*)(blk + b.off[BO_FRAC + k]);
+        d.ux[k] = (R*)(blk + b.off[BO_UX + k]);
+        d.uy[k] = (R*)(blk + b.off[BO_UY + k]);
+        d.uz[k] = (R*)(blk + b.off[BO_UZ + k]);
+        d.id[k] = (int32_t*)(blk + b.off[BO_ID + k]);
+        d.start[k] = (int32_t*)(blk + b.off[BO_START + k]);
+    }
+    d.tcell = (int32_t*)(blk + b.off[BO_TCELL]);
+    d.tfrac = (R*)(blk + b.off[BO_TFRAC]);
+    d.tux = (R*)(blk + b.off[BO_TUX]);
+    d.tuy = (R*)(blk + b.off[BO_TUY]);
+    d.tuz = (R*)(blk + b.off[BO_TUZ]);
+    d.ta0 = (R*)(blk + b.off[BO_TA0]);
+    d.ta1 = (R*)(blk + b.off[BO_TA1]);
+    d.tcode = (uint8_t*)(blk + b.off[BO_TCODE]);
+    d.err = err;
+    return d;
+}
+
+// Upload the initial (sorted) particle state into buffer 0 and start[] into both.
+template <typename R>
+static int upload_beam(fdtd_ctx* c, BeamHost& b, int N) {
+    const int64_t n = b.np;
+    char* blk = (char*)b.block;
+    std::vector<R> fr(n), ux(n), uy(n), uz(n);
+    for (int64_t i = 0; i < n; ++i) {
+        fr[i] = (R)b.frac0[i];
+        ux[i] = (R)b.u0[3 * i]; uy[i] = (R)b.u0[3 * i + 1]; uz[i] = (R)b.u0[3 * i + 2];
+    }
+    const size_t rn = sizeof(R) * n, in = sizeof(int32_t) * n, sn = sizeof(int32_t) * (N + 2);
+    HIPCHK(hipMemcpyAsync(blk + b.off[BO_CELL], b.cell0.data(), in, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(blk + b.off[BO_FRAC], fr.data(), rn, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(blk + b.off[BO_UX], ux.data(), rn, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(blk + b.off[BO_UY], uy.data(), rn, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(blk + b.off[BO_UZ], uz.data(), rn, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(blk + b.off[BO_ID], b.id0.data(), in, hipMemcpyHostToDevice, c->st));
+    for (int k = 0; k < 2; ++k)
+        HIPCHK(hipMemcpyAsync(blk + b.off[BO_START + k], b.start0.data(), sn, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));   // the host vectors above go out of scope
+    return FDTD_OK;
+}
+
+template <typename R>
+static int attach_t(fdtd_ctx* c, int id, SimHost& s, const fdtd_beam_desc_t* bd) {
+    const int N = s.d.N;
+    const int64_t n = bd->n;
+    BeamHost* b = new BeamHost();
+    b->sim = id;
+    b->N = N;
+    b->np = n;
+    b->q = bd->q; b->m = bd->m; b->w = bd->w;
+    // normalised coefficients (E in V/m, H' = Z0 H, u = gamma v / c)
+    const double c0 = 299792458.0, z0 = 376.730313668;
+    b->Sc = c0 * bd->dt / bd->dz;
+    b->a = bd->q * bd->dt / (2.0 * bd->m * c0);
+    b->jscale = z0 * bd->q * bd->w * c0;
+    // sort by (cell, frac, ux, uy, uz, input index): a total order of the input
+    std::vector<int64_t> ord(n);
+    std::vector<int32_t> cl(n);
+    std::vector<double> fr(n);
+    for (int64_t i = 0; i < n; ++i) {
+        ord[i] = i;
+        const double x = bd->x[i];
+        const double fl = std::floor(x);
+        cl[i] = (int32_t)fl;
+        fr[i] = x - fl;
+    }
+    const double* u = bd->u;
+    std::sort(ord.begin(), ord.end(), [&](int64_t p, int64_t q) {
+        if (cl[p] != cl[q]) return cl[p] < cl[q];
+        if (fr[p] != fr[q]) return fr[p] < fr[q];
+        for (int k = 0; k < 3; ++k)
+            if (u[3 * p + k] != u[3 * q + k]) return u[3 * p + k] < u[3 * q + k];
+        return p < q;
+    });
+    b->cell0.resize(n); b->id0.resize(n); b->frac0.resize(n); b->u0.resize(3 * n);
+    b->start0.assign(N + 2, 0);
+    std::vector<int64_t> occ(N + 2, 0);
+    for (int64_t k = 0; k < n; ++k) {
+        const int64_t i = ord[k];
+        b->cell0[k] = cl[i]; b->frac0[k] = fr[i]; b->id0[k] = (int32_t)i;
+        for (int j = 0; j < 3; ++j) b->u0[3 * k + j] = u[3 * i + j];
+        occ[cl[i]]++;
+    }
+    int64_t acc = 0, occupied = 0;
+    for (int cc = 0; cc <= N + 1; ++cc) { b->start0[cc] = (int32_t)acc; acc += occ[cc]; occupied += occ[cc] > 0; }
+    // cells per sort/deposit wave: about 256 particles per wave
+    int G = bd->group;
+    if (G <= 0) {
+        const double per_cell = occupied ? (double)n / occupied : 1.0;
+        G = (int)std::lround(256.0 / per_cell);
+    }
+    b->G = std::min(std::max(G, 1), kPicGMax);
+    // device block
+    Layout L;
+    const size_t rn = sizeof(R) * n, in = sizeof(int32_t) * n;
+    for (int k = 0; k < 2; ++k) L.add(in);                       // cell
+    for (int k = 0; k < 2; ++k) L.add(rn);                       // frac
+    for (int k = 0; k < 6; ++k) L.add(rn);                       // ux uy uz (x2, interleaved by k)
+    for (int k = 0; k < 2; ++k) L.add(in);                       // id
+    for (int k = 0; k < 2; ++k) L.add(sizeof(int32_t) * (N + 2)); // start
+    L.add(in);                                                   // tcell
+    for (int k = 0; k < 6; ++k) L.add(rn);                       // tfrac tux tuy tuz ta0 ta1
+    L.add(n);                                                    // tcode
+    for (int k = 0; k < L.n; ++k) b->off[k] = L.off[k];
+    // ux/uy/uz were added as ux0 ux1 uy0 uy1 uz0 uz1: matches BO_UX/BO_UY/BO_UZ + k
+    b->bytes = L.total;
+    if (hipMalloc(&b->block, b->bytes) != hipSuccess) {
+        delete b;
+        return fdtd_fail(FDTD_E_NOMEM, "hipMalloc of " + std::to_string(L.total) + " bytes (beam) failed");
+    }
+    if (c->beams.size() < c->sims.size()) c->beams.resize(c->sims.size(), nullptr);
+    c->beams[id] = b;
+    c->dev_bytes += b->bytes;
+    int rc = upload_beam<R>(c, *b, N);
+    if (rc) return rc;
+    c->dirty = true;
+    return FDTD_OK;
+}
+
 // Rebuild the device SimDev array and the tile list for active, unfinished sims.
 template <typename R>
 static int rebuild_t(fdtd_ctx* c) {
     const int nsim = (int)c->sims.size();
     std::vector<SimDev<R>> devs(std::max(nsim, 1));
     std::vector<int2> by_kind[5];
-    const int U = 64 * c->K - 2 * c->S;
+    // a coupled beam forces one step per launch: tiles then carry a 1-cell halo
+    bool pic = false;
+    for (int i = 0; i < nsim; ++i)
+        if (c->sims[i] && c->sims[i]->steps_done < c->sims[i]->d.steps && i < (int)c->beams.size() && c->beams[i])
+            pic = true;
+    c->S_run = pic ? 1 : c->S;
+    const int U = 64 * c->K - 2 * c->S_run;
     c->tile_sims.clear();
     for (int i = 0; i < nsim; ++i) {
         SimHost* s = c->sims[i];
@@ -293,6 +472,26 @@ static int rebuild_t(fdtd_ctx* c) {
         c->tile_sims.push_back(i);
         const int kind = s->d.slab_kind == FDTD_SLAB_NONE ? FDTD_SLAB_DIEL : s->d.slab_kind;
         for (int o = 0; o <= s->d.N; o += U) by_kind[kind].push_back(make_int2(i, o));
+    }
+    // particle beams of the active simulations
+    std::vector<BeamDev<R>> bdev;
+    std::vector<int2> pblocks, pitems;
+    for (int i : c->tile_sims) {
+        BeamHost* bh = i < (int)c->beams.size() ? c->beams[i] : nullptr;
+        if (!bh) continue;
+        const int bi = (int)bdev.size();
+        bdev.push_back(fill_beam_dev<R>(*bh, c->d_err));
+        for (int64_t p = 0; p < bh->np; p += 256) pblocks.push_back(make_int2(bi, (int)p));
+        for (int c0 = 0; c0 <= c->sims[i]->d.N; c0 += bh->G) pitems.push_back(make_int2(bi, c0));
+    }
+    c->has_pic = !bdev.empty();
+    c->npblocks = (int)pblocks.size();
+    c->npitems = (int)pitems.size();
+    if (c->has_pic) {
+        int rc = upload_grow(c, &c->d_beams, &c->d_beams_cap, bdev.data(), sizeof(BeamDev<R>) * bdev.size());
+        if (!rc) rc = upload_grow(c, (void**)&c->d_pblocks, &c->d_pblocks_cap, pblocks.data(), sizeof(int2) * pblocks.size());
+        if (!rc) rc = upload_grow(c, (void**)&c->d_pitems, &c->d_pitems_cap, pitems.data(), sizeof(int2) * pitems.size());
+        if (rc) return rc;
     }
     std::vector<int2> tiles;
     c->groups.clear();
@@ -331,10 +530,10 @@ static int launch_once(fdtd_ctx* c, int n, std::pair<hipEvent_t, hipEvent_t>* ev
         hipError_t e;
         if (c->prec == FDTD_F64)
             e = launch_stencil<double, false>((const SimDev<double>*)c->d_sims, c->d_tiles + g.offset, g.count,
-                                              c->launch_idx, c->gstep, n, c->K, c->S, g.kind, c->st, e0, e1);
+                                              c->launch_idx, c->gstep, n, c->K, c->S_run, g.kind, c->st, e0, e1);
         else
             e = launch_stencil<float, true>((const SimDev<float>*)c->d_sims, c->d_tiles + g.offset, g.count,
-                                            c->launch_idx, c->gstep, n, c->K, c->S, g.kind, c->st, e0, e1);
+                                            c->launch_idx, c->gstep, n, c->K, c->S_run, g.kind, c->st, e0, e1);
         if (e != hipSuccess) return fdtd_fail(FDTD_E_HIP, std::string("stencil launch: ") + hipGetErrorString(e));
     }
     return FDTD_OK;
@@ -352,8 +551,17 @@ int fdtd_run(fdtd_ctx* c, int64_t nsteps) {
             if (rc) return rc;
         }
         if (c->ntiles == 0) break;
-        // steps this launch: S, bounded by the remaining request and by PIC coupling
-        int n = (int)std::min<int64_t>(remaining, c->S);
+        // steps this launch: S, bounded by the remaining request; one step when
+        // a beam is coupled (the deposit feeds every E update)
+        int n = (int)std::min<int64_t>(remaining, c->S_run);
+        if (c->has_pic) {
+            hipError_t e = c->prec == FDTD_F64
+                ? launch_pic<double>((const SimDev<double>*)c->d_sims, (const BeamDev<double>*)c->d_beams,
+                                     c->d_pblocks, c->npblocks, c->d_pitems, c->npitems, c->launch_idx, c->st)
+                : launch_pic<float>((const SimDev<float>*)c->d_sims, (const BeamDev<float>*)c->d_beams,
+                                    c->d_pblocks, c->npblocks, c->d_pitems, c->npitems, c->launch_idx, c->st);
+            if (e != hipSuccess) return fdtd_fail(FDTD_E_HIP, std::string("pic launch: ") + hipGetErrorString(e));
+        }
         std::pair<hipEvent_t, hipEvent_t>* ev = nullptr;
         if (c->profile) {
             if (c->ev_used == c->events.size()) {
@@ -391,10 +599,18 @@ int fdtd_run_all(fdtd_ctx* c) {
     return fdtd_run(c, most);
 }
 
+static int check_pic_flag(fdtd_ctx* c) {
+    if (!c->d_err) return FDTD_OK;
+    int32_t e = 0;
+    HIPCHK(hipMemcpy(&e, c->d_err, sizeof(e), hipMemcpyDeviceToHost));
+    if (e) return fdtd_fail(FDTD_E_STATE, "particle sort: an index check failed (inconsistent cell starts)");
+    return FDTD_OK;
+}
+
 int fdtd_sync(fdtd_ctx* c) {
     if (!c) return fdtd_fail(FDTD_E_ARG, "null ctx");
     HIPCHK(hipStreamSynchronize(c->st));
-    return FDTD_OK;
+    return c->has_pic ? check_pic_flag(c) : FDTD_OK;
 }
 
 int fdtd_timing(fdtd_ctx* c, double* ms, int64_t* launches, int reset) {
@@ -548,6 +764,72 @@ int fdtd_read_snapshots(fdtd_ctx* c, int id, double* out, int64_t len) {
     if (len != n) return fdtd_fail(FDTD_E_ARG, "length mismatch: expected " + std::to_string(n));
     if (c->prec == FDTD_F64) { SimDev<double> d; fill_dev<double>(*s, d); return read_plain<double>(c, d.snap, n, out); }
     SimDev<float> d; fill_dev<float>(*s, d); return read_plain<float>(c, d.snap, n, out);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------
+// particle beams: C ABI
+// ------------------------------------------------------------------------
+template <typename R>
+static int read_beam(fdtd_ctx* c, SimHost& s, BeamHost& b, double* x, double* u, int8_t* alive) {
+    const int par = (int)(s.launches & 1);
+    const int64_t n = b.np;
+    const int N = s.d.N;
+    std::vector<int32_t> cl(n), id(n);
+    std::vector<R> fr(n), ux(n), uy(n), uz(n);
+    HIPCHK(hipStreamSynchronize(c->st));
+    int rc = check_pic_flag(c);
+    if (rc) return rc;
+    const char* blk = (const char*)b.block;
+    const size_t rn = sizeof(R) * n, in = sizeof(int32_t) * n;
+    HIPCHK(hipMemcpy(cl.data(), blk + b.off[BO_CELL + par], in, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(id.data(), blk + b.off[BO_ID + par], in, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(fr.data(), blk + b.off[BO_FRAC + par], rn, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(ux.data(), blk + b.off[BO_UX + par], rn, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(uy.data(), blk + b.off[BO_UY + par], rn, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(uz.data(), blk + b.off[BO_UZ + par], rn, hipMemcpyDeviceToHost));
+    for (int64_t k = 0; k < n; ++k) {
+        const int64_t i = id[k];
+        if (i < 0 || i >= n) return fdtd_fail(FDTD_E_STATE, "particle id out of range");
+        x[i] = (double)cl[k] + (double)fr[k];
+        u[3 * i] = (double)ux[k]; u[3 * i + 1] = (double)uy[k]; u[3 * i + 2] = (double)uz[k];
+        if (alive) alive[i] = (int8_t)(cl[k] >= 2 && cl[k] <= N - 3);
+    }
+    return FDTD_OK;
+}
+
+extern "C" {
+
+int pic_attach(fdtd_ctx* c, int id, const fdtd_beam_desc_t* bd) {
+    SimHost* s = get_sim(c, id);
+    if (!s || !bd) return fdtd_fail(FDTD_E_ARG, "bad sim id or null beam");
+    if (!s->d.has_current) return fdtd_fail(FDTD_E_ARG, "the simulation was created without a current array");
+    if (s->steps_done > 0 || s->launches > 0) return fdtd_fail(FDTD_E_STATE, "the simulation has already run");
+    if (id < (int)c->beams.size() && c->beams[id]) return fdtd_fail(FDTD_E_STATE, "a beam is already attached");
+    if (bd->n < 1 || bd->n > (int64_t)INT32_MAX / 2) return fdtd_fail(FDTD_E_ARG, "particle count out of range");
+    if (!bd->x || !bd->u) return fdtd_fail(FDTD_E_ARG, "null particle arrays");
+    if (!(bd->m > 0) || !(bd->dt > 0) || !(bd->dz > 0)) return fdtd_fail(FDTD_E_ARG, "m, dt and dz must be > 0");
+    if (!(299792458.0 * bd->dt / bd->dz < 1.0)) return fdtd_fail(FDTD_E_ARG, "c dt / dz must be < 1");
+    if (bd->group < 0 || bd->group > kPicGMax) return fdtd_fail(FDTD_E_ARG, "group must be 0..61");
+    const int N = s->d.N;
+    if (N < 8) return fdtd_fail(FDTD_E_ARG, "grid too small for particles");
+    for (int64_t i = 0; i < bd->n; ++i) {
+        const double x = bd->x[i];
+        if (!(x >= 2.0 && x < (double)(N - 2))) return fdtd_fail(FDTD_E_ARG, "particle position outside [2, N-2)");
+        for (int k = 0; k < 3; ++k)
+            if (!std::isfinite(bd->u[3 * i + k])) return fdtd_fail(FDTD_E_ARG, "non-finite particle momentum");
+    }
+    return c->prec == FDTD_F64 ? attach_t<double>(c, id, *s, bd) : attach_t<float>(c, id, *s, bd);
+}
+
+int pic_read(fdtd_ctx* c, int id, double* x, double* u, int8_t* alive, int64_t n) {
+    SimHost* s = get_sim(c, id);
+    if (!s || !x || !u) return fdtd_fail(FDTD_E_ARG, "bad sim id or null output");
+    BeamHost* b = id < (int)c->beams.size() ? c->beams[id] : nullptr;
+    if (!b) return fdtd_fail(FDTD_E_ARG, "no beam attached");
+    if (n != b->np) return fdtd_fail(FDTD_E_ARG, "length mismatch: expected " + std::to_string(b->np));
+    return c->prec == FDTD_F64 ? read_beam<double>(c, *s, *b, x, u, alive) : read_beam<float>(c, *s, *b, x, u, alive);
 }
 
 }  // extern "C"

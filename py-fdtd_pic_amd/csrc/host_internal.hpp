@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <cstdint>
 #include <utility>
 #include <vector>
 
@@ -20,6 +21,21 @@ struct SimHost {
     long long launches = 0;     // launches this simulation took part in
 };
 
+// One particle beam attached to a simulation (at most one per simulation).
+struct BeamHost {
+    int sim = -1;
+    int N = 0;                   // grid size of the simulation
+    int64_t np = 0;
+    int G = 8;                   // cells per sort/deposit wave
+    double q = 0, m = 0, w = 0;  // charge, mass, weight per unit transverse area
+    double a = 0, jscale = 0, Sc = 0;
+    void* block = nullptr;
+    size_t bytes = 0;
+    size_t off[32] = {0};
+    std::vector<int32_t> cell0, id0, start0;   // initial sorted state (kept for reset)
+    std::vector<double> frac0, u0;             // frac, then ux uy uz interleaved
+};
+
 struct TileGroup {
     int kind, offset, count;    // tiles [offset, offset+count) run the kernel for this slab kind
 };
@@ -28,6 +44,7 @@ struct fdtd_ctx {
     int prec = FDTD_F64;
     hipStream_t st = nullptr;
     int K = 2, S = 16;
+    int S_run = 16;             // steps per launch of the current tile list (1 with a beam)
     std::vector<SimHost*> sims;
     void* d_sims = nullptr;
     size_t d_sims_cap = 0;
@@ -43,6 +60,17 @@ struct fdtd_ctx {
     size_t ev_used = 0;
     long long launches = 0;
     long long dev_bytes = 0;
+    // particle beams
+    std::vector<BeamHost*> beams;            // indexed like sims (nullptr: no beam)
+    void* d_beams = nullptr;
+    size_t d_beams_cap = 0;
+    int2* d_pblocks = nullptr;               // push blocks (beam, first particle)
+    size_t d_pblocks_cap = 0;
+    int2* d_pitems = nullptr;                // sort/deposit waves (beam, first cell)
+    size_t d_pitems_cap = 0;
+    int npblocks = 0, npitems = 0;
+    bool has_pic = false;                    // some active simulation carries a beam
+    int32_t* d_err = nullptr;                // index-check flag of the sort kernel
 };
 
 int fdtd_fail(int code, const std::string& msg);

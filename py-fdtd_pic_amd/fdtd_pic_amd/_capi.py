@@ -50,6 +50,16 @@ class SimDesc(C.Structure):
     ]
 
 
+class BeamDesc(C.Structure):
+    _fields_ = [
+        ("n", C.c_int64),
+        ("q", C.c_double), ("m", C.c_double), ("w", C.c_double),
+        ("dt", C.c_double), ("dz", C.c_double),
+        ("x", _dp), ("u", _dp),
+        ("group", C.c_int32),
+    ]
+
+
 # every symbol include/fdtdpic.h declares, with (restype, argtypes)
 _vp = C.c_void_p
 SIGNATURES = {
@@ -72,6 +82,8 @@ SIGNATURES = {
     "fdtd_set_profile": (C.c_int, [_vp, C.c_int]),
     "fdtd_device_bytes": (C.c_int, [_vp, C.POINTER(C.c_int64)]),
     "fdtd_version": (C.c_char_p, []),
+    "pic_attach": (C.c_int, [_vp, C.c_int, C.POINTER(BeamDesc)]),
+    "pic_read": (C.c_int, [_vp, C.c_int, _dp, _dp, C.POINTER(C.c_int8), C.c_int64]),
 }
 
 _lib = None

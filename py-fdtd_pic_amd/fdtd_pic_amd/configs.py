@@ -29,6 +29,8 @@ SC = 0.5
 DT = SC * DZ / C0
 W = 32
 TWO_PI = 2 * math.pi
+E_CHARGE = 1.602176634e-19
+M_ELECTRON = 9.1093837139e-31      # CODATA 2022
 
 
 def base(N, steps, **kw):
@@ -60,6 +62,24 @@ def config3(seed=3, steps=20000, N=65536, n_particles=1_000_000):
     p["beam"] = dict(seed=seed, n=int(n_particles), cell_lo=30000 * N // 65536, cell_hi=40000 * N // 65536,
                      v_drift=0.1 * C0, v_th=1e-3 * C0, density=1e16)
     return p
+
+
+def beam_inputs(p):
+    """Seeded electron beam of a parameter dict with a "beam" entry:
+    positions uniform over [cell_lo, cell_hi) (cell units), velocity
+    v_drift z + N(0, v_th^2) per component, u = gamma v / c; macro-particle
+    weight w = density * extent / n per unit transverse area.  Returns
+    (x, u, q, m, w)."""
+    bm = p["beam"]
+    rng = np.random.default_rng(bm["seed"])
+    n = int(bm["n"])
+    x = bm["cell_lo"] + rng.uniform(0.0, bm["cell_hi"] - bm["cell_lo"], n)
+    v = rng.normal(0.0, bm["v_th"], (n, 3))
+    v[:, 2] += bm["v_drift"]
+    beta = v / C0
+    u = beta / np.sqrt(1.0 - np.sum(beta * beta, axis=1))[:, None]
+    w = bm["density"] * (bm["cell_hi"] - bm["cell_lo"]) * p["dz"] / n
+    return x, u, -E_CHARGE, M_ELECTRON, w
 
 
 def config4_instances(seed=4, count=4096, steps=10000):

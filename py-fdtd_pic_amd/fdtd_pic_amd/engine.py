@@ -106,6 +106,7 @@ class SimSpec:
 class Sim:
     def __init__(self, ctx, sid, spec):
         self.ctx, self.id, self.spec = ctx, sid, spec
+        self.beam_n = 0
 
     def read(self, field):
         s = self.spec
@@ -136,6 +137,16 @@ class Sim:
         check(self.ctx.lib.fdtd_sim_progress(self.ctx.h, self.id, C.byref(v)))
         return v.value
 
+    def particles(self):
+        """(x, u, alive) of the attached beam in input order: x in cell units,
+        u = gamma v / c as n x 3, alive as bool."""
+        n = self.beam_n
+        x, u = np.zeros(n), np.zeros((n, 3))
+        alive = np.zeros(n, dtype=np.int8)
+        check(self.ctx.lib.pic_read(self.ctx.h, self.id, dptr(x), dptr(u),
+                                    alive.ctypes.data_as(C.POINTER(C.c_int8)), n))
+        return x, u, alive.astype(bool)
+
     def reset(self):
         check(self.ctx.lib.fdtd_reset_sim(self.ctx.h, self.id))
 
@@ -158,11 +169,21 @@ class Context:
     def set_tiling(self, K, S=0):
         check(self.lib.fdtd_set_tiling(self.h, int(K), int(S)))
 
-    def add(self, spec):
+    def add(self, spec, beam=None):
+        """Add a simulation; beam (optional) is a Beam whose current couples to it."""
         sid = C.c_int()
+        if beam is not None:
+            spec.has_current = 1
         d = spec.to_struct()
         check(self.lib.fdtd_add_sim(self.h, C.byref(d), C.byref(sid)))
-        return Sim(self, sid.value, spec)
+        sim = Sim(self, sid.value, spec)
+        if beam is not None:
+            x, u = as_f64(beam.x), as_f64(beam.u).reshape(-1, 3)
+            bd = capi.BeamDesc(n=len(x), q=beam.q, m=beam.m, w=beam.w, dt=beam.dt, dz=beam.dz,
+                               x=dptr(x), u=dptr(u), group=int(beam.group))
+            check(self.lib.pic_attach(self.h, sim.id, C.byref(bd)))
+            sim.beam_n = len(x)
+        return sim
 
     def run(self, nsteps):
         check(self.lib.fdtd_run(self.h, int(nsteps)))
@@ -202,6 +223,17 @@ class Context:
             self.close()
         except Exception:
             pass
+
+
+class Beam:
+    """Macro-particle beam: positions x (cell units), u = gamma v / c (n x 3),
+    charge q and mass m of one physical particle, weight w per unit
+    transverse area (m^-2), the grid's dt and dz, and the cells per
+    sort/deposit wave (0: chosen from the density)."""
+
+    def __init__(self, x, u, q, m, w, dt, dz, group=0):
+        self.x, self.u, self.q, self.m, self.w = x, u, float(q), float(m), float(w)
+        self.dt, self.dz, self.group = float(dt), float(dz), int(group)
 
 
 # ---------------------------------------------------------------------------

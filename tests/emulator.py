@@ -99,3 +99,82 @@ def _expand(scalar, strip_vals, Ls, Rs, n):
         a[:Ls] = sv[:Ls]
         a[Rs:] = sv[Ls:]
     return a
+
+
+# ---------------------------------------------------------------------------
+# PIC sort/deposit wave (pic.hip: pic_sort_deposit), lane by lane
+# ---------------------------------------------------------------------------
+def _chunk(cell, base, hi):
+    lanes = np.arange(64)
+    i = base + lanes
+    valid = i < hi
+    c = np.where(valid, cell[np.minimum(i, len(cell) - 1)], 2 ** 31 - 1)
+    prev = np.r_[c[0], c[:-1]]
+    bnd = valid & ((lanes == 0) | (prev != c))
+    fl = np.maximum.accumulate(np.where(bnd, lanes, 0))
+    nxt = np.r_[c[1:], -1]
+    last = valid & ((lanes == 63) | (i + 1 >= hi) | (nxt != c))
+    return i, valid, c, fl, last
+
+
+def emulate_pic_sort_deposit(cell, code, a0, a1, start, G, N):
+    """Returns (new index per old particle, new start[], J) as the kernel computes them."""
+    n = len(cell)
+    new_idx = np.full(n, -1, dtype=np.int64)
+    nstart = start.copy()
+    J = np.zeros(N + 1)
+    L = G + 3
+    for c0 in range(0, N + 1, G):
+        cnt = np.zeros((L, 3), dtype=np.int64)
+        rk = np.zeros((L, 3), dtype=np.int64)
+        S = np.zeros((L, 4))
+        cend = min(c0 + G, N + 1)
+        kb = c0 - 2
+        clamp = lambda v: min(max(int(v), 0), n)
+        lo = clamp(start[max(c0 - 2, 0)])
+        hi = max(lo, clamp(start[min(c0 + G + 1, N + 1)]))
+        for base in range(lo, hi, 64):
+            i, valid, c, fl, last = _chunk(cell, base, hi)
+            ii = np.minimum(i, n - 1)
+            cd = np.where(valid, code[ii], 5)
+            d = (cd & 3) - 1
+            off = (cd >> 2) & 3
+            v = np.zeros((64, 4))
+            v[np.arange(64), off] += np.where(valid, a0[ii], 0.0)
+            v[np.arange(64), off + 1] += np.where(valid, a1[ii], 0.0)
+            for l in np.nonzero(last)[0]:
+                seg = slice(fl[l], l + 1)
+                lk = c[l] - kb
+                if 0 <= lk < L:
+                    for dd in range(3):
+                        cnt[lk, dd] += np.sum(d[seg] == dd - 1)
+                    S[lk] += v[seg].sum(axis=0)
+        olo = clamp(start[c0])
+        ohi = max(olo, clamp(start[cend]))
+        for base in range(olo, ohi, 64):
+            i, valid, c, fl, last = _chunk(cell, base, ohi)
+            ii = np.minimum(i, n - 1)
+            d = np.where(valid, (code[ii] & 3) - 1, 0)
+            ranks = np.zeros(64, dtype=np.int64)
+            for l in np.nonzero(valid)[0]:
+                lk = c[l] - kb
+                ranks[l] = rk[lk, d[l] + 1] + np.sum(d[fl[l]:l] == d[l])
+            for l in np.nonzero(last)[0]:
+                lk = c[l] - kb
+                for dd in range(3):
+                    rk[lk, dd] += np.sum(d[fl[l]:l + 1] == dd - 1)
+            for l in np.nonzero(valid)[0]:
+                cc, lk, r = c[l], c[l] - kb, ranks[l]
+                if d[l] < 0:
+                    ni = start[cc] - cnt[lk - 1, 2] + r
+                elif d[l] == 0:
+                    ni = start[cc] + cnt[lk, 0] + r
+                else:
+                    ni = start[cc + 1] - cnt[lk, 2] + cnt[lk + 1, 0] + r
+                new_idx[i[l]] = ni
+        for cc in range(c0, cend):
+            lk = cc - kb
+            nstart[cc] = start[cc] - cnt[lk - 1, 2] + cnt[lk, 0]
+            if cc <= N:
+                J[cc] = S[lk - 2, 3] + S[lk - 1, 2] + S[lk, 1] + S[lk + 1, 0]
+    return new_idx, nstart, J

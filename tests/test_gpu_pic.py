@@ -1,0 +1,113 @@
+"""PIC coupling on the GPU against the FP64 oracle (oracle/pic.py).
+
+Cases (config-1 slab, thermal electron beam drifting at 0.1c):
+  beam_only  no source: every field is radiated by the beam current
+  driven     a 1e9 V/m pulse crosses the beam before reaching the slab
+  absorbed   a beam drifting into the left boundary (particles parked)
+FP64: fields, probes, particle displacement and u within 1e-10 relative L2;
+FP32: within 1e-5.  Also: bitwise invariance of the run under a permutation
+of the input particles, agreement between sort/deposit wave widths, and a
+reduced config 3 (N = 65,536, 1e5 particles, 300 steps) in FP32.
+"""
+import numpy as np
+import pytest
+
+from refcases import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _params(kind, N=4096, steps=600, n=20000):
+    from fdtd_pic_amd import configs
+    p = configs.base(N, steps, slab_start=N // 2, slab_stop=N // 2 + 400)
+    C0 = configs.C0
+    lo, hi, vd = 1200, 1800, 0.1 * C0
+    if kind == "beam_only":
+        p["src_amp"] = 0.0
+    elif kind == "driven":
+        p["src_amp"] = 1e9
+    elif kind == "absorbed":
+        p["src_amp"] = 0.0
+        p["steps"] = 250               # about half of the beam crosses cell 2
+        lo, hi, vd = 3, 60, -0.3 * C0
+    p["beam"] = dict(seed=17, n=n, cell_lo=lo, cell_hi=hi, v_drift=vd, v_th=1e-3 * C0, density=1e16)
+    return p
+
+
+def _oracle(p):
+    from fdtd_pic_amd import configs
+    from oracle import pic as P, physical as ph
+    m = ph.build(p)
+    x, u, q, mm, w = configs.beam_inputs(p)
+    b = P.Beam(x, u, q, mm, w, m["dt"], p["N"], p["Sc"])
+    s, probes = P.run(m, b)
+    return dict(E=s.E, H=s.H, probes=probes, x=b.x, u=b.u, alive=b.alive, x0=x)
+
+
+def _device(p, prec, group=0, perm=None):
+    from fdtd_pic_amd import configs, engine
+    x, u, q, mm, w = configs.beam_inputs(p)
+    if perm is not None:
+        x, u = x[perm], u[perm]
+    dt = p["Sc"] * p["dz"] / configs.C0
+    with engine.Context(prec) as ctx:
+        sim = ctx.add(engine.physical_spec(p), beam=engine.Beam(x, u, q, mm, w, dt, p["dz"], group))
+        ctx.run_all()
+        xo, uo, alive = sim.particles()
+        return dict(E=sim.read("E"), H=sim.read("H"), probes=sim.probes(), x=xo, u=uo, alive=alive,
+                    J=sim.read("J"))
+
+
+@pytest.fixture(scope="module")
+def oracles():
+    return {k: (_params(k), _oracle(_params(k))) for k in ("beam_only", "driven", "absorbed")}
+
+
+@pytest.mark.parametrize("kind", ["beam_only", "driven", "absorbed"])
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-10), ("f32", 1e-5)])
+def test_pic_parity(oracles, kind, prec, tol):
+    p, o = oracles[kind]
+    print(f"pic parity {kind} {prec}", flush=True)
+    d = _device(p, prec)
+    errs = dict(E=rel_l2(d["E"], o["E"]), H=rel_l2(d["H"], o["H"]), probes=rel_l2(d["probes"], o["probes"]),
+                dx=rel_l2(d["x"] - o["x0"], o["x"] - o["x0"]), u=rel_l2(d["u"], o["u"]))
+    print(kind, prec, {k: f"{v:.2e}" for k, v in errs.items()}, "absorbed", int((~o["alive"]).sum()), flush=True)
+    assert np.abs(o["E"]).max() > 0
+    assert np.array_equal(d["alive"], o["alive"])
+    if kind == "absorbed":
+        assert 100 < (~o["alive"]).sum() < len(o["alive"]) - 100
+    for k, v in errs.items():
+        assert v <= tol, (kind, prec, k, v)
+
+
+def test_permutation_invariance_bitwise():
+    p = _params("driven", steps=200)
+    rng = np.random.default_rng(5)
+    perm = rng.permutation(p["beam"]["n"])
+    a = _device(p, "f32")
+    b = _device(p, "f32", perm=perm)
+    for k in ("E", "H", "J", "probes"):
+        assert np.array_equal(a[k], b[k]), k
+    assert np.array_equal(b["x"], a["x"][perm]) and np.array_equal(b["u"], a["u"][perm])
+
+
+def test_wave_width_independence():
+    p = _params("driven", steps=300)
+    ref = _device(p, "f64", group=0)
+    for g in (1, 5, 61):
+        d = _device(p, "f64", group=g)
+        assert rel_l2(d["E"], ref["E"]) < 1e-13, g
+        assert np.array_equal(d["x"], ref["x"]), g
+
+
+def test_reduced_config3_f32():
+    from fdtd_pic_amd import configs
+    p = configs.config3(steps=300, n_particles=100_000)
+    print("reduced config3 oracle", flush=True)
+    o = _oracle(p)
+    d = _device(p, "f32")
+    errs = dict(E=rel_l2(d["E"], o["E"]), H=rel_l2(d["H"], o["H"]),
+                dx=rel_l2(d["x"] - o["x0"], o["x"] - o["x0"]), u=rel_l2(d["u"], o["u"]))
+    print("config3 reduced f32", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    for k, v in errs.items():
+        assert v <= 1e-5, (k, v)
