@@ -38,6 +38,11 @@ def parse():
     ap.add_argument("--S", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU baseline sample length")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-pic", action="store_true", help="skip the config 3 (PIC) measurement")
+    ap.add_argument("--pic-steps", type=int, default=20000, help="time steps of the config 3 run")
+    ap.add_argument("--pic-particles", type=int, default=1_000_000)
+    ap.add_argument("--pic-only", action="store_true", help="print only the config 3 object")
+    ap.add_argument("--pic-group", type=int, default=0, help="cells per sort/deposit wave (0: automatic)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 256 MiB device copy (FETCH_SIZE/WRITE_SIZE calibration under rocprofv3 --pmc)")
     return ap.parse_args()
@@ -98,6 +103,65 @@ def cpu_baseline(p, target_s):
                        f"total; FP64 C, {threads} OpenMP threads), {secs:.1f} s")
 
 
+def pic_bench(a):
+    """Config 3: N = 65,536 grid with the config-1 slab and a 1e6-particle
+    electron beam, 20,000 steps, FP32 fields and particles.  One timed run
+    after one warm-up run; particles are re-uploaded (reset) outside the
+    timed region."""
+    from fdtd_pic_amd import configs, engine
+    p = configs.config3(steps=a.pic_steps, n_particles=a.pic_particles)
+    x, u, q, m, w = configs.beam_inputs(p)
+    beam = engine.Beam(x, u, q, m, w, configs.DT, p["dz"], a.pic_group)
+    ctx = engine.Context("f32", K=2, S=1)
+    sim = ctx.add(engine.physical_spec(p), beam=beam)
+    ctx.run_all()
+    ctx.sync()
+    sim.reset()
+    ctx.sync()
+    ctx.set_profile(True)
+    ctx.timing(reset=True)
+    t0 = time.perf_counter()
+    ctx.run_all()
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    kt = {k: ctx.kernel_timing(k) for k in ("stencil", "push", "sort")}
+    ctx.close()
+    n, T, N = p["beam"]["n"], p["steps"], p["N"]
+    pushes, cells = n * T, N * T
+    bpc = configs.bytes_per_cell_update(p, "f32")
+    bpp = 40.0          # cell int32 + frac + ux, uy, uz (FP32), read and written
+    kern_ms = sum(v[0] for v in kt.values())
+    achieved = (pushes * bpp + cells * bpc) / (kern_ms / 1e3) / 1e9
+    out = {"metric": "Gparticle-pushes/s", "value": round(pushes / wall / 1e9, 3),
+           "cells_value": round(cells / wall / 1e9, 3), "ms_per_run": round(wall * 1e3, 2),
+           "config": {"workload": f"config3: N={N} cells, {T} steps, config-1 slab, {n} electrons "
+                                  "(0.1c drift, 1e-3c thermal), FP32 + FP64 deposit sums", "K": 2, "S": 1},
+           "kernels_avg_us": {k: round(v[0] * 1e3 / max(v[1], 1), 2) for k, v in kt.items()},
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_push": bpp,
+                        "bytes_per_cell_update": round(bpc, 2), "traffic": None}}
+    if not a.no_cpu:
+        out["cpu_baseline"] = pic_cpu_baseline(p, x, u, q, m, w)
+    return out
+
+
+def pic_cpu_baseline(p, x, u, q, m, w, steps=40):
+    """The FP64 NumPy particle oracle (one core) on the first steps of the same workload."""
+    from oracle import pic as P, physical as ph
+    mm = ph.build(p)
+    b = P.Beam(x, u, q, m, w, mm["dt"], p["N"], p["Sc"])
+    s = ph.State(p["N"], mm["R"] - mm["F"])
+    Hprev = s.H.copy()
+    t0 = time.perf_counter()
+    for t in range(steps):
+        J = P.push_deposit(b, s.E, Hprev, s.H)
+        Hprev = s.H.copy()
+        ph.step(mm, s, t, J)
+    secs = time.perf_counter() - t0
+    return dict(value=len(x) * steps / secs / 1e9, unit="Gparticle-pushes/s", cores=1, kind="port",
+                sample=f"config3, first {steps} steps of {len(x)} particles, NumPy FP64 oracle, {secs:.1f} s")
+
+
 def cpu_run(cpu_ref, m, n, threads):
     return cpu_ref.CpuSim(m).run(0, n, threads)
 
@@ -110,6 +174,9 @@ def main():
         sys.exit(2)
     if a.calibrate_copy:
         calibrate_copy()
+        return
+    if a.pic_only:
+        print(json.dumps(pic_bench(a)), flush=True)
         return
     from fdtd_pic_amd import configs, engine
 
@@ -163,6 +230,8 @@ def main():
     ctx.close()
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, a.cpu_seconds)
+    if not a.no_pic:
+        out["pic"] = pic_bench(a)
     print(json.dumps(out), flush=True)
 
 

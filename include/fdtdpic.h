@@ -59,6 +59,11 @@ extern "C" {
 #define FDTD_FIELD_PSIH 6    /* psi_Hy, length N+1 */
 #define FDTD_FIELD_J 7       /* Jx as used by the E update, length N+1 */
 
+/* kernel kinds for fdtd_kernel_timing */
+#define FDTD_KERNEL_STENCIL 0  /* field stencil (E/H/ADE/Kerr/CPML) */
+#define FDTD_KERNEL_PUSH 1     /* particle gather + Boris push */
+#define FDTD_KERNEL_SORT 2     /* particle sort + CIC deposit */
+
 typedef struct fdtd_ctx fdtd_ctx;
 
 /*
@@ -171,6 +176,9 @@ const char *fdtd_last_error(void);
 /* Device time (ms) spent in stencil launches since the last reset, from HIP
  * events stamped by the kernels' own dispatch packets; also the launch count. */
 int fdtd_timing(fdtd_ctx *ctx, double *ms, int64_t *launches, int reset);
+/* Device time (ms) and launch count of one kernel kind (FDTD_KERNEL_*) since
+ * the last reset, under profiling. */
+int fdtd_kernel_timing(fdtd_ctx *ctx, int kind, double *ms, int64_t *launches);
 /* Profiling: when on, every stencil launch carries start/stop HIP events so
  * fdtd_timing reports the summed kernel durations and the launch count (the
  * average launch duration used for the roofline). */

@@ -19,7 +19,8 @@ hipError_t launch_stencil(const SimDev<R>* sims, const int2* tiles, int ntiles, 
                           hipEvent_t ev0, hipEvent_t ev1);
 template <typename R>
 hipError_t launch_pic(const SimDev<R>* sims, const BeamDev<R>* beams, const int2* push_blocks, int npush,
-                      const int2* items, int nitems, long long launch_idx, hipStream_t st);
+                      const int2* items, int nitems, long long launch_idx, hipStream_t st, hipEvent_t e0,
+                      hipEvent_t e1, hipEvent_t e2, hipEvent_t e3);
 }
 
 using namespace fdtd;
@@ -225,6 +226,7 @@ int fdtd_ctx_destroy(fdtd_ctx* c) {
     if (c->d_sims) (void)hipFree(c->d_sims);
     if (c->d_tiles) (void)hipFree(c->d_tiles);
     for (auto& e : c->events) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+    for (hipEvent_t e : c->throttle) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->st);
     delete c;
     return FDTD_OK;
@@ -519,9 +521,41 @@ static int rebuild_t(fdtd_ctx* c) {
     return FDTD_OK;
 }
 
+// Keeps at most about kThrottleEvery * kThrottleDepth launches queued on the
+// stream: a long fdtd_run would otherwise enqueue every kernel of the run
+// (three per step with a beam) before the first one finishes.
+static int throttle(fdtd_ctx* c) {
+    if (++c->throttle_launches % kThrottleEvery) return FDTD_OK;
+    if (c->throttle.size() < kThrottleDepth) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        c->throttle.push_back(e);
+    }
+    hipEvent_t& slot = c->throttle[c->throttle_head];
+    if (c->throttle_launches > (long long)(kThrottleEvery * kThrottleDepth)) HIPCHK(hipEventSynchronize(slot));
+    HIPCHK(hipEventRecord(slot, c->st));
+    c->throttle_head = (c->throttle_head + 1) % kThrottleDepth;
+    return FDTD_OK;
+}
+
+// Next profiling event pair (created on first use); nullptr on a HIP error.
+static EvPair* next_event(fdtd_ctx* c, int kind) {
+    if (c->ev_used == c->events.size()) {
+        EvPair p;
+        if (hipEventCreate(&p.first) != hipSuccess || hipEventCreate(&p.second) != hipSuccess) {
+            fdtd_fail(FDTD_E_HIP, "hipEventCreate failed");
+            return nullptr;
+        }
+        c->events.push_back(p);
+    }
+    EvPair* e = &c->events[c->ev_used++];
+    e->kind = kind;
+    return e;
+}
+
 // ev (profiling): ev->first is stamped at the start of the first group's
 // kernel, ev->second at the end of the last one.
-static int launch_once(fdtd_ctx* c, int n, std::pair<hipEvent_t, hipEvent_t>* ev) {
+static int launch_once(fdtd_ctx* c, int n, EvPair* ev) {
     const size_t ng = c->groups.size();
     for (size_t gi = 0; gi < ng; ++gi) {
         const TileGroup& g = c->groups[gi];
@@ -555,24 +589,37 @@ int fdtd_run(fdtd_ctx* c, int64_t nsteps) {
         // a beam is coupled (the deposit feeds every E update)
         int n = (int)std::min<int64_t>(remaining, c->S_run);
         if (c->has_pic) {
+            // event handles are copied out at once: next_event may grow (move) the vector
+            hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
+            if (c->profile) {
+                const EvPair* pe = next_event(c, FDTD_KERNEL_PUSH);
+                if (!pe) return FDTD_E_HIP;
+                e0 = pe->first; e1 = pe->second;
+                const EvPair* se = next_event(c, FDTD_KERNEL_SORT);
+                if (!se) return FDTD_E_HIP;
+                e2 = se->first; e3 = se->second;
+            }
+            const hipError_t stale = hipGetLastError();
+            if (stale != hipSuccess)
+                return fdtd_fail(FDTD_E_HIP, std::string("error pending before the pic launch at launch ") +
+                                                 std::to_string(c->launch_idx) + ": " + hipGetErrorString(stale));
             hipError_t e = c->prec == FDTD_F64
                 ? launch_pic<double>((const SimDev<double>*)c->d_sims, (const BeamDev<double>*)c->d_beams,
-                                     c->d_pblocks, c->npblocks, c->d_pitems, c->npitems, c->launch_idx, c->st)
+                                     c->d_pblocks, c->npblocks, c->d_pitems, c->npitems, c->launch_idx, c->st,
+                                     e0, e1, e2, e3)
                 : launch_pic<float>((const SimDev<float>*)c->d_sims, (const BeamDev<float>*)c->d_beams,
-                                    c->d_pblocks, c->npblocks, c->d_pitems, c->npitems, c->launch_idx, c->st);
-            if (e != hipSuccess) return fdtd_fail(FDTD_E_HIP, std::string("pic launch: ") + hipGetErrorString(e));
+                                    c->d_pblocks, c->npblocks, c->d_pitems, c->npitems, c->launch_idx, c->st,
+                                    e0, e1, e2, e3);
+            if (e != hipSuccess)
+                return fdtd_fail(FDTD_E_HIP, std::string("pic launch ") + std::to_string(c->launch_idx) + " (" +
+                                                 std::to_string(c->npblocks) + " push blocks, " +
+                                                 std::to_string(c->npitems) + " sort waves): " + hipGetErrorString(e));
         }
-        std::pair<hipEvent_t, hipEvent_t>* ev = nullptr;
-        if (c->profile) {
-            if (c->ev_used == c->events.size()) {
-                hipEvent_t a, b;
-                HIPCHK(hipEventCreate(&a));
-                HIPCHK(hipEventCreate(&b));
-                c->events.emplace_back(a, b);
-            }
-            ev = &c->events[c->ev_used++];
-        }
+        EvPair* ev = c->profile ? next_event(c, FDTD_KERNEL_STENCIL) : nullptr;
+        if (c->profile && !ev) return FDTD_E_HIP;
         int rc = launch_once(c, n, ev);
+        if (rc) return rc;
+        rc = throttle(c);
         if (rc) return rc;
         c->launches++;
         bool finished = false;
@@ -613,17 +660,27 @@ int fdtd_sync(fdtd_ctx* c) {
     return c->has_pic ? check_pic_flag(c) : FDTD_OK;
 }
 
-int fdtd_timing(fdtd_ctx* c, double* ms, int64_t* launches, int reset) {
+int fdtd_kernel_timing(fdtd_ctx* c, int kind, double* ms, int64_t* launches) {
     if (!c) return fdtd_fail(FDTD_E_ARG, "null ctx");
+    if (kind < FDTD_KERNEL_STENCIL || kind > FDTD_KERNEL_SORT) return fdtd_fail(FDTD_E_ARG, "bad kernel kind");
     HIPCHK(hipStreamSynchronize(c->st));
     double tot = 0;
+    int64_t n = 0;
     for (size_t i = 0; i < c->ev_used; ++i) {
+        if (c->events[i].kind != kind) continue;
         float m = 0;
         HIPCHK(hipEventElapsedTime(&m, c->events[i].first, c->events[i].second));
         tot += m;
+        ++n;
     }
     if (ms) *ms = tot;
-    if (launches) *launches = (int64_t)c->ev_used;
+    if (launches) *launches = n;
+    return FDTD_OK;
+}
+
+int fdtd_timing(fdtd_ctx* c, double* ms, int64_t* launches, int reset) {
+    int rc = fdtd_kernel_timing(c, FDTD_KERNEL_STENCIL, ms, launches);
+    if (rc) return rc;
     if (reset) c->ev_used = 0;
     return FDTD_OK;
 }

@@ -36,6 +36,11 @@ struct BeamHost {
     std::vector<double> frac0, u0;             // frac, then ux uy uz interleaved
 };
 
+struct EvPair {
+    hipEvent_t first = nullptr, second = nullptr;
+    int kind = 0;               // FDTD_KERNEL_*
+};
+
 struct TileGroup {
     int kind, offset, count;    // tiles [offset, offset+count) run the kernel for this slab kind
 };
@@ -56,7 +61,7 @@ struct fdtd_ctx {
     long long launch_idx = 0, gstep = 0;
     bool dirty = true;
     bool profile = false;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+    std::vector<EvPair> events;   // profiling: start/stop per launch, tagged by kernel kind
     size_t ev_used = 0;
     long long launches = 0;
     long long dev_bytes = 0;
@@ -70,10 +75,18 @@ struct fdtd_ctx {
     size_t d_pitems_cap = 0;
     int npblocks = 0, npitems = 0;
     bool has_pic = false;                    // some active simulation carries a beam
+    // bound on queued work: an event every kThrottleEvery launches; the host
+    // waits on the one recorded kThrottleDepth events back before queueing more
+    std::vector<hipEvent_t> throttle;
+    size_t throttle_head = 0;
+    long long throttle_launches = 0;
     int32_t* d_err = nullptr;                // index-check flag of the sort kernel
 };
 
 int fdtd_fail(int code, const std::string& msg);
+
+constexpr int kThrottleEvery = 128;
+constexpr size_t kThrottleDepth = 4;
 
 // Default steps per launch for K cells per lane: the tile keeps 75% of its
 // cells as output (U = 64K - 2S = 48K).

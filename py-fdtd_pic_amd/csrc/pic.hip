@@ -24,6 +24,7 @@
 //                     S0(m) + S-1(m+1) from per-cell FP64 sums in a fixed
 //                     order.  The wave reads cells c0-2 .. c0+G so every value
 //                     it needs is its own.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "device_common.hpp"
@@ -244,19 +245,39 @@ __global__ __launch_bounds__(64) void pic_sort_deposit(const SimDev<R>* __restri
     }
 }
 
+// With profiling events the kernels go through hipExtLaunchKernelGGL, which
+// stamps the events from their own dispatch packets.
 template <typename R>
 hipError_t launch_pic(const SimDev<R>* sims, const BeamDev<R>* beams, const int2* push_blocks, int npush,
-                      const int2* items, int nitems, long long launch_idx, hipStream_t st) {
-    if (npush > 0)
-        hipLaunchKernelGGL((pic_push<R>), dim3(npush), dim3(256), 0, st, sims, beams, push_blocks, launch_idx);
-    if (nitems > 0)
-        hipLaunchKernelGGL((pic_sort_deposit<R>), dim3(nitems), dim3(64), 0, st, sims, beams, items, launch_idx);
-    return hipGetLastError();
+                      const int2* items, int nitems, long long launch_idx, hipStream_t st, hipEvent_t e0,
+                      hipEvent_t e1, hipEvent_t e2, hipEvent_t e3) {
+    hipError_t e = hipSuccess;
+    if (npush > 0) {
+        if (e0 || e1)
+            hipExtLaunchKernelGGL((pic_push<R>), dim3(npush), dim3(256), 0, st, e0, e1, 0, sims, beams,
+                                  push_blocks, launch_idx);
+        else
+            hipLaunchKernelGGL((pic_push<R>), dim3(npush), dim3(256), 0, st, sims, beams, push_blocks, launch_idx);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (nitems > 0) {
+        if (e2 || e3)
+            hipExtLaunchKernelGGL((pic_sort_deposit<R>), dim3(nitems), dim3(64), 0, st, e2, e3, 0, sims, beams,
+                                  items, launch_idx);
+        else
+            hipLaunchKernelGGL((pic_sort_deposit<R>), dim3(nitems), dim3(64), 0, st, sims, beams, items,
+                               launch_idx);
+        e = hipGetLastError();
+    }
+    return e;
 }
 
 template hipError_t launch_pic<double>(const SimDev<double>*, const BeamDev<double>*, const int2*, int,
-                                       const int2*, int, long long, hipStream_t);
+                                       const int2*, int, long long, hipStream_t, hipEvent_t, hipEvent_t,
+                                       hipEvent_t, hipEvent_t);
 template hipError_t launch_pic<float>(const SimDev<float>*, const BeamDev<float>*, const int2*, int,
-                                      const int2*, int, long long, hipStream_t);
+                                      const int2*, int, long long, hipStream_t, hipEvent_t, hipEvent_t,
+                                      hipEvent_t, hipEvent_t);
 
 }  // namespace fdtd

@@ -82,6 +82,7 @@ SIGNATURES = {
     "fdtd_set_profile": (C.c_int, [_vp, C.c_int]),
     "fdtd_device_bytes": (C.c_int, [_vp, C.POINTER(C.c_int64)]),
     "fdtd_version": (C.c_char_p, []),
+    "fdtd_kernel_timing": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "pic_attach": (C.c_int, [_vp, C.c_int, C.POINTER(BeamDesc)]),
     "pic_read": (C.c_int, [_vp, C.c_int, _dp, _dp, C.POINTER(C.c_int8), C.c_int64]),
 }

@@ -202,6 +202,13 @@ class Context:
         check(self.lib.fdtd_timing(self.h, C.byref(ms), C.byref(n), 1 if reset else 0))
         return ms.value, n.value
 
+    def kernel_timing(self, kind):
+        """(ms, launches) of one kernel kind: "stencil", "push" or "sort"."""
+        k = {"stencil": 0, "push": 1, "sort": 2}[kind]
+        ms, n = C.c_double(), C.c_int64()
+        check(self.lib.fdtd_kernel_timing(self.h, k, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
     def device_bytes(self):
         v = C.c_int64()
         check(self.lib.fdtd_device_bytes(self.h, C.byref(v)))
