@@ -418,7 +418,10 @@ static int attach_t(fdtd_ctx* c, int id, SimHost& s, const fdtd_beam_desc_t* bd)
     }
     int64_t acc = 0, occupied = 0;
     for (int cc = 0; cc <= N + 1; ++cc) { b->start0[cc] = (int32_t)acc; acc += occ[cc]; occupied += occ[cc] > 0; }
-    // cells per sort/deposit wave: about 256 particles per wave
+    // cells per sort/deposit wave: about 256 particles per wave.  The wave
+    // also reads 3 neighbour cells, but its serial chunk loop, not the
+    // re-read, sets the kernel time (measured best at 2-4 cells for 100
+    // particles per cell on MI355X).
     int G = bd->group;
     if (G <= 0) {
         const double per_cell = occupied ? (double)n / occupied : 1.0;

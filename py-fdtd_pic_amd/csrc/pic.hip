@@ -105,24 +105,84 @@ __global__ __launch_bounds__(256) void pic_push(const SimDev<R>* __restrict__ si
     b.tcode[i] = (uint8_t)code;
 }
 
+// 64-bit DPP move (two 32-bit halves); lanes the control does not feed read 0.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, ROWMASK, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWMASK, 0xf, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Inclusive scan of four FP64 values per lane, segmented at fl (the first lane
+// of the lane's segment): row_shr 1,2,4,8 inside 16-lane rows, then
+// row_bcast:15 (rows 1 and 3 take the last lane of the row below) and
+// row_bcast:31 (rows 2 and 3 take lane 31).  A step adds its source only when
+// the source lane lies in the lane's own segment, so every partial stays a
+// contiguous run of the segment.
+__device__ __forceinline__ void seg_scan4(double (&v)[4], int lane, int fl) {
+#define FDTD_SEG_STEP(CTRL, RM, COND)                                      \
+    {                                                                      \
+        const bool ok = (COND);                                            \
+        _Pragma("unroll") for (int q = 0; q < 4; ++q) {                    \
+            const double t = dpp_f64<CTRL, RM>(v[q]);                      \
+            v[q] = ok ? v[q] + t : v[q];                                   \
+        }                                                                  \
+    }
+    FDTD_SEG_STEP(0x111, 0xf, (lane & 15) >= 1 && lane - 1 >= fl)
+    FDTD_SEG_STEP(0x112, 0xf, (lane & 15) >= 2 && lane - 2 >= fl)
+    FDTD_SEG_STEP(0x114, 0xf, (lane & 15) >= 4 && lane - 4 >= fl)
+    FDTD_SEG_STEP(0x118, 0xf, (lane & 15) >= 8 && lane - 8 >= fl)
+    FDTD_SEG_STEP(0x142, 0xa, ((lane >> 4) & 1) && (lane & ~15) - 1 >= fl)
+    FDTD_SEG_STEP(0x143, 0xc, lane >= 32 && 31 >= fl)
+#undef FDTD_SEG_STEP
+}
+
+// The sort/deposit workgroup is a single wave: LDS accesses then execute in
+// program order, so a wavefront-scope fence (a compiler barrier, no wait on
+// outstanding global loads) orders the cross-lane LDS traffic; a workgroup
+// barrier would also drain the prefetched loads of the next chunk.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 // Segment bookkeeping of one 64-particle chunk of a cell-sorted range.
 struct Chunk {
-    int c;                          // old cell (INT_MAX for lanes past the end)
     bool valid, last;               // last: final lane of its cell in this chunk
     int fl;                         // first lane of this lane's cell in the chunk
 };
 
-__device__ __forceinline__ Chunk chunk_info(const int32_t* __restrict__ cell, int i, int hi, int lane) {
+// c: this lane's old cell (any value when !valid); i: particle index, hi: end of range.
+__device__ __forceinline__ Chunk chunk_info(int c, int i, int hi, int lane) {
     Chunk k;
     k.valid = i < hi;
-    k.c = k.valid ? cell[i] : 0x7fffffff;
-    const int cp = __shfl_up(k.c, 1);
-    const int cn = __shfl_down(k.c, 1);
-    const unsigned long long bnd = __ballot(k.valid && (lane == 0 || cp != k.c));
+    const int cp = dpp_shr1_i(c);       // lane-1 (lane 0: unused)
+    const int cn = dpp_shl1_i(c);       // lane+1 (lane 63: unused)
+    const unsigned long long bnd = __ballot(k.valid && (lane == 0 || cp != c));
     const unsigned long long m = bnd & lanes_le(lane);
     k.fl = m ? 63 - __clzll((long long)m) : 0;
-    k.last = k.valid && (lane == 63 || i + 1 >= hi || cn != k.c);
+    k.last = k.valid && (lane == 63 || i + 1 >= hi || cn != c);
     return k;
+}
+
+constexpr int kBatch = 8;           // chunks fetched per memory round trip in the sort kernel
+
+// Per-particle inputs of pass 1.
+template <typename R>
+struct P1 {
+    int c, code;
+    R a0, a1;
+};
+template <typename R>
+__device__ __forceinline__ P1<R> load_p1(const BeamDev<R>& b, const int32_t* cell, int i, int hi) {
+    P1<R> p;
+    const bool v = i < hi;
+    p.c = v ? cell[i] : 0x7fffffff;
+    p.code = v ? (int)b.tcode[i] : 5;
+    p.a0 = v ? b.ta0[i] : R(0);
+    p.a1 = v ? b.ta1[i] : R(0);
+    return p;
 }
 
 template <typename R>
@@ -149,89 +209,111 @@ __global__ __launch_bounds__(64) void pic_sort_deposit(const SimDev<R>* __restri
         rk[k][0] = rk[k][1] = rk[k][2] = 0;
         S[k][0] = S[k][1] = S[k][2] = S[k][3] = 0.0;
     }
-    __syncthreads();
+    wave_sync();
     const int32_t* st = b.start[par];
     const int32_t* cell = b.cell[par];
     auto clampi = [&](int v) { return min(max(v, 0), np); };
     const int lo = clampi(st[max(c0 - 2, 0)]);
     const int hi = max(lo, clampi(st[min(c0 + G + 1, N + 1)]));
-
-    // pass 1: direction counts and segmented J' sums over cells c0-2 .. c0+G
-    for (int base = lo; base < hi; base += 64) {
-        const int i = base + lane;
-        const Chunk k = chunk_info(cell, i, hi, lane);
-        const int code = k.valid ? (int)b.tcode[i] : 5;
-        const int d = (code & 3) - 1;
-        const int off = (code >> 2) & 3;
-        const double a0 = k.valid ? (double)b.ta0[i] : 0.0;
-        const double a1 = k.valid ? (double)b.ta1[i] : 0.0;
-        const unsigned long long seg = lanes_le(lane) & ~lanes_lt(k.fl);
-        const unsigned long long bd0 = __ballot(k.valid && d == -1);
-        const unsigned long long bd1 = __ballot(k.valid && d == 0);
-        const unsigned long long bd2 = __ballot(k.valid && d == 1);
-        double v[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) v[q] = (q == off ? a0 : 0.0) + (q == off + 1 ? a1 : 0.0);
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const double t = __shfl_up(v[q], o);
-                if (lane - o >= k.fl) v[q] += t;
-            }
-        }
-        const int lk = k.valid ? k.c - kb : -1;
-        if (k.last && lk >= 0 && lk < L) {
-            cnt[lk][0] += __popcll(bd0 & seg);
-            cnt[lk][1] += __popcll(bd1 & seg);
-            cnt[lk][2] += __popcll(bd2 & seg);
-#pragma unroll
-            for (int q = 0; q < 4; ++q) S[lk][q] += v[q];
-        }
-        __syncthreads();
-    }
-
-    // pass 2: stable ranks and the scatter of this wave's own particles
     const int olo = clampi(st[c0]);
     const int ohi = max(olo, clampi(st[cend]));
-    for (int base = olo; base < ohi; base += 64) {
-        const int i = base + lane;
-        const Chunk k = chunk_info(cell, i, ohi, lane);
-        const int code = k.valid ? (int)b.tcode[i] : 5;
-        const int d = (code & 3) - 1;
-        const unsigned long long seg = lanes_le(lane) & ~lanes_lt(k.fl);
-        const unsigned long long bd0 = __ballot(k.valid && d == -1);
-        const unsigned long long bd1 = __ballot(k.valid && d == 0);
-        const unsigned long long bd2 = __ballot(k.valid && d == 1);
-        const unsigned long long mine = d < 0 ? bd0 : (d == 0 ? bd1 : bd2);
-        const int lk = k.valid ? k.c - kb : -1;
-        const bool in = k.valid && k.c >= 0 && k.c <= N && lk >= 1 && lk + 1 < L;
-        int r = 0;
-        if (in) r = rk[lk][d + 1] + __popcll(mine & lanes_lt(lane) & ~lanes_lt(k.fl));
-        __syncthreads();
-        if (in && k.last) {
-            rk[lk][0] += __popcll(bd0 & seg);
-            rk[lk][1] += __popcll(bd1 & seg);
-            rk[lk][2] += __popcll(bd2 & seg);
-        }
-        if (in) {
-            const int c = k.c;
-            int ni;
-            if (d < 0) ni = st[c] - cnt[lk - 1][2] + r;
-            else if (d == 0) ni = st[c] + cnt[lk][0] + r;
-            else ni = st[c + 1] - cnt[lk][2] + cnt[lk + 1][0] + r;
-            if (ni >= 0 && ni < np) {
-                b.cell[po][ni] = b.tcell[i];
-                b.frac[po][ni] = b.tfrac[i];
-                b.ux[po][ni] = b.tux[i];
-                b.uy[po][ni] = b.tuy[i];
-                b.uz[po][ni] = b.tuz[i];
-                b.id[po][ni] = b.id[par][i];
-            } else {
-                b.err[0] = 1;
+
+    // pass 1: direction counts and segmented J' sums over cells c0-2 .. c0+G.
+    // Chunks are fetched kBatch at a time (one memory round trip per batch)
+    // and then processed in order from registers.
+    for (int bb = lo; bb < hi; bb += 64 * kBatch) {
+        P1<R> pk[kBatch];
+#pragma unroll
+        for (int q = 0; q < kBatch; ++q) pk[q] = load_p1(b, cell, bb + 64 * q + lane, hi);
+#pragma unroll
+        for (int q = 0; q < kBatch; ++q) {
+            const int base = bb + 64 * q;
+            if (base >= hi) break;
+            const P1<R>& cur = pk[q];
+            const int i = base + lane;
+            const Chunk k = chunk_info(cur.c, i, hi, lane);
+            const int d = (cur.code & 3) - 1;
+            const int off = (cur.code >> 2) & 3;
+            const double a0 = (double)cur.a0, a1 = (double)cur.a1;
+            const unsigned long long seg = lanes_le(lane) & ~lanes_lt(k.fl);
+            const unsigned long long bd0 = __ballot(k.valid && d == -1);
+            const unsigned long long bd1 = __ballot(k.valid && d == 0);
+            const unsigned long long bd2 = __ballot(k.valid && d == 1);
+            double v[4];
+#pragma unroll
+            for (int w = 0; w < 4; ++w) v[w] = (w == off ? a0 : 0.0) + (w == off + 1 ? a1 : 0.0);
+            seg_scan4(v, lane, k.fl);
+            const int lk = k.valid ? cur.c - kb : -1;
+            if (k.last && lk >= 0 && lk < L) {
+                cnt[lk][0] += __popcll(bd0 & seg);
+                cnt[lk][1] += __popcll(bd1 & seg);
+                cnt[lk][2] += __popcll(bd2 & seg);
+#pragma unroll
+                for (int w = 0; w < 4; ++w) S[lk][w] += v[w];
             }
+            wave_sync();
         }
-        __syncthreads();
+    }
+
+    // pass 2: stable ranks and the scatter of this wave's own particles,
+    // kBatch chunks per memory round trip
+    for (int bb = olo; bb < ohi; bb += 64 * kBatch) {
+        int cc[kBatch], cd[kBatch], tc[kBatch], tid[kBatch];
+        R tf[kBatch], tx[kBatch], ty[kBatch], tz[kBatch];
+#pragma unroll
+        for (int q = 0; q < kBatch; ++q) {
+            const int j = bb + 64 * q + lane;
+            const bool v = j < ohi;
+            cc[q] = v ? cell[j] : 0x7fffffff;
+            cd[q] = v ? (int)b.tcode[j] : 5;
+            tc[q] = v ? b.tcell[j] : 0;
+            tid[q] = v ? b.id[par][j] : 0;
+            tf[q] = v ? b.tfrac[j] : R(0);
+            tx[q] = v ? b.tux[j] : R(0);
+            ty[q] = v ? b.tuy[j] : R(0);
+            tz[q] = v ? b.tuz[j] : R(0);
+        }
+#pragma unroll
+        for (int q = 0; q < kBatch; ++q) {
+            const int base = bb + 64 * q;
+            if (base >= ohi) break;
+            const int c_cur = cc[q], code = cd[q];
+            const int i = base + lane;
+            const Chunk k = chunk_info(c_cur, i, ohi, lane);
+            const int d = (code & 3) - 1;
+            const unsigned long long seg = lanes_le(lane) & ~lanes_lt(k.fl);
+            const unsigned long long bd0 = __ballot(k.valid && d == -1);
+            const unsigned long long bd1 = __ballot(k.valid && d == 0);
+            const unsigned long long bd2 = __ballot(k.valid && d == 1);
+            const unsigned long long mine = d < 0 ? bd0 : (d == 0 ? bd1 : bd2);
+            const int lk = k.valid ? c_cur - kb : -1;
+            const bool in = k.valid && c_cur >= 0 && c_cur <= N && lk >= 1 && lk + 1 < L;
+            int r = 0;
+            if (in) r = rk[lk][d + 1] + __popcll(mine & lanes_lt(lane) & ~lanes_lt(k.fl));
+            wave_sync();
+            if (in && k.last) {
+                rk[lk][0] += __popcll(bd0 & seg);
+                rk[lk][1] += __popcll(bd1 & seg);
+                rk[lk][2] += __popcll(bd2 & seg);
+            }
+            if (in) {
+                int ni;
+                if (d < 0) ni = st[c_cur] - cnt[lk - 1][2] + r;
+                else if (d == 0) ni = st[c_cur] + cnt[lk][0] + r;
+                else ni = st[c_cur + 1] - cnt[lk][2] + cnt[lk + 1][0] + r;
+                if (ni >= 0 && ni < np) {
+                    b.cell[po][ni] = tc[q];
+                    b.frac[po][ni] = tf[q];
+                    b.ux[po][ni] = tx[q];
+                    b.uy[po][ni] = ty[q];
+                    b.uz[po][ni] = tz[q];
+                    b.id[po][ni] = tid[q];
+                } else {
+                    b.err[0] = 1;
+                }
+            }
+            wave_sync();
+        }
     }
 
     // new cell starts and J' of this wave's nodes
