@@ -171,6 +171,15 @@ int fdtd_read_probes(fdtd_ctx *ctx, int sim_id, double *out, int64_t len);
 /* snapshots as snap_rows x (N+1) (Ex_History, MasterController.py:160) */
 int fdtd_read_snapshots(fdtd_ctx *ctx, int sim_id, double *out, int64_t len);
 
+/* ---- spectra ---------------------------------------------------------- */
+/* FP64 DFT X(f) = sum_t x_t exp(-2 pi i f t dt) dt of probe `probe` of each
+ * listed simulation, computed on the device from the resident probe series
+ * (the per-frequency sums behind TransformHandler.RefTester,
+ * TransformHandler.py:34-78, and the two-pass reflection protocol of
+ * Solver_Engine.py:275-371).  out: nsims x nf x (re, im).  Synchronous. */
+int fdtd_probe_spectra(fdtd_ctx *ctx, int nsims, const int *sim_ids, int probe, const double *freqs, int nf,
+                       double dt, double *out);
+
 /* ---- diagnostics ------------------------------------------------------ */
 const char *fdtd_last_error(void);
 /* Device time (ms) spent in stencil launches since the last reset, from HIP

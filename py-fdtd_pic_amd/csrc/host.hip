@@ -18,6 +18,9 @@ hipError_t launch_stencil(const SimDev<R>* sims, const int2* tiles, int ntiles, 
                           long long gstep0, int nsteps, int K, int S, int kind, hipStream_t st,
                           hipEvent_t ev0, hipEvent_t ev1);
 template <typename R>
+hipError_t launch_dft(const R* const* series, const int* lens, int nseries, const double* freqs, int nf, double dt,
+                      double* out, hipStream_t st);
+template <typename R>
 hipError_t launch_pic(const SimDev<R>* sims, const BeamDev<R>* beams, const int2* push_blocks, int npush,
                       const int2* items, int nitems, long long launch_idx, hipStream_t st, hipEvent_t e0,
                       hipEvent_t e1, hipEvent_t e2, hipEvent_t e3);
@@ -221,7 +224,7 @@ int fdtd_ctx_destroy(fdtd_ctx* c) {
         if (s) { if (s->block) (void)hipFree(s->block); delete s; }
     for (BeamHost* b : c->beams)
         if (b) { if (b->block) (void)hipFree(b->block); delete b; }
-    for (void* p : {(void*)c->d_beams, (void*)c->d_pblocks, (void*)c->d_pitems, (void*)c->d_err})
+    for (void* p : {(void*)c->d_beams, (void*)c->d_pblocks, (void*)c->d_pitems, (void*)c->d_err, c->d_scratch})
         if (p) (void)hipFree(p);
     if (c->d_sims) (void)hipFree(c->d_sims);
     if (c->d_tiles) (void)hipFree(c->d_tiles);
@@ -893,3 +896,58 @@ int pic_read(fdtd_ctx* c, int id, double* x, double* u, int8_t* alive, int64_t n
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------------
+// spectra of probe series (FP64 DFT on the device)
+// ------------------------------------------------------------------------
+template <typename R>
+static int spectra_t(fdtd_ctx* c, int nsims, const int* ids, int probe, const double* freqs, int nf, double dt,
+                     double* out) {
+    std::vector<const R*> ptrs(nsims);
+    std::vector<int> lens(nsims);
+    for (int k = 0; k < nsims; ++k) {
+        SimHost* s = get_sim(c, ids[k]);
+        if (!s) return fdtd_fail(FDTD_E_ARG, "bad sim id " + std::to_string(ids[k]));
+        if (probe < 0 || probe >= s->d.nprobe) return fdtd_fail(FDTD_E_ARG, "probe index out of range");
+        SimDev<R> d;
+        fill_dev<R>(*s, d);
+        ptrs[k] = d.probe_out + (size_t)probe * s->d.steps;
+        lens[k] = s->d.steps;
+    }
+    const size_t o_lens = align_up(sizeof(void*) * nsims, 256);
+    const size_t o_freqs = o_lens + align_up(sizeof(int) * nsims, 256);
+    const size_t o_out = o_freqs + align_up(sizeof(double) * nf, 256);
+    const size_t need = o_out + sizeof(double) * 2 * (size_t)nsims * nf;
+    if (need > c->d_scratch_cap) {
+        if (c->d_scratch) HIPCHK(hipFree(c->d_scratch));
+        c->d_scratch = nullptr;
+        c->d_scratch_cap = 0;
+        HIPCHK(hipMalloc(&c->d_scratch, need));
+        c->d_scratch_cap = need;
+    }
+    char* base = (char*)c->d_scratch;
+    const R** d_ptrs = (const R**)base;
+    int* d_lens = (int*)(base + o_lens);
+    double* d_freqs = (double*)(base + o_freqs);
+    double* d_out = (double*)(base + o_out);
+    HIPCHK(hipMemcpyAsync(d_ptrs, ptrs.data(), sizeof(void*) * nsims, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(d_lens, lens.data(), sizeof(int) * nsims, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(d_freqs, freqs, sizeof(double) * nf, hipMemcpyHostToDevice, c->st));
+    for (int k0 = 0; k0 < nsims; k0 += 32768) {
+        const int nk = std::min(32768, nsims - k0);
+        hipError_t e = launch_dft<R>(d_ptrs + k0, d_lens + k0, nk, d_freqs, nf, dt, d_out + 2 * (size_t)k0 * nf, c->st);
+        if (e != hipSuccess) return fdtd_fail(FDTD_E_HIP, std::string("dft launch: ") + hipGetErrorString(e));
+    }
+    HIPCHK(hipMemcpyAsync(out, d_out, sizeof(double) * 2 * (size_t)nsims * nf, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return FDTD_OK;
+}
+
+extern "C" int fdtd_probe_spectra(fdtd_ctx* c, int nsims, const int* sim_ids, int probe, const double* freqs, int nf,
+                                  double dt, double* out) {
+    if (!c || nsims < 0 || (nsims > 0 && (!sim_ids || !out)) || nf < 0 || (nf > 0 && !freqs))
+        return fdtd_fail(FDTD_E_ARG, "bad argument");
+    if (nsims == 0 || nf == 0) return FDTD_OK;
+    return c->prec == FDTD_F64 ? spectra_t<double>(c, nsims, sim_ids, probe, freqs, nf, dt, out)
+                               : spectra_t<float>(c, nsims, sim_ids, probe, freqs, nf, dt, out);
+}

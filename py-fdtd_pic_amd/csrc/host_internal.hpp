@@ -81,6 +81,8 @@ struct fdtd_ctx {
     size_t throttle_head = 0;
     long long throttle_launches = 0;
     int32_t* d_err = nullptr;                // index-check flag of the sort kernel
+    void* d_scratch = nullptr;               // spectra: pointer table, frequencies, results
+    size_t d_scratch_cap = 0;
 };
 
 int fdtd_fail(int code, const std::string& msg);

@@ -106,6 +106,31 @@ def config4_instances(seed=4, count=4096, steps=10000):
     return out
 
 
+def config5_requests(seed=5, count=2000, rate=200.0):
+    """Served stream: Poisson arrivals (mean gap 1/rate s), config-3-style
+    requests with Nz = min(262,144, round(LogNormal(ln 8192, 1))) (at least
+    2,000), Np ~ U_int[0, 1e6] electrons over cells [0.458 Nz, 0.61 Nz) (the
+    config-3 beam window), T ~ U_int[5,000, 50,000] steps and the config-1
+    slab at [1000, 1400).  Returns (arrival times in s, parameter dicts)."""
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, count)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    Nz = np.minimum(262144, np.round(rng.lognormal(math.log(8192), 1.0, count))).astype(int)
+    Nz = np.maximum(Nz, 2000)
+    Np = rng.integers(0, 1_000_001, count)
+    T = rng.integers(5000, 50_001, count)
+    out = []
+    for i in range(count):
+        N = int(Nz[i])
+        p = base(N, int(T[i]), slab_start=1000, slab_stop=1400)
+        if Np[i] > 0:
+            lo, hi = int(0.458 * N), int(0.61 * N)
+            p["beam"] = dict(seed=seed * 100003 + i, n=int(Np[i]), cell_lo=lo, cell_hi=hi,
+                             v_drift=0.1 * C0, v_th=1e-3 * C0, density=1e16)
+        out.append(p)
+    return arrivals, out
+
+
 def bytes_per_cell_update(p, precision="f32"):
     """Algorithmic HBM bytes per cell-update of the one-step streaming schedule.
 

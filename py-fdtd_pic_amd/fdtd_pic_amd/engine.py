@@ -209,6 +209,15 @@ class Context:
         check(self.lib.fdtd_kernel_timing(self.h, k, C.byref(ms), C.byref(n)))
         return ms.value, n.value
 
+    def spectra(self, sims, probe, freqs, dt):
+        """Complex FP64 DFT (len(sims) x len(freqs)) of probe `probe` of each sim, on the device."""
+        ids = np.ascontiguousarray([s.id for s in sims], dtype=np.int32)
+        f = as_f64(freqs)
+        out = np.zeros((len(ids), len(f), 2))
+        check(self.lib.fdtd_probe_spectra(self.h, len(ids), ids.ctypes.data_as(C.POINTER(C.c_int)), int(probe),
+                                          dptr(f), len(f), float(dt), dptr(out)))
+        return out[..., 0] + 1j * out[..., 1]
+
     def device_bytes(self):
         v = C.c_int64()
         check(self.lib.fdtd_device_bytes(self.h, C.byref(v)))

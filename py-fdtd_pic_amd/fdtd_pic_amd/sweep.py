@@ -1,0 +1,212 @@
+"""Many independent simulations on one GPU: batched sweeps and served streams.
+
+The reference runs one simulation at a time: LoopedSim
+(MasterController.py:530-569) loops an input-frequency sweep over full
+Controller runs, each a two-pass reflection measurement
+(IntegratorLinLor1D, Solver_Engine.py:275-371: pass 0 without the slab,
+pass 1 with it) followed by the spectral stage (TransformHandler.RefTester,
+TransformHandler.py:34-78; MasterController.results :472-505).  Here a Job
+is one such point (both passes) and a Scheduler keeps many jobs resident:
+
+* field-only passes share one context (temporally blocked, K cells per lane,
+  S steps per launch); passes with a beam share a second context on its own
+  stream (one step per launch, particle kernels batched over every beam);
+* a job is admitted when its device bytes fit the HBM budget and a slot is
+  free; its state stays on the device until both passes finish, then its
+  spectra are computed on the device (FP64 DFT of the probe series) and the
+  simulations are freed;
+* contexts advance in slices that are multiples of S, so a job's launch
+  boundaries relative to its own start do not depend on what else runs:
+  the result of a job is bitwise the same alone and in a mix;
+* an autotuner picks (K, S) for the field context on the admitted set
+  (timed trial launches, then every simulation is reset).
+
+Spectra (two-pass protocol): R(f) = |X1_r - X0_r| / |X0_r| at the reflection
+probe, T(f) = |X1_t| / |X0_t| at the transmission probe.
+"""
+import math
+import time
+
+import numpy as np
+
+from . import configs, engine
+
+DEFAULT_FREQS = np.linspace(100e12, 300e12, 256)
+TILINGS = [(1, 16), (1, 24), (2, 32), (2, 48), (4, 64), (4, 96)]
+_TUNE_CACHE = {}
+
+
+def pass_params(p, with_slab):
+    q = dict(p)
+    if not with_slab:
+        q["slab_stop"] = q["slab_start"]
+    return q
+
+
+def estimate_bytes(p, precision="f32"):
+    """Device bytes of one job (both passes), from the library's block layout."""
+    w = 8 if precision == "f64" else 4
+    N, T = p["N"], p["steps"]
+    ns = max(p["slab_stop"] - p["slab_start"], 0)
+    strip = 2 * p["pml_width"] + 4
+    per_pass = w * (8 * strip + 4 * ns + 2 * T + 2 * (4 * (N + 1) + 5 * ns + 2 * strip)
+                    + len(p.get("probes", [])) * T) + 64 * 256
+    slab_pass = per_pass
+    vac_pass = per_pass - w * (4 * ns + 10 * ns)
+    total = slab_pass + vac_pass
+    if p.get("beam"):
+        n = p["beam"]["n"]
+        beam = 2 * (4 * n + 4 * w * n + 4 * n + 4 * (N + 2)) + 4 * n + 6 * w * n + n + w * (N + 1)
+        total += 2 * beam
+    return int(total)
+
+
+class Job:
+    """One sweep point: two passes of a parameter dict (see configs)."""
+
+    def __init__(self, jid, p, arrival=0.0):
+        self.id, self.p, self.arrival = jid, p, float(arrival)
+        self.sims = []           # [vacuum pass, slab pass]
+        self.ctx = None
+        self.admitted = self.finished = None
+        self.R = self.T = None
+        self.X = None            # (X0_r, X1_r, X0_t, X1_t)
+        self.bytes = 0
+
+    @property
+    def has_beam(self):
+        return bool(self.p.get("beam"))
+
+
+class Scheduler:
+    def __init__(self, precision="f32", budget_bytes=64 << 30, max_slots=4096, freqs=None, K=None, S=None,
+                 autotune=False, slice_launches=8, beam_slice_steps=64):
+        self.precision = precision
+        self.budget = int(budget_bytes)
+        self.max_slots = int(max_slots)
+        self.freqs = DEFAULT_FREQS if freqs is None else np.asarray(freqs, dtype=np.float64)
+        self.field = engine.Context(precision)
+        self.beam = engine.Context(precision, K=2, S=1)
+        self.K, self.S = K or 2, S or (48 if precision == "f32" else 32)
+        self.field.set_tiling(self.K, self.S)
+        self.autotune_enabled = autotune
+        self.slice_launches = int(slice_launches)
+        self.beam_slice_steps = int(beam_slice_steps)
+        self.active, self.done = [], []
+        self.used = 0
+        self.tuned = False
+
+    # -- admission -------------------------------------------------------
+    def fits(self, job):
+        return len(self.active) < self.max_slots and self.used + estimate_bytes(job.p, self.precision) <= self.budget
+
+    def admit(self, job, now=None):
+        ctx = self.beam if job.has_beam else self.field
+        job.bytes = estimate_bytes(job.p, self.precision)
+        beam = None
+        if job.has_beam:
+            x, u, q, m, w = configs.beam_inputs(job.p)
+            beam = engine.Beam(x, u, q, m, w, job.p["Sc"] * job.p["dz"] / configs.C0, job.p["dz"])
+        for with_slab in (False, True):
+            spec = engine.physical_spec(pass_params(job.p, with_slab))
+            job.sims.append(ctx.add(spec, beam=beam))
+        job.ctx = ctx
+        job.admitted = time.perf_counter() if now is None else now
+        self.used += job.bytes
+        self.active.append(job)
+
+    # -- tuning ----------------------------------------------------------
+    def autotune(self, trial_launches=4):
+        """Time each tiling on the admitted field passes, keep the fastest, reset them."""
+        field_jobs = [j for j in self.active if not j.has_beam]
+        if not field_jobs:
+            return self.K, self.S
+        cells = sum(s.spec.N for j in field_jobs for s in j.sims)
+        key = (self.precision, int(math.log2(max(cells, 1))))
+        if key in _TUNE_CACHE:
+            best = _TUNE_CACHE[key]
+        else:
+            rates = {}
+            for K, S in TILINGS:
+                self.field.set_tiling(K, S)
+                self.field.run(S)                     # warm-up launch (tile list rebuild)
+                self.field.sync()
+                t0 = time.perf_counter()
+                self.field.run(S * trial_launches)
+                self.field.sync()
+                rates[(K, S)] = S * trial_launches / (time.perf_counter() - t0)
+            best = max(rates, key=rates.get)
+            _TUNE_CACHE[key] = best
+            for j in field_jobs:
+                for s in j.sims:
+                    s.reset()
+        self.K, self.S = best
+        self.field.set_tiling(*best)
+        self.tuned = True
+        return best
+
+    # -- stepping --------------------------------------------------------
+    def step(self):
+        """Advance both contexts by one slice (asynchronously, on their own streams)."""
+        if any(not j.has_beam for j in self.active):
+            self.field.run(self.S * self.slice_launches)
+        if any(j.has_beam for j in self.active):
+            self.beam.run(self.beam_slice_steps)
+        self.field.sync()
+        self.beam.sync()
+
+    def collect(self, now=None):
+        """Finish jobs whose two passes are complete: spectra on the device, then free."""
+        fin = [j for j in self.active if all(s.progress() >= s.spec.steps for s in j.sims)]
+        if not fin:
+            return []
+        for ctx in (self.field, self.beam):
+            mine = [j for j in fin if j.ctx is ctx]
+            if not mine:
+                continue
+            sims = [s for j in mine for s in j.sims]
+            dt = mine[0].p["Sc"] * mine[0].p["dz"] / configs.C0
+            Xr = ctx.spectra(sims, 0, self.freqs, dt)
+            Xt = ctx.spectra(sims, 1, self.freqs, dt)
+            for k, j in enumerate(mine):
+                X0r, X1r, X0t, X1t = Xr[2 * k], Xr[2 * k + 1], Xt[2 * k], Xt[2 * k + 1]
+                j.X = (X0r, X1r, X0t, X1t)
+                j.R = np.abs(X1r - X0r) / np.abs(X0r)
+                j.T = np.abs(X1t) / np.abs(X0t)
+        t = time.perf_counter() if now is None else now
+        for j in fin:
+            j.finished = t
+            for s in j.sims:
+                s.remove()
+            j.sims = []
+            self.used -= j.bytes
+            self.active.remove(j)
+            self.done.append(j)
+        return fin
+
+    def run_batch(self, jobs):
+        """Admit everything that fits, tune, run to completion, admitting the rest as room frees."""
+        pending = list(jobs)
+        while pending or self.active:
+            while pending and (self.fits(pending[0]) or not self.active):
+                self.admit(pending.pop(0))
+            if self.autotune_enabled and not self.tuned:
+                self.autotune()
+            self.step()
+            self.collect()
+        return self.done
+
+    def close(self):
+        self.field.close()
+        self.beam.close()
+
+
+def run_alone(p, precision="f32", K=None, S=None, freqs=None):
+    """One job by itself (the plain per-simulation loop)."""
+    sch = Scheduler(precision, K=K, S=S, freqs=freqs)
+    try:
+        job = Job(0, p)
+        sch.run_batch([job])
+        return job
+    finally:
+        sch.close()
