@@ -180,6 +180,26 @@ int fdtd_read_snapshots(fdtd_ctx *ctx, int sim_id, double *out, int64_t len);
 int fdtd_probe_spectra(fdtd_ctx *ctx, int nsims, const int *sim_ids, int probe, const double *freqs, int nf,
                        double dt, double *out);
 
+/* ---- single reference update kernels ---------------------------------- */
+/* One BaseFDTD11 per-step function on the device, FP64, in the reference's
+ * expression order (no FMA contraction), for callers that drive the
+ * per-step functions one by one (Solver_Engine.py:297-316 style).  arrays
+ * are host buffers (copied in, outputs copied back); ranges holds two
+ * half-open index ranges [lo0,hi0) [lo1,hi1) (empty when hi <= lo).
+ *   op  function (BaseFDTD11.py)        arrays                                scalars
+ *   0   ADE_ExUpdate :663-669           Ex*, Hy, Jx, UpExMat, den_Exdz        -
+ *   1   ADE_HyUpdate :640-656           Hy*, Ex, UpHySelf, UpHyMat, den_Hydz  -
+ *   2   CPML_Psi_e_Update :364-376      Ex*, psi_Ex*, Hy, beX, ceX, Cb        -
+ *   3   CPML_Psi_m_Update :381-393      Hy*, psi_Hy*, Ex, bmY, cmY, C2        -
+ *   4   ADE_DxUpdate :750-760           Dx*, Hy, den_Exdz                     dt/dz
+ *   5   ADE_ExCreate :712-725           Ex*, Dx, P                            eps0
+ *   6   ADE_PolarisationCurrent_Ex :609-633  P*, P_prev, Ex                  A, B, C
+ *   7   AcubicFinder :803-853           Acubic*, Dx                           cub, qua, one, eps0
+ *   8   NonLinExUpdate :858-877         Ex*, Dx, Acubic                       eps0, chi3, sqrt(1.2)
+ * (* = written).  Needs an FP64 context. */
+int fdtd_step_op(fdtd_ctx *ctx, int op, int narr, double *const *arrays, const int64_t *lens, int nscal,
+                 const double *scalars, const int32_t *ranges);
+
 /* ---- diagnostics ------------------------------------------------------ */
 const char *fdtd_last_error(void);
 /* Device time (ms) spent in stencil launches since the last reset, from HIP

@@ -290,12 +290,27 @@ def ADE_PolarisationCurrent_Ex(V, P, C_V, C_P, counts):
 
 
 def ADE_TempPolCurr(V, P, C_V, C_P):
-    """BaseFDTD11.ADE_TempPolCurr (:487-538): history shift by copy (host bookkeeping)."""
+    """BaseFDTD11.ADE_TempPolCurr (:487-538): history shift of P and E by copy
+    (the reference goes through lists for the same effect) and of Dx, Hy, Jx,
+    psi by reference; returns the reference's 12-tuple.  The reference's
+    breakpoint() trap on an unchanged polarisation (:535-537) is not kept."""
     V.tempTempVarPol = np.array(V.tempVarPol, copy=True)
     V.tempVarPol = np.array(V.polarisationCurr, copy=True)
+    V.tempTempVarDx = V.tempVarDx
+    V.tempVarDx = V.Dx
     V.tempTempVarE = np.array(V.tempVarE, copy=True)
     V.tempVarE = np.array(V.Ex, copy=True)
-    return V.tempTempVarPol, V.tempVarPol, V.tempVarE, V.tempTempVarE
+    V.tempTempVarHy = V.tempVarHy
+    V.tempVarHy = V.Hy
+    V.tempTempVarJx = V.tempVarJx
+    V.tempVarJx = V.Jx
+    C_V.tempTempVarPsiEx = C_V.tempVarPsiEx
+    C_V.tempVarPsiEx = C_V.psi_Ex
+    C_V.tempTempVarPsiHy = C_V.tempVarPsiHy
+    C_V.tempVarPsiHy = C_V.psi_Hy
+    return (V.tempTempVarPol, V.tempVarPol, V.tempVarE, V.tempTempVarE, V.tempTempVarHy, V.tempVarHy,
+            V.tempTempVarJx, V.tempVarJx, C_V.tempTempVarPsiEx, C_V.tempVarPsiEx, C_V.tempTempVarPsiHy,
+            C_V.tempVarPsiHy)
 
 
 def AcubicFinder(V, P):

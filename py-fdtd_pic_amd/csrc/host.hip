@@ -17,6 +17,12 @@ template <typename R, bool KAHAN>
 hipError_t launch_stencil(const SimDev<R>* sims, const int2* tiles, int ntiles, long long launch_idx,
                           long long gstep0, int nsteps, int K, int S, int kind, hipStream_t st,
                           hipEvent_t ev0, hipEvent_t ev1);
+struct OpArgs {
+    double* a[8];
+    double s[4];
+    int lo[2], hi[2];
+};
+hipError_t launch_step_op(int op, const OpArgs& g, hipStream_t st);
 template <typename R>
 hipError_t launch_dft(const R* const* series, const int* lens, int nseries, const double* freqs, int nf, double dt,
                       double* out, hipStream_t st);
@@ -950,4 +956,64 @@ extern "C" int fdtd_probe_spectra(fdtd_ctx* c, int nsims, const int* sim_ids, in
     if (nsims == 0 || nf == 0) return FDTD_OK;
     return c->prec == FDTD_F64 ? spectra_t<double>(c, nsims, sim_ids, probe, freqs, nf, dt, out)
                                : spectra_t<float>(c, nsims, sim_ids, probe, freqs, nf, dt, out);
+}
+
+// ------------------------------------------------------------------------
+// single reference update kernels (compatibility path of BaseFDTD11)
+// ------------------------------------------------------------------------
+namespace {
+struct OpSpec {
+    int narr, nscal, nout;
+    int prev, next;        // reads index n-1 / n+1
+};
+// indexed by op: EX_UPDATE HY_UPDATE PSI_E PSI_M DX_UPDATE EX_CREATE POLARISE ACUBIC NONLIN_EX
+const OpSpec kOps[9] = {{5, 0, 1, 1, 0}, {5, 0, 1, 0, 1}, {6, 0, 2, 1, 0}, {6, 0, 2, 0, 1}, {3, 1, 1, 1, 0},
+                        {3, 1, 1, 0, 0}, {3, 3, 1, 0, 0}, {2, 4, 1, 0, 0}, {3, 3, 1, 0, 0}};
+}  // namespace
+
+extern "C" int fdtd_step_op(fdtd_ctx* c, int op, int narr, double* const* arrays, const int64_t* lens, int nscal,
+                            const double* scalars, const int32_t* ranges) {
+    if (!c || op < 0 || op > 8 || !arrays || !lens || !ranges) return fdtd_fail(FDTD_E_ARG, "bad argument");
+    if (c->prec != FDTD_F64) return fdtd_fail(FDTD_E_ARG, "the single-op path runs in an FP64 context");
+    const OpSpec& o = kOps[op];
+    if (narr != o.narr || nscal != o.nscal || (nscal > 0 && !scalars))
+        return fdtd_fail(FDTD_E_ARG, "op " + std::to_string(op) + " takes " + std::to_string(o.narr) + " arrays and " +
+                                         std::to_string(o.nscal) + " scalars");
+    int64_t minlen = INT64_MAX, total = 0;
+    for (int k = 0; k < narr; ++k) {
+        if (!arrays[k] || lens[k] < 0) return fdtd_fail(FDTD_E_ARG, "null or negative-length array");
+        minlen = std::min(minlen, lens[k]);
+        total += lens[k];
+    }
+    if (minlen > INT32_MAX) return fdtd_fail(FDTD_E_ARG, "arrays too long");
+    for (int r = 0; r < 2; ++r) {
+        const int lo = ranges[2 * r], hi = ranges[2 * r + 1];
+        if (hi <= lo) continue;
+        if (lo - o.prev < 0 || hi + o.next > minlen) return fdtd_fail(FDTD_E_ARG, "index range outside the arrays");
+    }
+    std::vector<size_t> off(narr);
+    size_t need = 0;
+    for (int k = 0; k < narr; ++k) { off[k] = need; need = align_up(need + sizeof(double) * lens[k], 256); }
+    if (need > c->d_scratch_cap) {
+        if (c->d_scratch) HIPCHK(hipFree(c->d_scratch));
+        c->d_scratch = nullptr;
+        c->d_scratch_cap = 0;
+        HIPCHK(hipMalloc(&c->d_scratch, need));
+        c->d_scratch_cap = need;
+    }
+    OpArgs g;
+    std::memset(&g, 0, sizeof(g));
+    for (int k = 0; k < narr; ++k) {
+        g.a[k] = (double*)((char*)c->d_scratch + off[k]);
+        if (lens[k]) HIPCHK(hipMemcpyAsync(g.a[k], arrays[k], sizeof(double) * lens[k], hipMemcpyHostToDevice, c->st));
+    }
+    for (int k = 0; k < nscal; ++k) g.s[k] = scalars[k];
+    for (int r = 0; r < 2; ++r) { g.lo[r] = ranges[2 * r]; g.hi[r] = ranges[2 * r + 1]; }
+    hipError_t e = launch_step_op(op, g, c->st);
+    if (e != hipSuccess) return fdtd_fail(FDTD_E_HIP, std::string("op launch: ") + hipGetErrorString(e));
+    for (int k = 0; k < o.nout; ++k)
+        if (lens[k]) HIPCHK(hipMemcpyAsync(arrays[k], g.a[k], sizeof(double) * lens[k], hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    (void)total;
+    return FDTD_OK;
 }
