@@ -43,13 +43,18 @@ def parse():
     ap.add_argument("--pic-particles", type=int, default=1_000_000)
     ap.add_argument("--pic-only", action="store_true", help="print only the config 3 object")
     ap.add_argument("--pic-group", type=int, default=0, help="cells per sort/deposit wave (0: automatic)")
+    ap.add_argument("--workload", default="config2", choices=["config2", "config4", "config5"],
+                    help="config4: batched sweep vs the per-simulation loop; config5: served trace replay")
+    ap.add_argument("--c4-count", type=int, default=4096, help="config 4 instances")
+    ap.add_argument("--c4-sample", type=int, default=24, help="instances timed in the per-simulation loop")
+    ap.add_argument("--c5-requests", type=int, default=8, help="requests of the config 5 trace replayed")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 256 MiB device copy (FETCH_SIZE/WRITE_SIZE calibration under rocprofv3 --pmc)")
     return ap.parse_args()
 
 
 # Best measured tilings on MI355X for config 2 (cells per lane, steps per launch).
-DEFAULT_TILING = {"f32": (2, 48), "f64": (2, 32)}
+DEFAULT_TILING = {"f32": (2, 48), "f64": (2, 48)}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "stencil_tiles_config2_hbm_traffic.txt")
 
 
@@ -162,6 +167,55 @@ def pic_cpu_baseline(p, x, u, q, m, w, steps=40):
                 sample=f"config3, first {steps} steps of {len(x)} particles, NumPy FP64 oracle, {secs:.1f} s")
 
 
+def config4_bench(a):
+    """Config 4: 4,096 two-pass sweep points (1,024 with beams) batched on one
+    GPU, against the plain per-simulation loop (each job alone, one after the
+    other) timed on a seeded sample of the same jobs and scaled by work."""
+    from fdtd_pic_amd import configs, serve, sweep
+    inst = configs.config4_instances()[:a.c4_count]
+    jobs = [sweep.Job(i, p) for i, p in enumerate(inst)]
+    sch = sweep.Scheduler("f32", budget_bytes=200 << 30, autotune=True)
+    t0 = time.perf_counter()
+    sch.run_batch(jobs)
+    wall = time.perf_counter() - t0
+    tiling = (sch.K, sch.S)
+    sch.close()
+    cells = sum(serve.work(p)[0] for p in inst)
+    pushes = sum(serve.work(p)[1] for p in inst)
+    rng = __import__("numpy").random.default_rng(44)
+    sample = sorted(rng.choice(len(inst), size=min(a.c4_sample, len(inst)), replace=False).tolist())
+    t1 = time.perf_counter()
+    for i in sample:
+        sweep.run_alone(inst[i], "f32")
+    alone = time.perf_counter() - t1
+    # the per-job loop's time scales with the work of the jobs it runs
+    w = lambda p: serve.work(p)[0] + 4 * serve.work(p)[1]
+    scale = sum(w(p) for p in inst) / sum(w(inst[i]) for i in sample)
+    est = alone * scale
+    return {"metric": "config4 sweep: batched vs per-simulation GPU loop", "jobs": len(inst),
+            "beam_jobs": sum(1 for p in inst if p.get("beam")), "batched_s": round(wall, 3),
+            "gcell_updates_per_s": round(cells / wall / 1e9, 3),
+            "gparticle_pushes_per_s": round(pushes / wall / 1e9, 3), "autotuned_tiling": list(tiling),
+            "per_sim_loop_sample": len(sample), "per_sim_loop_sample_s": round(alone, 3),
+            "per_sim_loop_est_s": round(est, 3), "speedup_vs_per_sim_loop": round(est / wall, 2),
+            "spectra": "R(f), T(f) at 256 frequencies per job, FP64 DFT on the device"}
+
+
+def config5_bench(a):
+    """Config 5: the first requests of the seeded Poisson trace, replayed in
+    real time (overload: the full trace offers ~14x what one GPU sustains)."""
+    from fdtd_pic_amd import configs, serve
+    arr, ps = configs.config5_requests()
+    n = a.c5_requests
+    srv = serve.Server("f32", budget_bytes=200 << 30)
+    jobs, wall = srv.replay(arr[:n], ps[:n])
+    srv.close()
+    out = serve.summarize(jobs, wall)
+    out.update(metric="config5 served stream (trace prefix replay)",
+               trace=f"first {n} of 2000 requests, Poisson 200/s, seed 5")
+    return out
+
+
 def cpu_run(cpu_ref, m, n, threads):
     return cpu_ref.CpuSim(m).run(0, n, threads)
 
@@ -177,6 +231,12 @@ def main():
         return
     if a.pic_only:
         print(json.dumps(pic_bench(a)), flush=True)
+        return
+    if a.workload == "config4":
+        print(json.dumps(config4_bench(a)), flush=True)
+        return
+    if a.workload == "config5":
+        print(json.dumps(config5_bench(a)), flush=True)
         return
     from fdtd_pic_amd import configs, engine
 

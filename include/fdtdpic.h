@@ -61,8 +61,8 @@ extern "C" {
 
 /* kernel kinds for fdtd_kernel_timing */
 #define FDTD_KERNEL_STENCIL 0  /* field stencil (E/H/ADE/Kerr/CPML) */
-#define FDTD_KERNEL_PUSH 1     /* particle gather + Boris push */
-#define FDTD_KERNEL_SORT 2     /* particle sort + CIC deposit */
+#define FDTD_KERNEL_PUSH 1     /* particle gather + Boris push + per-cell counts, ranks, CIC sums */
+#define FDTD_KERNEL_SORT 2     /* particle scatter (stable re-sort) + J', cell starts */
 
 typedef struct fdtd_ctx fdtd_ctx;
 
@@ -152,7 +152,7 @@ typedef struct {
     double dt, dz;      /* time step (s) and cell size (m) of the simulation */
     const double *x;    /* positions in cell units (E node k at x = k), in [2, N-2) */
     const double *u;    /* gamma*v/c, n x 3 (ux, uy, uz) */
-    int32_t group;      /* cells per sort/deposit wave, 1..61 (0: picked from the density) */
+    int32_t group;      /* cells per push/count group, 1..64 (0: picked from the density) */
 } fdtd_beam_desc_t;
 
 /* Attaches a beam to a simulation created with has_current = 1 that has not

@@ -36,11 +36,14 @@ enum SlabKind { SLAB_NONE = 0, SLAB_DIEL = 1, SLAB_LINEAR = 2, SLAB_KERR_REF = 3
 // Particle beam of one simulation.  Particles are kept sorted by cell
 // (stable), structure of arrays, ping-ponged per step like the fields;
 // start[c] is the first particle of cell c (start[N+1] = np).  Live
-// particles occupy cells [2, N-3]; cells 1 and N-2 park absorbed ones.
-constexpr int kPicGMax = 61;        // cells per sort/deposit wave (LDS tables of kPicGMax+3)
+// particles occupy cells [2, N-3]; cells 1 and N-2 park absorbed ones, which
+// keep their index from then on.
+constexpr int kPicGMax = 64;        // cells per push/count group (LDS tables)
 template <typename R>
 struct BeamDev {
     int32_t sim, N, np, G;
+    int32_t nwaves;      // push/count waves of this beam (grid-stride over cell groups)
+    int32_t nnode;       // scatter node blocks of this beam (grid-stride over nodes)
     R a;                 // q dt / (2 m c)
     R jscale;            // Z0 q w c: J' of one particle per unit u_x/gamma
     R Sc;                // c dt / dz
@@ -48,10 +51,15 @@ struct BeamDev {
     R *frac[2], *ux[2], *uy[2], *uz[2];
     int32_t* id[2];
     int32_t* start[2];   // N+2
-    // per-step records at the old (sorted) index, written by the push
+    int32_t* range[2];   // [cmin, cmax] of the live particles' cells (empty: cmin > cmax)
+    int32_t* urange[2];  // union of the live ranges so far (where J and start[] can change)
+    // per-step records at the old (sorted) index, push/count -> scatter
     int32_t* tcell;
-    R *tfrac, *tux, *tuy, *tuz, *ta0, *ta1;
-    uint8_t* tcode;      // (d+1) | (node offset + 1) << 2, d = new cell - old cell
+    R *tfrac, *tux, *tuy, *tuz;
+    int32_t* trank;      // rank << 2 | (d + 1), d = new cell - old cell, rank among (cell, d)
+    // per-cell results of the push/count pass, valid on the live range +-1
+    int32_t *cup, *cdown;  // particles moving up / down out of the cell
+    double* S;             // 4 per cell: J' sums onto nodes c-1, c, c+1, c+2
     int32_t* err;        // set when an index check fails (never expected)
 };
 

@@ -329,8 +329,8 @@ static int upload_grow(fdtd_ctx* c, void** buf, size_t* cap, const void* src, si
 // ------------------------------------------------------------------------
 // particle beams
 // ------------------------------------------------------------------------
-enum { BO_CELL = 0, BO_FRAC = 2, BO_UX = 4, BO_UY = 6, BO_UZ = 8, BO_ID = 10, BO_START = 12,
-       BO_TCELL = 14, BO_TFRAC, BO_TUX, BO_TUY, BO_TUZ, BO_TA0, BO_TA1, BO_TCODE, BO_END };
+enum { BO_CELL = 0, BO_FRAC = 2, BO_UX = 4, BO_UY = 6, BO_UZ = 8, BO_ID = 10, BO_START = 12, BO_RANGE = 14,
+       BO_URANGE = 16, BO_TCELL = 18, BO_TFRAC, BO_TUX, BO_TUY, BO_TUZ, BO_TRANK, BO_CUP, BO_CDOWN, BO_S, BO_END };
 
 template <typename R>
 static BeamDev<R> fill_beam_dev(BeamHost& b, int32_t* err) {
@@ -338,6 +338,7 @@ static BeamDev<R> fill_beam_dev(BeamHost& b, int32_t* err) {
     std::memset(&d, 0, sizeof(d));
     char* blk = (char*)b.block;
     d.sim = b.sim; d.N = b.N; d.np = (int32_t)b.np; d.G = b.G;
+    d.nwaves = b.nwaves; d.nnode = b.nnode;
     d.a = (R)b.a; d.jscale = (R)b.jscale; d.Sc = (R)b.Sc;
     for (int k = 0; k < 2; ++k) {
         d.cell[k] = (int32_t*)(blk + b.off[BO_CELL + k]);
@@ -347,15 +348,18 @@ static BeamDev<R> fill_beam_dev(BeamHost& b, int32_t* err) {
         d.uz[k] = (R*)(blk + b.off[BO_UZ + k]);
         d.id[k] = (int32_t*)(blk + b.off[BO_ID + k]);
         d.start[k] = (int32_t*)(blk + b.off[BO_START + k]);
+        d.range[k] = (int32_t*)(blk + b.off[BO_RANGE + k]);
+        d.urange[k] = (int32_t*)(blk + b.off[BO_URANGE + k]);
     }
     d.tcell = (int32_t*)(blk + b.off[BO_TCELL]);
     d.tfrac = (R*)(blk + b.off[BO_TFRAC]);
     d.tux = (R*)(blk + b.off[BO_TUX]);
     d.tuy = (R*)(blk + b.off[BO_TUY]);
     d.tuz = (R*)(blk + b.off[BO_TUZ]);
-    d.ta0 = (R*)(blk + b.off[BO_TA0]);
-    d.ta1 = (R*)(blk + b.off[BO_TA1]);
-    d.tcode = (uint8_t*)(blk + b.off[BO_TCODE]);
+    d.trank = (int32_t*)(blk + b.off[BO_TRANK]);
+    d.cup = (int32_t*)(blk + b.off[BO_CUP]);
+    d.cdown = (int32_t*)(blk + b.off[BO_CDOWN]);
+    d.S = (double*)(blk + b.off[BO_S]);
     d.err = err;
     return d;
 }
@@ -377,8 +381,11 @@ static int upload_beam(fdtd_ctx* c, BeamHost& b, int N) {
     HIPCHK(hipMemcpyAsync(blk + b.off[BO_UY], uy.data(), rn, hipMemcpyHostToDevice, c->st));
     HIPCHK(hipMemcpyAsync(blk + b.off[BO_UZ], uz.data(), rn, hipMemcpyHostToDevice, c->st));
     HIPCHK(hipMemcpyAsync(blk + b.off[BO_ID], b.id0.data(), in, hipMemcpyHostToDevice, c->st));
-    for (int k = 0; k < 2; ++k)
+    for (int k = 0; k < 2; ++k) {
         HIPCHK(hipMemcpyAsync(blk + b.off[BO_START + k], b.start0.data(), sn, hipMemcpyHostToDevice, c->st));
+        HIPCHK(hipMemcpyAsync(blk + b.off[BO_RANGE + k], b.range0, sizeof(b.range0), hipMemcpyHostToDevice, c->st));
+        HIPCHK(hipMemcpyAsync(blk + b.off[BO_URANGE + k], b.range0, sizeof(b.range0), hipMemcpyHostToDevice, c->st));
+    }
     HIPCHK(hipStreamSynchronize(c->st));   // the host vectors above go out of scope
     return FDTD_OK;
 }
@@ -427,29 +434,35 @@ static int attach_t(fdtd_ctx* c, int id, SimHost& s, const fdtd_beam_desc_t* bd)
     }
     int64_t acc = 0, occupied = 0;
     for (int cc = 0; cc <= N + 1; ++cc) { b->start0[cc] = (int32_t)acc; acc += occ[cc]; occupied += occ[cc] > 0; }
-    // cells per sort/deposit wave: about 256 particles per wave.  The wave
-    // also reads 3 neighbour cells, but its serial chunk loop, not the
-    // re-read, sets the kernel time (measured best at 2-4 cells for 100
-    // particles per cell on MI355X).
+    // live range (cells of the particles; all live at attach)
+    b->range0[0] = b->cell0.front();
+    b->range0[1] = b->cell0.back();
+    // cells per push/count group: about 64-128 particles per wave (one or two
+    // chunks); a wave loops over groups of the live range in a grid stride
     int G = bd->group;
     if (G <= 0) {
         const double per_cell = occupied ? (double)n / occupied : 1.0;
-        G = (int)std::lround(256.0 / per_cell);
+        G = (int)std::lround(96.0 / per_cell);
     }
     b->G = std::min(std::max(G, 1), kPicGMax);
+    const int span = b->range0[1] - b->range0[0] + 3;
+    b->nwaves = (int)std::min<int64_t>(32768, std::max<int64_t>(1, (span + b->G - 1) / b->G));
+    b->nnode = std::min(64, std::max(1, (span + 8 + 255) / 256 + 1));
     // device block
     Layout L;
-    const size_t rn = sizeof(R) * n, in = sizeof(int32_t) * n;
+    const size_t rn = sizeof(R) * n, in = sizeof(int32_t) * n, cn = sizeof(int32_t) * (N + 2);
     for (int k = 0; k < 2; ++k) L.add(in);                       // cell
     for (int k = 0; k < 2; ++k) L.add(rn);                       // frac
-    for (int k = 0; k < 6; ++k) L.add(rn);                       // ux uy uz (x2, interleaved by k)
+    for (int k = 0; k < 6; ++k) L.add(rn);                       // ux0 ux1 uy0 uy1 uz0 uz1 (BO_UX/UY/UZ + k)
     for (int k = 0; k < 2; ++k) L.add(in);                       // id
-    for (int k = 0; k < 2; ++k) L.add(sizeof(int32_t) * (N + 2)); // start
+    for (int k = 0; k < 2; ++k) L.add(cn);                       // start
+    for (int k = 0; k < 4; ++k) L.add(2 * sizeof(int32_t));      // range x2, urange x2
     L.add(in);                                                   // tcell
-    for (int k = 0; k < 6; ++k) L.add(rn);                       // tfrac tux tuy tuz ta0 ta1
-    L.add(n);                                                    // tcode
+    for (int k = 0; k < 4; ++k) L.add(rn);                       // tfrac tux tuy tuz
+    L.add(in);                                                   // trank
+    L.add(cn); L.add(cn);                                        // cup cdown
+    L.add(4 * sizeof(double) * (N + 2));                         // S
     for (int k = 0; k < L.n; ++k) b->off[k] = L.off[k];
-    // ux/uy/uz were added as ux0 ux1 uy0 uy1 uz0 uz1: matches BO_UX/BO_UY/BO_UZ + k
     b->bytes = L.total;
     if (hipMalloc(&b->block, b->bytes) != hipSuccess) {
         delete b;
@@ -495,8 +508,9 @@ static int rebuild_t(fdtd_ctx* c) {
         if (!bh) continue;
         const int bi = (int)bdev.size();
         bdev.push_back(fill_beam_dev<R>(*bh, c->d_err));
+        for (int w = 0; w < bh->nwaves; ++w) pitems.push_back(make_int2(bi, w));
         for (int64_t p = 0; p < bh->np; p += 256) pblocks.push_back(make_int2(bi, (int)p));
-        for (int c0 = 0; c0 <= c->sims[i]->d.N; c0 += bh->G) pitems.push_back(make_int2(bi, c0));
+        for (int k = 0; k < bh->nnode; ++k) pblocks.push_back(make_int2(bi, -(k + 1)));
     }
     c->has_pic = !bdev.empty();
     c->npblocks = (int)pblocks.size();
@@ -617,15 +631,15 @@ int fdtd_run(fdtd_ctx* c, int64_t nsteps) {
                                                  std::to_string(c->launch_idx) + ": " + hipGetErrorString(stale));
             hipError_t e = c->prec == FDTD_F64
                 ? launch_pic<double>((const SimDev<double>*)c->d_sims, (const BeamDev<double>*)c->d_beams,
-                                     c->d_pblocks, c->npblocks, c->d_pitems, c->npitems, c->launch_idx, c->st,
+                                     c->d_pitems, c->npitems, c->d_pblocks, c->npblocks, c->launch_idx, c->st,
                                      e0, e1, e2, e3)
                 : launch_pic<float>((const SimDev<float>*)c->d_sims, (const BeamDev<float>*)c->d_beams,
-                                    c->d_pblocks, c->npblocks, c->d_pitems, c->npitems, c->launch_idx, c->st,
+                                    c->d_pitems, c->npitems, c->d_pblocks, c->npblocks, c->launch_idx, c->st,
                                     e0, e1, e2, e3);
             if (e != hipSuccess)
                 return fdtd_fail(FDTD_E_HIP, std::string("pic launch ") + std::to_string(c->launch_idx) + " (" +
-                                                 std::to_string(c->npblocks) + " push blocks, " +
-                                                 std::to_string(c->npitems) + " sort waves): " + hipGetErrorString(e));
+                                                 std::to_string(c->npitems) + " push waves, " +
+                                                 std::to_string(c->npblocks) + " scatter blocks): " + hipGetErrorString(e));
         }
         EvPair* ev = c->profile ? next_event(c, FDTD_KERNEL_STENCIL) : nullptr;
         if (c->profile && !ev) return FDTD_E_HIP;
@@ -880,7 +894,7 @@ int pic_attach(fdtd_ctx* c, int id, const fdtd_beam_desc_t* bd) {
     if (!bd->x || !bd->u) return fdtd_fail(FDTD_E_ARG, "null particle arrays");
     if (!(bd->m > 0) || !(bd->dt > 0) || !(bd->dz > 0)) return fdtd_fail(FDTD_E_ARG, "m, dt and dz must be > 0");
     if (!(299792458.0 * bd->dt / bd->dz < 1.0)) return fdtd_fail(FDTD_E_ARG, "c dt / dz must be < 1");
-    if (bd->group < 0 || bd->group > kPicGMax) return fdtd_fail(FDTD_E_ARG, "group must be 0..61");
+    if (bd->group < 0 || bd->group > kPicGMax) return fdtd_fail(FDTD_E_ARG, "group must be 0..64");
     const int N = s->d.N;
     if (N < 8) return fdtd_fail(FDTD_E_ARG, "grid too small for particles");
     for (int64_t i = 0; i < bd->n; ++i) {

@@ -26,7 +26,9 @@ struct BeamHost {
     int sim = -1;
     int N = 0;                   // grid size of the simulation
     int64_t np = 0;
-    int G = 8;                   // cells per sort/deposit wave
+    int G = 1;                   // cells per push/count group
+    int nwaves = 1, nnode = 1;   // push/count waves, scatter node blocks
+    int32_t range0[2] = {0, -1}; // live cell range at attach
     double q = 0, m = 0, w = 0;  // charge, mass, weight per unit transverse area
     double a = 0, jscale = 0, Sc = 0;
     void* block = nullptr;
@@ -69,9 +71,9 @@ struct fdtd_ctx {
     std::vector<BeamHost*> beams;            // indexed like sims (nullptr: no beam)
     void* d_beams = nullptr;
     size_t d_beams_cap = 0;
-    int2* d_pblocks = nullptr;               // push blocks (beam, first particle)
+    int2* d_pblocks = nullptr;               // scatter blocks (beam, first particle | -(node block + 1))
     size_t d_pblocks_cap = 0;
-    int2* d_pitems = nullptr;                // sort/deposit waves (beam, first cell)
+    int2* d_pitems = nullptr;                // push/count waves (beam, wave index)
     size_t d_pitems_cap = 0;
     int npblocks = 0, npitems = 0;
     bool has_pic = false;                    // some active simulation carries a beam

@@ -1,6 +1,6 @@
 // Particle-in-cell kernels for gfx950: relativistic Boris push with linear
-// gather, then a deterministic stable re-sort by cell fused with the CIC
-// current deposit (segmented FP64 sums, no floating-point atomics).
+// gather, deterministic stable re-sort by cell and CIC current deposit with
+// FP64 segmented sums (no floating-point atomics).
 //
 // The reference has no particle code; the current enters the grid through
 // V.Jx in ADE_ExUpdate (BaseFDTD11.py:663-669) as Ex += (dHy - Jx)*UpExMat.
@@ -8,22 +8,24 @@
 // deposit writes J' = Z0*Jx (J_x*dz in the reference's units).
 //
 // Step n, before the field launch of step n (which consumes J^{n+1/2}):
-//   pic_push          one thread per particle (sorted order): gather E^n at
-//                     x^n and B^n = -(H^{n-1/2} + H^{n+1/2})/(2c) at x^n - 1/2,
-//                     Boris, x^{n+1} = x^n + Sc u_z/gamma; writes the new state
-//                     and the CIC weights of J' at x^{n+1/2} to records at the
-//                     particle's old index.
-//   pic_sort_deposit  one wave per G cells: counts of particles moving down /
-//                     staying / moving up per cell (|d| <= 1 since Sc*beta < 1),
-//                     stable ranks, the new index
-//                        d=-1: start[c] - up(c-1) + r
-//                        d= 0: start[c] + down(c) + r
-//                        d=+1: start[c+1] - up(c) + down(c+1) + r
-//                     (no global scan: only neighbour cells enter), the new
-//                     start[] of its cells, and J'[m] = S+2(m-2) + S+1(m-1) +
-//                     S0(m) + S-1(m+1) from per-cell FP64 sums in a fixed
-//                     order.  The wave reads cells c0-2 .. c0+G so every value
-//                     it needs is its own.
+//   pic_push_count  one wave per group of G cells of the live range (grid-
+//                   stride): per particle, gather E^n at x^n and
+//                   B^n = -(H^{n-1/2} + H^{n+1/2})/(2c) at x^n - 1/2, Boris,
+//                   x^{n+1} = x^n + Sc u_z/gamma; per cell, the particles
+//                   moving down / up (|d| <= 1 since Sc*beta < 1), each
+//                   particle's stable rank among (cell, d), and four FP64
+//                   sums of the CIC weights of J' at x^{n+1/2} onto nodes
+//                   c-1 .. c+2 (segmented DPP scans in a fixed order).
+//   pic_scatter     one thread per particle: the new index
+//                      d=-1: start[c] - up(c-1) + r
+//                      d= 0: start[c] + down(c) + r
+//                      d=+1: start[c+1] - up(c) + down(c+1) + r
+//                   (parked particles keep theirs); node threads form
+//                   J'[m] = S+2(m-2) + S+1(m-1) + S0(m) + S-1(m+1), the new
+//                   start[] and the next live range (a single writer each:
+//                   the first and the last non-empty cell).
+// Absorbed particles stay in the array (cells 1 and N-2), so the particle
+// count is constant and no global scan is needed.
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
@@ -35,75 +37,6 @@ __device__ __forceinline__ unsigned long long lanes_le(int lane) {
     return lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
 }
 __device__ __forceinline__ unsigned long long lanes_lt(int lane) { return (1ull << lane) - 1ull; }
-
-template <typename R>
-__global__ __launch_bounds__(256) void pic_push(const SimDev<R>* __restrict__ sims,
-                                                const BeamDev<R>* __restrict__ beams,
-                                                const int2* __restrict__ blocks, long long launch_idx) {
-    const int2 bl = blocks[blockIdx.x];
-    const BeamDev<R>& b = beams[bl.x];
-    const int i = bl.y + (int)threadIdx.x;
-    if (i >= b.np) return;
-    const SimDev<R>& sd = sims[b.sim];
-    const int par = (int)((launch_idx - sd.launch_begin) & 1);
-    const int N = b.N;
-    int c = b.cell[par][i];
-    R f = b.frac[par][i];
-    R ux = b.ux[par][i], uy = b.uy[par][i], uz = b.uz[par][i];
-    int code = 1 | (1 << 2);            // d = 0, node offset 0
-    R a0 = R(0), a1 = R(0);
-    if (c >= 2 && c <= N - 3) {
-        const R* E = sd.E[par];
-        const R* Hc = sd.H[par];
-        const R* Hp = sd.H[par ^ 1];
-        const R Eg = (R(1) - f) * E[c] + f * E[c + 1];
-        int j;
-        R g;
-        if (f >= R(0.5)) { j = c; g = f - R(0.5); } else { j = c - 1; g = f + R(0.5); }
-        const R h0 = R(0.5) * (Hp[j] + Hc[j]);
-        const R h1 = R(0.5) * (Hp[j + 1] + Hc[j + 1]);
-        const R Hg = (R(1) - g) * h0 + g * h1;
-        // Boris: E = (Eg, 0, 0), B = (0, -Hg/c, 0)
-        const R aE = b.a * Eg;
-        const R umx = ux + aE, umy = uy, umz = uz;
-        const R gm = sqrt(R(1) + umx * umx + umy * umy + umz * umz);
-        const R ty = -b.a * Hg / gm;
-        const R sy = R(2) * ty / (R(1) + ty * ty);
-        const R upx = umx - umz * ty;
-        const R upz = umz + umx * ty;
-        ux = umx - upz * sy + aE;
-        uz = umz + upx * sy;
-        uy = umy;
-        const R gam = sqrt(R(1) + ux * ux + uy * uy + uz * uz);
-        const R bx = ux / gam, bz = uz / gam;
-        const R dx = b.Sc * bz;
-        // CIC of J' at x^{n+1/2}, relative to cell c: xh in (-1/4, 5/4)
-        const R xh = f + R(0.5) * dx;
-        const int m0 = xh < R(0) ? -1 : (xh >= R(1) ? 1 : 0);
-        const R w1 = xh - (R)m0;
-        const R q = b.jscale * bx;
-        a0 = q * (R(1) - w1);
-        a1 = q * w1;
-        // new position
-        R nf = f + dx;
-        int d = nf < R(0) ? -1 : (nf >= R(1) ? 1 : 0);
-        nf -= (R)d;
-        int nc = c + d;
-        if (nc < 2) { nc = 1; nf = R(0.5); }
-        if (nc > N - 3) { nc = N - 2; nf = R(0.5); }
-        c = nc;
-        f = nf;
-        code = (d + 1) | ((m0 + 1) << 2);
-    }
-    b.tcell[i] = c;
-    b.tfrac[i] = f;
-    b.tux[i] = ux;
-    b.tuy[i] = uy;
-    b.tuz[i] = uz;
-    b.ta0[i] = a0;
-    b.ta1[i] = a1;
-    b.tcode[i] = (uint8_t)code;
-}
 
 // 64-bit DPP move (two 32-bit halves); lanes the control does not feed read 0.
 template <int CTRL, int ROWMASK>
@@ -166,33 +99,63 @@ __device__ __forceinline__ Chunk chunk_info(int c, int i, int hi, int lane) {
     return k;
 }
 
-constexpr int kBatch = 8;           // chunks fetched per memory round trip in the sort kernel
 
-// Per-particle inputs of pass 1.
+constexpr int kBatch = 2;           // chunks of 64 particles fetched per memory round trip
+
+// Boris push of one live particle; returns d and the CIC weights (node offset off = m0 + 1).
 template <typename R>
-struct P1 {
-    int c, code;
-    R a0, a1;
-};
-template <typename R>
-__device__ __forceinline__ P1<R> load_p1(const BeamDev<R>& b, const int32_t* cell, int i, int hi) {
-    P1<R> p;
-    const bool v = i < hi;
-    p.c = v ? cell[i] : 0x7fffffff;
-    p.code = v ? (int)b.tcode[i] : 5;
-    p.a0 = v ? b.ta0[i] : R(0);
-    p.a1 = v ? b.ta1[i] : R(0);
-    return p;
+__device__ __forceinline__ void push_one(const BeamDev<R>& b, const SimDev<R>& sd, int par, int& c, R& f, R& ux,
+                                         R& uy, R& uz, int& d, int& off, double& a0, double& a1) {
+    const int N = b.N;
+    const R* E = sd.E[par];
+    const R* Hc = sd.H[par];
+    const R* Hp = sd.H[par ^ 1];
+    const R Eg = (R(1) - f) * E[c] + f * E[c + 1];
+    int j;
+    R g;
+    if (f >= R(0.5)) { j = c; g = f - R(0.5); } else { j = c - 1; g = f + R(0.5); }
+    const R h0 = R(0.5) * (Hp[j] + Hc[j]);
+    const R h1 = R(0.5) * (Hp[j + 1] + Hc[j + 1]);
+    const R Hg = (R(1) - g) * h0 + g * h1;
+    // Boris: E = (Eg, 0, 0), B = (0, -Hg/c, 0)
+    const R aE = b.a * Eg;
+    const R umx = ux + aE, umy = uy, umz = uz;
+    const R gm = sqrt(R(1) + umx * umx + umy * umy + umz * umz);
+    const R ty = -b.a * Hg / gm;
+    const R sy = R(2) * ty / (R(1) + ty * ty);
+    const R upx = umx - umz * ty;
+    const R upz = umz + umx * ty;
+    ux = umx - upz * sy + aE;
+    uz = umz + upx * sy;
+    uy = umy;
+    const R gam = sqrt(R(1) + ux * ux + uy * uy + uz * uz);
+    const R bx = ux / gam, bz = uz / gam;
+    const R dx = b.Sc * bz;
+    // CIC of J' at x^{n+1/2}, relative to cell c: xh in (-1/4, 5/4)
+    const R xh = f + R(0.5) * dx;
+    const int m0 = xh < R(0) ? -1 : (xh >= R(1) ? 1 : 0);
+    const R w1 = xh - (R)m0;
+    const R q = b.jscale * bx;
+    a0 = (double)(q * (R(1) - w1));
+    a1 = (double)(q * w1);
+    off = m0 + 1;
+    // new position
+    R nf = f + dx;
+    d = nf < R(0) ? -1 : (nf >= R(1) ? 1 : 0);
+    nf -= (R)d;
+    int nc = c + d;
+    if (nc < 2) { nc = 1; nf = R(0.5); }
+    if (nc > N - 3) { nc = N - 2; nf = R(0.5); }
+    c = nc;
+    f = nf;
 }
 
 template <typename R>
-__global__ __launch_bounds__(64) void pic_sort_deposit(const SimDev<R>* __restrict__ sims,
-                                                       const BeamDev<R>* __restrict__ beams,
-                                                       const int2* __restrict__ items, long long launch_idx) {
-    constexpr int L = kPicGMax + 3;
-    __shared__ int cnt[L][3];       // down, stay, up per local cell
-    __shared__ int rk[L][3];        // running ranks (second pass)
-    __shared__ double S[L][4];      // J' sums to nodes c-1, c, c+1, c+2
+__global__ __launch_bounds__(64) void pic_push_count(const SimDev<R>* __restrict__ sims,
+                                                     const BeamDev<R>* __restrict__ beams,
+                                                     const int2* __restrict__ items, long long launch_idx) {
+    __shared__ int cnt[kPicGMax][3];    // down, stay, up so far per group cell (also the rank carry)
+    __shared__ double Sl[kPicGMax][4];  // J' sums onto nodes c-1 .. c+2
     const int2 it = items[blockIdx.x];
     const BeamDev<R>& b = beams[it.x];
     const SimDev<R>& sd = sims[b.sim];
@@ -200,129 +163,187 @@ __global__ __launch_bounds__(64) void pic_sort_deposit(const SimDev<R>* __restri
     const int po = par ^ 1;
     const int lane = threadIdx.x;
     const int N = b.N, np = b.np;
-    const int c0 = it.y;
-    const int G = min(b.G, kPicGMax);
-    const int cend = min(c0 + G, N + 1);           // own cells [c0, cend)
-    const int kb = c0 - 2;                          // local index k = c - kb
-    for (int k = lane; k < L; k += 64) {
-        cnt[k][0] = cnt[k][1] = cnt[k][2] = 0;
-        rk[k][0] = rk[k][1] = rk[k][2] = 0;
-        S[k][0] = S[k][1] = S[k][2] = S[k][3] = 0.0;
+    if (it.y == 0 && lane == 0) {            // next step's ranges start from here
+        b.range[po][0] = N;
+        b.range[po][1] = -1;
+        b.urange[po][0] = b.urange[par][0];
+        b.urange[po][1] = b.urange[par][1];
     }
-    wave_sync();
+    const int cmin = b.range[par][0], cmax = b.range[par][1];
+    if (cmin > cmax) return;
+    const int G = max(1, min(b.G, kPicGMax));
+    const int lo_cell = max(cmin - 1, 0), hi_cell = min(cmax + 1, N);   // counts cover the range +-1
+    const int ngroups = (hi_cell - lo_cell + G) / G;
     const int32_t* st = b.start[par];
     const int32_t* cell = b.cell[par];
     auto clampi = [&](int v) { return min(max(v, 0), np); };
-    const int lo = clampi(st[max(c0 - 2, 0)]);
-    const int hi = max(lo, clampi(st[min(c0 + G + 1, N + 1)]));
-    const int olo = clampi(st[c0]);
-    const int ohi = max(olo, clampi(st[cend]));
-
-    // pass 1: direction counts and segmented J' sums over cells c0-2 .. c0+G.
-    // Chunks are fetched kBatch at a time (one memory round trip per batch)
-    // and then processed in order from registers.
-    for (int bb = lo; bb < hi; bb += 64 * kBatch) {
-        P1<R> pk[kBatch];
-#pragma unroll
-        for (int q = 0; q < kBatch; ++q) pk[q] = load_p1(b, cell, bb + 64 * q + lane, hi);
-#pragma unroll
-        for (int q = 0; q < kBatch; ++q) {
-            const int base = bb + 64 * q;
-            if (base >= hi) break;
-            const P1<R>& cur = pk[q];
-            const int i = base + lane;
-            const Chunk k = chunk_info(cur.c, i, hi, lane);
-            const int d = (cur.code & 3) - 1;
-            const int off = (cur.code >> 2) & 3;
-            const double a0 = (double)cur.a0, a1 = (double)cur.a1;
-            const unsigned long long seg = lanes_le(lane) & ~lanes_lt(k.fl);
-            const unsigned long long bd0 = __ballot(k.valid && d == -1);
-            const unsigned long long bd1 = __ballot(k.valid && d == 0);
-            const unsigned long long bd2 = __ballot(k.valid && d == 1);
-            double v[4];
-#pragma unroll
-            for (int w = 0; w < 4; ++w) v[w] = (w == off ? a0 : 0.0) + (w == off + 1 ? a1 : 0.0);
-            seg_scan4(v, lane, k.fl);
-            const int lk = k.valid ? cur.c - kb : -1;
-            if (k.last && lk >= 0 && lk < L) {
-                cnt[lk][0] += __popcll(bd0 & seg);
-                cnt[lk][1] += __popcll(bd1 & seg);
-                cnt[lk][2] += __popcll(bd2 & seg);
-#pragma unroll
-                for (int w = 0; w < 4; ++w) S[lk][w] += v[w];
-            }
-            wave_sync();
+    for (int gi = it.y; gi < ngroups; gi += b.nwaves) {
+        const int g0 = lo_cell + gi * G;
+        const int g1 = min(g0 + G, hi_cell + 1);
+        for (int k = lane; k < G; k += 64) {
+            cnt[k][0] = cnt[k][1] = cnt[k][2] = 0;
+            Sl[k][0] = Sl[k][1] = Sl[k][2] = Sl[k][3] = 0.0;
         }
-    }
-
-    // pass 2: stable ranks and the scatter of this wave's own particles,
-    // kBatch chunks per memory round trip
-    for (int bb = olo; bb < ohi; bb += 64 * kBatch) {
-        int cc[kBatch], cd[kBatch], tc[kBatch], tid[kBatch];
-        R tf[kBatch], tx[kBatch], ty[kBatch], tz[kBatch];
+        wave_sync();
+        const int plo = clampi(st[g0]);
+        const int phi = max(plo, clampi(st[g1]));
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int bb = plo; bb < phi; bb += 64 * kBatch) {
+            int cc[kBatch];
+            R ff[kBatch], xx[kBatch], yy[kBatch], zz[kBatch];
 #pragma unroll
-        for (int q = 0; q < kBatch; ++q) {
-            const int j = bb + 64 * q + lane;
-            const bool v = j < ohi;
-            cc[q] = v ? cell[j] : 0x7fffffff;
-            cd[q] = v ? (int)b.tcode[j] : 5;
-            tc[q] = v ? b.tcell[j] : 0;
-            tid[q] = v ? b.id[par][j] : 0;
-            tf[q] = v ? b.tfrac[j] : R(0);
-            tx[q] = v ? b.tux[j] : R(0);
-            ty[q] = v ? b.tuy[j] : R(0);
-            tz[q] = v ? b.tuz[j] : R(0);
-        }
-#pragma unroll
-        for (int q = 0; q < kBatch; ++q) {
-            const int base = bb + 64 * q;
-            if (base >= ohi) break;
-            const int c_cur = cc[q], code = cd[q];
-            const int i = base + lane;
-            const Chunk k = chunk_info(c_cur, i, ohi, lane);
-            const int d = (code & 3) - 1;
-            const unsigned long long seg = lanes_le(lane) & ~lanes_lt(k.fl);
-            const unsigned long long bd0 = __ballot(k.valid && d == -1);
-            const unsigned long long bd1 = __ballot(k.valid && d == 0);
-            const unsigned long long bd2 = __ballot(k.valid && d == 1);
-            const unsigned long long mine = d < 0 ? bd0 : (d == 0 ? bd1 : bd2);
-            const int lk = k.valid ? c_cur - kb : -1;
-            const bool in = k.valid && c_cur >= 0 && c_cur <= N && lk >= 1 && lk + 1 < L;
-            int r = 0;
-            if (in) r = rk[lk][d + 1] + __popcll(mine & lanes_lt(lane) & ~lanes_lt(k.fl));
-            wave_sync();
-            if (in && k.last) {
-                rk[lk][0] += __popcll(bd0 & seg);
-                rk[lk][1] += __popcll(bd1 & seg);
-                rk[lk][2] += __popcll(bd2 & seg);
+            for (int q = 0; q < kBatch; ++q) {
+                const int i = bb + 64 * q + lane;
+                const bool v = i < phi;
+                cc[q] = v ? cell[i] : 0x7fffffff;
+                ff[q] = v ? b.frac[par][i] : R(0);
+                xx[q] = v ? b.ux[par][i] : R(0);
+                yy[q] = v ? b.uy[par][i] : R(0);
+                zz[q] = v ? b.uz[par][i] : R(0);
             }
-            if (in) {
-                int ni;
-                if (d < 0) ni = st[c_cur] - cnt[lk - 1][2] + r;
-                else if (d == 0) ni = st[c_cur] + cnt[lk][0] + r;
-                else ni = st[c_cur + 1] - cnt[lk][2] + cnt[lk + 1][0] + r;
-                if (ni >= 0 && ni < np) {
-                    b.cell[po][ni] = tc[q];
-                    b.frac[po][ni] = tf[q];
-                    b.ux[po][ni] = tx[q];
-                    b.uy[po][ni] = ty[q];
-                    b.uz[po][ni] = tz[q];
-                    b.id[po][ni] = tid[q];
+#pragma unroll
+            for (int q = 0; q < kBatch; ++q) {
+                const int base = bb + 64 * q;
+                if (base >= phi) break;
+                const int i = base + lane;
+                const int c_old = cc[q];
+                const Chunk k = chunk_info(c_old, i, phi, lane);
+                int c = c_old, d = 0, off = 1;
+                R f = ff[q], ux = xx[q], uy = yy[q], uz = zz[q];
+                double a0 = 0.0, a1 = 0.0;
+                const bool live = k.valid && c_old >= 2 && c_old <= N - 3;
+                if (live) push_one<R>(b, sd, par, c, f, ux, uy, uz, d, off, a0, a1);
+                const unsigned long long seg = lanes_le(lane) & ~lanes_lt(k.fl);
+                const unsigned long long bd0 = __ballot(k.valid && d == -1);
+                const unsigned long long bd1 = __ballot(k.valid && d == 0);
+                const unsigned long long bd2 = __ballot(k.valid && d == 1);
+                const unsigned long long mine = d < 0 ? bd0 : (d == 0 ? bd1 : bd2);
+                double v[4];
+#pragma unroll
+                for (int w = 0; w < 4; ++w) v[w] = (w == off ? a0 : 0.0) + (w == off + 1 ? a1 : 0.0);
+                if (G == 1) {
+                    // one cell per group: each lane accumulates its particles
+                    // in chunk order, one fixed-order reduction after the group
+#pragma unroll
+                    for (int w = 0; w < 4; ++w) acc[w] += v[w];
                 } else {
-                    b.err[0] = 1;
+                    seg_scan4(v, lane, k.fl);
+                }
+                const int lk = k.valid ? c_old - g0 : -1;
+                const bool in = lk >= 0 && lk < G;
+                int r = 0;
+                if (in) r = cnt[lk][d + 1] + __popcll(mine & lanes_lt(lane) & ~lanes_lt(k.fl));
+                wave_sync();
+                if (in && k.last) {
+                    cnt[lk][0] += __popcll(bd0 & seg);
+                    cnt[lk][1] += __popcll(bd1 & seg);
+                    cnt[lk][2] += __popcll(bd2 & seg);
+                    if (G != 1) {
+#pragma unroll
+                        for (int w = 0; w < 4; ++w) Sl[lk][w] += v[w];
+                    }
+                }
+                if (in) {
+                    b.tcell[i] = c;
+                    b.tfrac[i] = f;
+                    b.tux[i] = ux;
+                    b.tuy[i] = uy;
+                    b.tuz[i] = uz;
+                    b.trank[i] = (r << 2) | (d + 1);
+                }
+                wave_sync();
+            }
+        }
+        if (G == 1) {
+            seg_scan4(acc, lane, 0);        // lane 63 holds the wave sums
+            if (lane == 63) {
+#pragma unroll
+                for (int w = 0; w < 4; ++w) Sl[0][w] = acc[w];
+            }
+            wave_sync();
+        }
+        for (int k = lane; k < g1 - g0; k += 64) {
+            const int c = g0 + k;
+            b.cup[c] = cnt[k][2];
+            b.cdown[c] = cnt[k][0];
+#pragma unroll
+            for (int w = 0; w < 4; ++w) b.S[4 * (size_t)c + w] = Sl[k][w];
+        }
+        wave_sync();
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void pic_scatter(const SimDev<R>* __restrict__ sims,
+                                                   const BeamDev<R>* __restrict__ beams,
+                                                   const int2* __restrict__ blocks, long long launch_idx) {
+    const int2 bl = blocks[blockIdx.x];
+    const BeamDev<R>& b = beams[bl.x];
+    const SimDev<R>& sd = sims[b.sim];
+    const int par = (int)((launch_idx - sd.launch_begin) & 1);
+    const int po = par ^ 1;
+    const int N = b.N, np = b.np;
+    const int32_t* st = b.start[par];
+    const int cmin = b.range[par][0], cmax = b.range[par][1];
+    // counts and sums exist on [cmin-1, cmax+1] this step; zero elsewhere
+    auto inr = [&](int c) { return c >= cmin - 1 && c <= cmax + 1 && cmin <= cmax; };
+    auto up = [&](int c) { return inr(c) ? b.cup[c] : 0; };
+    auto down = [&](int c) { return inr(c) ? b.cdown[c] : 0; };
+    if (bl.y >= 0) {
+        const int i = bl.y + (int)threadIdx.x;
+        if (i >= np) return;
+        const int c = b.cell[par][i];
+        if (c >= 2 && c <= N - 3) {
+            const int tr = b.trank[i];
+            const int d = (tr & 3) - 1, r = tr >> 2;
+            int ni;
+            if (d < 0) ni = st[c] - up(c - 1) + r;
+            else if (d == 0) ni = st[c] + down(c) + r;
+            else ni = st[c + 1] - up(c) + down(c + 1) + r;
+            if (ni >= 0 && ni < np) {
+                b.cell[po][ni] = b.tcell[i];
+                b.frac[po][ni] = b.tfrac[i];
+                b.ux[po][ni] = b.tux[i];
+                b.uy[po][ni] = b.tuy[i];
+                b.uz[po][ni] = b.tuz[i];
+                b.id[po][ni] = b.id[par][i];
+            } else {
+                b.err[0] = 1;
+            }
+        } else {        // parked: unchanged, same index
+            b.cell[po][i] = c;
+            b.frac[po][i] = b.frac[par][i];
+            b.ux[po][i] = b.ux[par][i];
+            b.uy[po][i] = b.uy[par][i];
+            b.uz[po][i] = b.uz[par][i];
+            b.id[po][i] = b.id[par][i];
+        }
+        return;
+    }
+    // node threads: J', new start[] and the next live range over the union range
+    const int k = -bl.y - 1;
+    const int e0 = max(b.urange[par][0] - 3, 0), e1 = min(b.urange[par][1] + 4, N);
+    const int nlo = st[2] + down(2);                 // new start of cell 2 (cell 1 sends nothing up)
+    const int nhi = st[N - 2] - up(N - 3);           // new start of cell N-2
+    R* J = const_cast<R*>(sd.J);
+    for (int m = e0 + k * 256 + (int)threadIdx.x; m <= e1; m += b.nnode * 256) {
+        auto Sq = [&](int c, int w) { return inr(c) ? b.S[4 * (size_t)c + w] : 0.0; };
+        const double j = Sq(m - 2, 3) + Sq(m - 1, 2) + Sq(m, 1) + Sq(m + 1, 0);
+        J[m] = (R)j;
+        const int ns = st[m] - up(m - 1) + down(m);
+        b.start[po][m] = ns;
+        if (m >= 2 && m <= N - 3) {
+            const int ns1 = st[m + 1] - up(m) + down(m + 1);
+            if (ns1 > ns) {
+                if (ns == nlo) {
+                    b.range[po][0] = m;
+                    b.urange[po][0] = min(b.urange[par][0], m);
+                }
+                if (ns1 == nhi) {
+                    b.range[po][1] = m;
+                    b.urange[po][1] = max(b.urange[par][1], m);
                 }
             }
-            wave_sync();
-        }
-    }
-
-    // new cell starts and J' of this wave's nodes
-    for (int c = c0 + lane; c < cend; c += 64) {
-        const int lk = c - kb;
-        b.start[po][c] = st[c] - cnt[lk - 1][2] + cnt[lk][0];
-        if (c <= N) {
-            const double j = S[lk - 2][3] + S[lk - 1][2] + S[lk][1] + S[lk + 1][0];
-            const_cast<R*>(sd.J)[c] = (R)j;
         }
     }
 }
@@ -330,25 +351,26 @@ __global__ __launch_bounds__(64) void pic_sort_deposit(const SimDev<R>* __restri
 // With profiling events the kernels go through hipExtLaunchKernelGGL, which
 // stamps the events from their own dispatch packets.
 template <typename R>
-hipError_t launch_pic(const SimDev<R>* sims, const BeamDev<R>* beams, const int2* push_blocks, int npush,
-                      const int2* items, int nitems, long long launch_idx, hipStream_t st, hipEvent_t e0,
+hipError_t launch_pic(const SimDev<R>* sims, const BeamDev<R>* beams, const int2* push_items, int npush,
+                      const int2* scatter_blocks, int nscatter, long long launch_idx, hipStream_t st, hipEvent_t e0,
                       hipEvent_t e1, hipEvent_t e2, hipEvent_t e3) {
     hipError_t e = hipSuccess;
     if (npush > 0) {
         if (e0 || e1)
-            hipExtLaunchKernelGGL((pic_push<R>), dim3(npush), dim3(256), 0, st, e0, e1, 0, sims, beams,
-                                  push_blocks, launch_idx);
+            hipExtLaunchKernelGGL((pic_push_count<R>), dim3(npush), dim3(64), 0, st, e0, e1, 0, sims, beams,
+                                  push_items, launch_idx);
         else
-            hipLaunchKernelGGL((pic_push<R>), dim3(npush), dim3(256), 0, st, sims, beams, push_blocks, launch_idx);
+            hipLaunchKernelGGL((pic_push_count<R>), dim3(npush), dim3(64), 0, st, sims, beams, push_items,
+                               launch_idx);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    if (nitems > 0) {
+    if (nscatter > 0) {
         if (e2 || e3)
-            hipExtLaunchKernelGGL((pic_sort_deposit<R>), dim3(nitems), dim3(64), 0, st, e2, e3, 0, sims, beams,
-                                  items, launch_idx);
+            hipExtLaunchKernelGGL((pic_scatter<R>), dim3(nscatter), dim3(256), 0, st, e2, e3, 0, sims, beams,
+                                  scatter_blocks, launch_idx);
         else
-            hipLaunchKernelGGL((pic_sort_deposit<R>), dim3(nitems), dim3(64), 0, st, sims, beams, items,
+            hipLaunchKernelGGL((pic_scatter<R>), dim3(nscatter), dim3(256), 0, st, sims, beams, scatter_blocks,
                                launch_idx);
         e = hipGetLastError();
     }

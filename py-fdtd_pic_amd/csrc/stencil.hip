@@ -84,6 +84,33 @@ __device__ __forceinline__ double kerr_series(double z) {
 
 constexpr double kKerrSeriesCut = 0.01;
 
+// FP32 mode: the root is x (1 + delta) with x = u/eps in FP64 and the small
+// correction delta = y - 1 = -z (1 - 3z + 12z^2 - 55z^3 + 273z^4 - 1428z^5)
+// (z < 0.01: |delta| < 0.01, truncation < 1e-10 relative) in FP32: its FP32
+// rounding moves E by < 1e-9 of an FP32 ulp.  The series/closed-form choice
+// stays on z.
+__device__ __forceinline__ float kerr_delta_f32(float z) {
+    float t = 1428.0f;
+    t = fmaf(t, -z, 273.0f);
+    t = fmaf(t, -z, 55.0f);
+    t = fmaf(t, -z, 12.0f);
+    t = fmaf(t, -z, 3.0f);
+    t = fmaf(t, -z, 1.0f);
+    return -z * t;
+}
+template <typename R>
+__device__ __forceinline__ R kerr_series_e(double x, double kz, double& z) {
+    if constexpr (sizeof(R) == 4) {
+        const float xf = (float)x;
+        const float zf = (float)kz * xf * xf;
+        z = (double)zf;
+        return (R)fma(x, (double)kerr_delta_f32(zf), x);
+    } else {
+        z = kz * x * x;
+        return (R)(x * kerr_series<R>(z));
+    }
+}
+
 // ADE + D update for one slab cell (no branches: ade == 0 is encoded as
 // zero coefficients).  FP64 keeps the reference form P' = A P + B Pm + C E
 // (ADE_PolarisationCurrent_Ex :620-631); FP32 keeps the velocity form
@@ -187,8 +214,7 @@ __device__ __forceinline__ void e_pure_slab(TILE_PARAMS, const TileCtx<R>& c, co
                                           cP[j], sA[j], sB[j], sC[j]);
         if constexpr (KIND == SLAB_KERR_E) {
             x[j] = u * ied[j];
-            z[j] = kz[j] * x[j] * x[j];
-            E[j] = (R)(x[j] * kerr_series<R>(z[j]));
+            E[j] = kerr_series_e<R>(x[j], kz[j], z[j]);
         } else {
             E[j] = slab_e_simple<R, KIND>(sd, u, D[j], ie[j]);
         }
@@ -222,8 +248,7 @@ __device__ __forceinline__ void e_mixed(TILE_PARAMS, const TileCtx<R>& c, const 
                                           sB[j], sC[j]);
         if constexpr (KIND == SLAB_KERR_E) {
             x[j] = u * ied[j];
-            z[j] = sl ? kz[j] * x[j] * x[j] : 0.0;
-            Es[j] = (R)(x[j] * kerr_series<R>(z[j]));
+            Es[j] = kerr_series_e<R>(x[j], sl ? kz[j] : 0.0, z[j]);
         } else {
             x[j] = z[j] = 0.0;
             Es[j] = slab_e_simple<R, KIND>(sd, u, Dn, ie[j]);
@@ -372,12 +397,15 @@ __device__ __forceinline__ void run_steps(TILE_PARAMS, const TileCtx<R>& c, cons
 template <typename R, int K, bool KAHAN, int KIND>
 __global__ __launch_bounds__(64) void stencil_tiles(const SimDev<R>* __restrict__ sims,
                                                     const int2* __restrict__ tiles, int ntiles,
-                                                    int tiles_per_xcd, long long launch_idx,
+                                                    long long launch_idx,
                                                     long long gstep0, int nsteps, int S) {
     constexpr bool DFORM = KIND >= SLAB_LINEAR;
-    // XCD-aware order: block b runs on XCD b%8; give each XCD a contiguous run of tiles.
-    const int b = blockIdx.x;
-    const int tid = (b & 7) * tiles_per_xcd + (b >> 3);
+    // Tile t runs as block t: consecutive tiles go round-robin to the 8 XCDs
+    // (block b runs on XCD b % 8), so the costly slab tiles, which are
+    // contiguous along the grid, spread over every XCD.  (A contiguous run of
+    // tiles per XCD keeps neighbours in one L2 but put all of config 2's slab
+    // on three XCDs: 25.4 us launches against 18.0 us here.)
+    const int tid = blockIdx.x;
     if (tid >= ntiles) return;
     const int2 tl = tiles[tid];
     const SimDev<R>& sd = sims[tl.x];
@@ -483,9 +511,33 @@ __global__ __launch_bounds__(64) void stencil_tiles(const SimDev<R>* __restrict_
     // one lean loop per tile class
     const bool gen = c.Jc != nullptr || c.snap_interval > 0 || c.npr > 2;
     const bool extras = tile_strip || tile_src || tile_tf || c.npr > 0;
+    // A slab-boundary tile of the E-form Kerr slab runs the pure-slab loop:
+    // a vacuum cell is a slab cell with D = E, P = 0, eps = 1, chi3 = 0 when
+    // the D and E updates share the coefficient (kD == ke0, jD == 1); FP64
+    // then gives the vacuum update bit for bit.
+    unsigned emu_mask = 0;
+    bool emu_tile = false;
+    if constexpr (KIND == SLAB_KERR_E) {
+        const bool all_valid = __all(c.base >= 0 && c.base + K - 1 <= N);
+        emu_tile = any_slab && !pure_slab && !extras && !gen && all_valid && sd.ke0 == sd.kD && sd.jD == R(1);
+        if (emu_tile) {
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                if ((slab_mask >> j) & 1u) continue;
+                emu_mask |= 1u << j;
+                D[j] = E[j];
+                cD[j] = cE[j];
+                P[j] = Pm[j] = cP[j] = R(0);
+                sA[j] = sB[j] = sC[j] = R(0);
+                ie[j] = R(1);
+                ied[j] = 1.0;
+                kz[j] = 0.0;
+            }
+        }
+    }
     if (gen)
         run_steps<R, K, KAHAN, KIND, M_MIXED | M_STRIP | M_SRC | M_GENERAL>(TILE_ARGS, c, sd);
-    else if (pure_slab && !extras)
+    else if ((pure_slab || emu_tile) && !extras)
         run_steps<R, K, KAHAN, KIND, M_PURE_SLAB>(TILE_ARGS, c, sd);
     else if (!any_slab && !extras)
         run_steps<R, K, KAHAN, KIND, 0>(TILE_ARGS, c, sd);
@@ -502,7 +554,12 @@ __global__ __launch_bounds__(64) void stencil_tiles(const SimDev<R>* __restrict_
         const int n = c.base + j;
         sd.E[po][n] = E[j];
         sd.H[po][n] = H[j];
-        if (KAHAN) { sd.cE[po][n] = cE[j]; sd.cH[po][n] = cH[j]; }
+        if (KAHAN) {
+            // an emulated vacuum cell carries its compensation in cD: E - cE = D - cD
+            const R ce = ((emu_mask >> j) & 1u) ? (R)((double)E[j] - ((double)D[j] - (double)cD[j])) : cE[j];
+            sd.cE[po][n] = ce;
+            sd.cH[po][n] = cH[j];
+        }
         if ((slab_mask >> j) & 1u) {
             const int i = n - F;
             sd.D[po][i] = D[j]; sd.P[po][i] = P[j]; sd.Pm[po][i] = Pm[j];
@@ -523,14 +580,13 @@ template <typename R, int K, bool KAHAN, int KIND>
 static hipError_t launch_k(const SimDev<R>* sims, const int2* tiles, int ntiles, long long launch_idx,
                            long long gstep0, int nsteps, int S, hipStream_t st, hipEvent_t ev0,
                            hipEvent_t ev1) {
-    const int per_xcd = (ntiles + 7) / 8;
-    const int grid = per_xcd * 8;
+    const int grid = ntiles;
     if (ev0 || ev1)
         hipExtLaunchKernelGGL((stencil_tiles<R, K, KAHAN, KIND>), dim3(grid), dim3(64), 0, st, ev0, ev1, 0,
-                              sims, tiles, ntiles, per_xcd, launch_idx, gstep0, nsteps, S);
+                              sims, tiles, ntiles, launch_idx, gstep0, nsteps, S);
     else
         hipLaunchKernelGGL((stencil_tiles<R, K, KAHAN, KIND>), dim3(grid), dim3(64), 0, st, sims, tiles,
-                           ntiles, per_xcd, launch_idx, gstep0, nsteps, S);
+                           ntiles, launch_idx, gstep0, nsteps, S);
     return hipGetLastError();
 }
 

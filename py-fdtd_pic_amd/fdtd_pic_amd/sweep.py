@@ -87,7 +87,7 @@ class Scheduler:
         self.freqs = DEFAULT_FREQS if freqs is None else np.asarray(freqs, dtype=np.float64)
         self.field = engine.Context(precision)
         self.beam = engine.Context(precision, K=2, S=1)
-        self.K, self.S = K or 2, S or (48 if precision == "f32" else 32)
+        self.K, self.S = K or 2, S or 48
         self.field.set_tiling(self.K, self.S)
         self.autotune_enabled = autotune
         self.slice_launches = int(slice_launches)

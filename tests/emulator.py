@@ -102,7 +102,7 @@ def _expand(scalar, strip_vals, Ls, Rs, n):
 
 
 # ---------------------------------------------------------------------------
-# PIC sort/deposit wave (pic.hip: pic_sort_deposit), lane by lane
+# PIC push/count + scatter (pic.hip: pic_push_count, pic_scatter), lane by lane
 # ---------------------------------------------------------------------------
 def _chunk(cell, base, hi):
     lanes = np.arange(64)
@@ -117,64 +117,82 @@ def _chunk(cell, base, hi):
     return i, valid, c, fl, last
 
 
-def emulate_pic_sort_deposit(cell, code, a0, a1, start, G, N):
-    """Returns (new index per old particle, new start[], J) as the kernel computes them."""
+def emulate_pic_step(cell, d_live, off_live, a0, a1, start, crange, urange, G, N):
+    """One step of the two kernels given each live particle's move d and CIC
+    record.  Returns (new index per particle, new start, J, next range, next
+    union range, cup, cdown)."""
     n = len(cell)
-    new_idx = np.full(n, -1, dtype=np.int64)
+    live = (cell >= 2) & (cell <= N - 3)
+    d = np.where(live, d_live, 0)
+    off = np.where(live, off_live, 1)
+    a0 = np.where(live, a0, 0.0)
+    a1 = np.where(live, a1, 0.0)
+    cup = np.full(N + 2, -999, dtype=np.int64)      # garbage outside what the kernel writes
+    cdown = np.full(N + 2, -999, dtype=np.int64)
+    S = np.full((N + 2, 4), np.nan)
+    rank = np.full(n, -1, dtype=np.int64)
+    cmin, cmax = crange
+    nrange = [N, -1]
+    nurange = list(urange)
+    if cmin <= cmax:
+        lo_cell, hi_cell = max(cmin - 1, 0), min(cmax + 1, N)
+        ngroups = (hi_cell - lo_cell + G) // G
+        for gi in range(ngroups):
+            g0 = lo_cell + gi * G
+            g1 = min(g0 + G, hi_cell + 1)
+            cnt = np.zeros((G, 3), dtype=np.int64)
+            Sl = np.zeros((G, 4))
+            plo, phi = start[g0], start[g1]
+            for base in range(plo, phi, 64):
+                i, valid, c, fl, last = _chunk(cell, base, phi)
+                ii = np.minimum(i, n - 1)
+                dd = np.where(valid, d[ii], 0)
+                for l in np.nonzero(valid)[0]:
+                    lk = c[l] - g0
+                    rank[i[l]] = cnt[lk, dd[l] + 1] + np.sum(dd[fl[l]:l] == dd[l])
+                v = np.zeros((64, 4))
+                v[np.arange(64), np.where(valid, off[ii], 1)] += np.where(valid, a0[ii], 0.0)
+                v[np.arange(64), np.where(valid, off[ii], 1) + 1] += np.where(valid, a1[ii], 0.0)
+                for l in np.nonzero(last)[0]:
+                    lk = c[l] - g0
+                    for q in range(3):
+                        cnt[lk, q] += np.sum(dd[fl[l]:l + 1] == q - 1)
+                    Sl[lk] += v[fl[l]:l + 1].sum(axis=0)
+            for k in range(g1 - g0):
+                cup[g0 + k], cdown[g0 + k], S[g0 + k] = cnt[k, 2], cnt[k, 0], Sl[k]
+    inr = lambda c: cmin <= cmax and cmin - 1 <= c <= cmax + 1
+    up = lambda c: cup[c] if inr(c) else 0
+    down = lambda c: cdown[c] if inr(c) else 0
+    new_idx = np.empty(n, dtype=np.int64)
+    for i in range(n):
+        c = cell[i]
+        if live[i]:
+            r = rank[i]
+            if d[i] < 0:
+                new_idx[i] = start[c] - up(c - 1) + r
+            elif d[i] == 0:
+                new_idx[i] = start[c] + down(c) + r
+            else:
+                new_idx[i] = start[c + 1] - up(c) + down(c + 1) + r
+        else:
+            new_idx[i] = i
     nstart = start.copy()
     J = np.zeros(N + 1)
-    L = G + 3
-    for c0 in range(0, N + 1, G):
-        cnt = np.zeros((L, 3), dtype=np.int64)
-        rk = np.zeros((L, 3), dtype=np.int64)
-        S = np.zeros((L, 4))
-        cend = min(c0 + G, N + 1)
-        kb = c0 - 2
-        clamp = lambda v: min(max(int(v), 0), n)
-        lo = clamp(start[max(c0 - 2, 0)])
-        hi = max(lo, clamp(start[min(c0 + G + 1, N + 1)]))
-        for base in range(lo, hi, 64):
-            i, valid, c, fl, last = _chunk(cell, base, hi)
-            ii = np.minimum(i, n - 1)
-            cd = np.where(valid, code[ii], 5)
-            d = (cd & 3) - 1
-            off = (cd >> 2) & 3
-            v = np.zeros((64, 4))
-            v[np.arange(64), off] += np.where(valid, a0[ii], 0.0)
-            v[np.arange(64), off + 1] += np.where(valid, a1[ii], 0.0)
-            for l in np.nonzero(last)[0]:
-                seg = slice(fl[l], l + 1)
-                lk = c[l] - kb
-                if 0 <= lk < L:
-                    for dd in range(3):
-                        cnt[lk, dd] += np.sum(d[seg] == dd - 1)
-                    S[lk] += v[seg].sum(axis=0)
-        olo = clamp(start[c0])
-        ohi = max(olo, clamp(start[cend]))
-        for base in range(olo, ohi, 64):
-            i, valid, c, fl, last = _chunk(cell, base, ohi)
-            ii = np.minimum(i, n - 1)
-            d = np.where(valid, (code[ii] & 3) - 1, 0)
-            ranks = np.zeros(64, dtype=np.int64)
-            for l in np.nonzero(valid)[0]:
-                lk = c[l] - kb
-                ranks[l] = rk[lk, d[l] + 1] + np.sum(d[fl[l]:l] == d[l])
-            for l in np.nonzero(last)[0]:
-                lk = c[l] - kb
-                for dd in range(3):
-                    rk[lk, dd] += np.sum(d[fl[l]:l + 1] == dd - 1)
-            for l in np.nonzero(valid)[0]:
-                cc, lk, r = c[l], c[l] - kb, ranks[l]
-                if d[l] < 0:
-                    ni = start[cc] - cnt[lk - 1, 2] + r
-                elif d[l] == 0:
-                    ni = start[cc] + cnt[lk, 0] + r
-                else:
-                    ni = start[cc + 1] - cnt[lk, 2] + cnt[lk + 1, 0] + r
-                new_idx[i[l]] = ni
-        for cc in range(c0, cend):
-            lk = cc - kb
-            nstart[cc] = start[cc] - cnt[lk - 1, 2] + cnt[lk, 0]
-            if cc <= N:
-                J[cc] = S[lk - 2, 3] + S[lk - 1, 2] + S[lk, 1] + S[lk + 1, 0]
-    return new_idx, nstart, J
+    e0, e1 = max(urange[0] - 3, 0), min(urange[1] + 4, N)
+    nlo = start[2] + down(2)
+    nhi = start[N - 2] - up(N - 3)
+    Sq = lambda c, w: S[c, w] if inr(c) else 0.0
+    for m in range(e0, e1 + 1):
+        J[m] = Sq(m - 2, 3) + Sq(m - 1, 2) + Sq(m, 1) + Sq(m + 1, 0)
+        ns = start[m] - up(m - 1) + down(m)
+        nstart[m] = ns
+        if 2 <= m <= N - 3:
+            ns1 = start[m + 1] - up(m) + down(m + 1)
+            if ns1 > ns:
+                if ns == nlo:
+                    nrange[0] = m
+                    nurange[0] = min(urange[0], m)
+                if ns1 == nhi:
+                    nrange[1] = m
+                    nurange[1] = max(urange[1], m)
+    return new_idx, nstart, J, tuple(nrange), tuple(nurange), cup, cdown

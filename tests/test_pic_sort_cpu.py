@@ -1,50 +1,58 @@
-"""CPU check of the sort/deposit index arithmetic of pic_sort_deposit
-(pic.hip), emulated lane by lane: for random cell-sorted beams with moves of
--1/0/+1 cell and absorption into the graveyard cells, the new indices form
-the stable sort by new cell, the new start[] is the exclusive count scan, and
-J' equals the CIC sum of every particle's two weights (np.bincount)."""
+"""CPU check of the index arithmetic of the PIC kernels (pic.hip:
+pic_push_count and pic_scatter), emulated lane by lane over several steps of
+random moves: the new indices form the stable sort by new cell with parked
+particles keeping their index, start[] stays the exclusive count scan, J'
+equals the CIC sum of every live particle's two weights (np.bincount), and
+the single-writer live range equals the min/max of the live cells."""
 import numpy as np
 import pytest
 
-from emulator import emulate_pic_sort_deposit
+from emulator import emulate_pic_step
 
 
-def _beam(rng, N, n, dense_lo, dense_hi, p_dead=0.05):
-    cell = np.sort(rng.integers(dense_lo, dense_hi, n))
-    dead = rng.random(n) < p_dead
-    cell = np.where(dead, np.where(rng.random(n) < 0.5, 1, N - 2), cell)
-    cell = np.sort(cell)
+def _moves(rng, cell, N):
     live = (cell >= 2) & (cell <= N - 3)
-    d = np.where(live, rng.integers(-1, 2, n), 0)
-    newc = cell + d
+    d = rng.integers(-1, 2, len(cell))
+    newc = np.where(live, cell + d, cell)
     newc = np.where(live & (newc < 2), 1, newc)
     newc = np.where(live & (newc > N - 3), N - 2, newc)
-    d = newc - cell
-    off = np.where(live, rng.integers(0, 3, n), 1)
-    code = (d + 1) | (off << 2)
-    a0 = np.where(live, rng.normal(size=n), 0.0)
-    a1 = np.where(live, rng.normal(size=n), 0.0)
-    start = np.searchsorted(cell, np.arange(N + 2), side="left").astype(np.int64)
-    return cell, newc, code, a0, a1, off, start
+    return np.where(live, newc - cell, 0), newc
 
 
-@pytest.mark.parametrize("N,n,lo,hi,G", [
-    (200, 3000, 2, 198, 4),        # dense, many chunks per wave
-    (200, 3000, 2, 198, 1),
-    (300, 500, 100, 140, 7),       # clustered, empty regions
-    (64, 2000, 2, 62, 61),         # one wave covers everything
-    (500, 40, 2, 498, 16),         # sparse: chunks span many cells
-    (120, 900, 2, 5, 3),           # packed against the left graveyard
+@pytest.mark.parametrize("N,n,lo,hi,G,steps", [
+    (200, 3000, 2, 198, 1, 4),       # dense, two chunks per group
+    (200, 3000, 2, 198, 4, 3),
+    (300, 500, 100, 140, 7, 4),      # clustered, empty regions
+    (64, 2000, 2, 62, 64, 3),        # one group covers everything
+    (500, 40, 2, 498, 16, 4),        # sparse: chunks span many cells
+    (120, 900, 2, 6, 3, 6),          # drifting into the left graveyard
 ])
-def test_sort_deposit_indices(N, n, lo, hi, G):
+def test_pic_step_indices(N, n, lo, hi, G, steps):
     rng = np.random.default_rng(N * 7 + n + G)
-    cell, newc, code, a0, a1, off, start = _beam(rng, N, n, lo, hi)
-    new_idx, nstart, J = emulate_pic_sort_deposit(cell, code, a0, a1, start, G, N)
-    assert np.array_equal(np.sort(new_idx), np.arange(n))
-    order = np.argsort(new_idx)
-    assert np.array_equal(order, np.argsort(newc, kind="stable"))
-    assert np.array_equal(nstart[:N + 1], np.searchsorted(np.sort(newc), np.arange(N + 1), side="left"))
-    node0 = cell + off - 1
-    Jref = np.bincount(node0, weights=a0, minlength=N + 2)[:N + 1] + \
-        np.bincount(node0 + 1, weights=a1, minlength=N + 3)[:N + 1]
-    np.testing.assert_allclose(J, Jref, rtol=0, atol=1e-12)
+    cell = np.sort(rng.integers(lo, hi, n))
+    ids = np.arange(n)
+    start = np.searchsorted(cell, np.arange(N + 2), side="left").astype(np.int64)
+    live = (cell >= 2) & (cell <= N - 3)
+    crange = (int(cell[live].min()), int(cell[live].max()))
+    urange = crange
+    for step in range(steps):
+        live = (cell >= 2) & (cell <= N - 3)
+        d, newc = _moves(rng, cell, N)
+        off = rng.integers(0, 3, n)
+        a0, a1 = rng.normal(size=n), rng.normal(size=n)
+        new_idx, nstart, J, nrange, nurange, cup, cdown = emulate_pic_step(
+            cell, d, off, a0, a1, start, crange, urange, G, N)
+        assert np.array_equal(np.sort(new_idx), np.arange(n)), step
+        assert np.all(new_idx[~live] == np.nonzero(~live)[0])
+        order = np.argsort(new_idx)
+        assert np.array_equal(order, np.argsort(newc, kind="stable")), step
+        newcell = newc[order]
+        assert np.array_equal(nstart, np.searchsorted(newcell, np.arange(N + 2), side="left")), step
+        node0 = cell + off - 1
+        Jref = np.bincount(node0[live], weights=a0[live], minlength=N + 3)[:N + 1] + \
+            np.bincount(node0[live] + 1, weights=a1[live], minlength=N + 3)[:N + 1]
+        np.testing.assert_allclose(J, Jref, rtol=0, atol=1e-12)
+        nl = (newcell >= 2) & (newcell <= N - 3)
+        exp = (int(newcell[nl].min()), int(newcell[nl].max())) if nl.any() else (N, -1)
+        assert nrange == exp, (step, nrange, exp)
+        cell, ids, start, crange, urange = newcell, ids[order], nstart, nrange, nurange
