@@ -102,14 +102,31 @@ __device__ __forceinline__ Chunk chunk_info(int c, int i, int hi, int lane) {
 
 constexpr int kBatch = 2;           // chunks of 64 particles fetched per memory round trip
 
+// Beam and field arrays are global memory, but their pointers are read from
+// descriptors, so the compiler would address them as generic (flat) memory;
+// flat accesses share the LDS wait counter with the rank table, which would
+// make every LDS access wait for the particle stores.  These casts keep them
+// global.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+template <typename T>
+__device__ __forceinline__ gptr<T> gp(T* p) { return (gptr<T>)p; }
+template <typename T>
+__device__ __forceinline__ gptr<const T> gp(const T* p) { return (gptr<const T>)p; }
+
+// 1/x: the hardware reciprocal in FP32 (1 ulp; the FP32 mode's tolerance is
+// 1e-5), an IEEE division in FP64.
+__device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double recip(double x) { return 1.0 / x; }
+
 // Boris push of one live particle; returns d and the CIC weights (node offset off = m0 + 1).
 template <typename R>
 __device__ __forceinline__ void push_one(const BeamDev<R>& b, const SimDev<R>& sd, int par, int& c, R& f, R& ux,
                                          R& uy, R& uz, int& d, int& off, double& a0, double& a1) {
     const int N = b.N;
-    const R* E = sd.E[par];
-    const R* Hc = sd.H[par];
-    const R* Hp = sd.H[par ^ 1];
+    const gptr<const R> E = gp((const R*)sd.E[par]);
+    const gptr<const R> Hc = gp((const R*)sd.H[par]);
+    const gptr<const R> Hp = gp((const R*)sd.H[par ^ 1]);
     const R Eg = (R(1) - f) * E[c] + f * E[c + 1];
     int j;
     R g;
@@ -121,15 +138,15 @@ __device__ __forceinline__ void push_one(const BeamDev<R>& b, const SimDev<R>& s
     const R aE = b.a * Eg;
     const R umx = ux + aE, umy = uy, umz = uz;
     const R gm = sqrt(R(1) + umx * umx + umy * umy + umz * umz);
-    const R ty = -b.a * Hg / gm;
+    const R ty = -b.a * Hg * recip(gm);
     const R sy = R(2) * ty / (R(1) + ty * ty);
     const R upx = umx - umz * ty;
     const R upz = umz + umx * ty;
     ux = umx - upz * sy + aE;
     uz = umz + upx * sy;
     uy = umy;
-    const R gam = sqrt(R(1) + ux * ux + uy * uy + uz * uz);
-    const R bx = ux / gam, bz = uz / gam;
+    const R rg = recip(sqrt(R(1) + ux * ux + uy * uy + uz * uz));
+    const R bx = ux * rg, bz = uz * rg;
     const R dx = b.Sc * bz;
     // CIC of J' at x^{n+1/2}, relative to cell c: xh in (-1/4, 5/4)
     const R xh = f + R(0.5) * dx;
@@ -150,18 +167,28 @@ __device__ __forceinline__ void push_one(const BeamDev<R>& b, const SimDev<R>& s
     f = nf;
 }
 
+// kPushWaves independent waves per workgroup (a CU holds a limited number of
+// workgroups; single-wave workgroups would cap the resident waves), each with
+// its own LDS tables; item 4*block + wave, items with x < 0 are padding.
+constexpr int kPushWaves = 4;
+
 template <typename R>
-__global__ __launch_bounds__(64) void pic_push_count(const SimDev<R>* __restrict__ sims,
-                                                     const BeamDev<R>* __restrict__ beams,
-                                                     const int2* __restrict__ items, long long launch_idx) {
-    __shared__ int cnt[kPicGMax][3];    // down, stay, up so far per group cell (also the rank carry)
-    __shared__ double Sl[kPicGMax][4];  // J' sums onto nodes c-1 .. c+2
-    const int2 it = items[blockIdx.x];
+__global__ __launch_bounds__(64 * kPushWaves) void pic_push_count(const SimDev<R>* __restrict__ sims,
+                                                                  const BeamDev<R>* __restrict__ beams,
+                                                                  const int2* __restrict__ items,
+                                                                  long long launch_idx) {
+    __shared__ int cnt_all[kPushWaves][kPicGMax][3];    // down, stay, up so far per group cell (rank carry)
+    __shared__ double S_all[kPushWaves][kPicGMax][4];   // J' sums onto nodes c-1 .. c+2
+    const int wv = threadIdx.x >> 6;
+    int (&cnt)[kPicGMax][3] = cnt_all[wv];
+    double (&Sl)[kPicGMax][4] = S_all[wv];
+    const int2 it = items[blockIdx.x * kPushWaves + wv];
+    if (it.x < 0) return;
     const BeamDev<R>& b = beams[it.x];
     const SimDev<R>& sd = sims[b.sim];
     const int par = (int)((launch_idx - sd.launch_begin) & 1);
     const int po = par ^ 1;
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
     const int N = b.N, np = b.np;
     if (it.y == 0 && lane == 0) {            // next step's ranges start from here
         b.range[po][0] = N;
@@ -174,8 +201,13 @@ __global__ __launch_bounds__(64) void pic_push_count(const SimDev<R>* __restrict
     const int G = max(1, min(b.G, kPicGMax));
     const int lo_cell = max(cmin - 1, 0), hi_cell = min(cmax + 1, N);   // counts cover the range +-1
     const int ngroups = (hi_cell - lo_cell + G) / G;
-    const int32_t* st = b.start[par];
-    const int32_t* cell = b.cell[par];
+    const gptr<const int32_t> st = gp((const int32_t*)b.start[par]);
+    const gptr<const int32_t> cell = gp((const int32_t*)b.cell[par]);
+    const gptr<const R> pf = gp((const R*)b.frac[par]), px = gp((const R*)b.ux[par]);
+    const gptr<const R> py = gp((const R*)b.uy[par]), pz = gp((const R*)b.uz[par]);
+    const gptr<int32_t> tcell = gp(b.tcell), trank = gp(b.trank), cup = gp(b.cup), cdown = gp(b.cdown);
+    const gptr<R> tfrac = gp(b.tfrac), tux = gp(b.tux), tuy = gp(b.tuy), tuz = gp(b.tuz);
+    const gptr<double> Sg = gp(b.S);
     auto clampi = [&](int v) { return min(max(v, 0), np); };
     for (int gi = it.y; gi < ngroups; gi += b.nwaves) {
         const int g0 = lo_cell + gi * G;
@@ -196,10 +228,10 @@ __global__ __launch_bounds__(64) void pic_push_count(const SimDev<R>* __restrict
                 const int i = bb + 64 * q + lane;
                 const bool v = i < phi;
                 cc[q] = v ? cell[i] : 0x7fffffff;
-                ff[q] = v ? b.frac[par][i] : R(0);
-                xx[q] = v ? b.ux[par][i] : R(0);
-                yy[q] = v ? b.uy[par][i] : R(0);
-                zz[q] = v ? b.uz[par][i] : R(0);
+                ff[q] = v ? pf[i] : R(0);
+                xx[q] = v ? px[i] : R(0);
+                yy[q] = v ? py[i] : R(0);
+                zz[q] = v ? pz[i] : R(0);
             }
 #pragma unroll
             for (int q = 0; q < kBatch; ++q) {
@@ -244,12 +276,12 @@ __global__ __launch_bounds__(64) void pic_push_count(const SimDev<R>* __restrict
                     }
                 }
                 if (in) {
-                    b.tcell[i] = c;
-                    b.tfrac[i] = f;
-                    b.tux[i] = ux;
-                    b.tuy[i] = uy;
-                    b.tuz[i] = uz;
-                    b.trank[i] = (r << 2) | (d + 1);
+                    tcell[i] = c;
+                    tfrac[i] = f;
+                    tux[i] = ux;
+                    tuy[i] = uy;
+                    tuz[i] = uz;
+                    trank[i] = (r << 2) | (d + 1);
                 }
                 wave_sync();
             }
@@ -264,10 +296,10 @@ __global__ __launch_bounds__(64) void pic_push_count(const SimDev<R>* __restrict
         }
         for (int k = lane; k < g1 - g0; k += 64) {
             const int c = g0 + k;
-            b.cup[c] = cnt[k][2];
-            b.cdown[c] = cnt[k][0];
+            cup[c] = cnt[k][2];
+            cdown[c] = cnt[k][0];
 #pragma unroll
-            for (int w = 0; w < 4; ++w) b.S[4 * (size_t)c + w] = Sl[k][w];
+            for (int w = 0; w < 4; ++w) Sg[4 * (size_t)c + w] = Sl[k][w];
         }
         wave_sync();
     }
@@ -356,12 +388,13 @@ hipError_t launch_pic(const SimDev<R>* sims, const BeamDev<R>* beams, const int2
                       hipEvent_t e1, hipEvent_t e2, hipEvent_t e3) {
     hipError_t e = hipSuccess;
     if (npush > 0) {
+        const int nb = (npush + kPushWaves - 1) / kPushWaves;    // items are padded to whole blocks
         if (e0 || e1)
-            hipExtLaunchKernelGGL((pic_push_count<R>), dim3(npush), dim3(64), 0, st, e0, e1, 0, sims, beams,
-                                  push_items, launch_idx);
+            hipExtLaunchKernelGGL((pic_push_count<R>), dim3(nb), dim3(64 * kPushWaves), 0, st, e0, e1, 0, sims,
+                                  beams, push_items, launch_idx);
         else
-            hipLaunchKernelGGL((pic_push_count<R>), dim3(npush), dim3(64), 0, st, sims, beams, push_items,
-                               launch_idx);
+            hipLaunchKernelGGL((pic_push_count<R>), dim3(nb), dim3(64 * kPushWaves), 0, st, sims, beams,
+                               push_items, launch_idx);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

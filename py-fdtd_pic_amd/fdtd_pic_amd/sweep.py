@@ -32,7 +32,7 @@ import numpy as np
 from . import configs, engine
 
 DEFAULT_FREQS = np.linspace(100e12, 300e12, 256)
-TILINGS = [(1, 16), (1, 24), (2, 32), (2, 48), (4, 64), (4, 96)]
+TILINGS = [(1, 16), (1, 24), (2, 32), (2, 40), (2, 48), (4, 64), (4, 96)]
 _TUNE_CACHE = {}
 
 
@@ -56,8 +56,9 @@ def estimate_bytes(p, precision="f32"):
     total = slab_pass + vac_pass
     if p.get("beam"):
         n = p["beam"]["n"]
-        beam = 2 * (4 * n + 4 * w * n + 4 * n + 4 * (N + 2)) + 4 * n + 6 * w * n + n + w * (N + 1)
-        total += 2 * beam
+        # (cell, frac, ux, uy, uz, id, start) x 2, records (cell, frac, u, rank), per-cell counts and sums
+        beam = 2 * (4 * n + 4 * w * n + 4 * n + 4 * (N + 2)) + 8 * n + 4 * w * n + 40 * (N + 2) + 16 * 256
+        total += 2 * (beam + w * (N + 1))
     return int(total)
 
 
@@ -171,8 +172,9 @@ class Scheduler:
             for k, j in enumerate(mine):
                 X0r, X1r, X0t, X1t = Xr[2 * k], Xr[2 * k + 1], Xt[2 * k], Xt[2 * k + 1]
                 j.X = (X0r, X1r, X0t, X1t)
-                j.R = np.abs(X1r - X0r) / np.abs(X0r)
-                j.T = np.abs(X1t) / np.abs(X0t)
+                with np.errstate(divide="ignore", invalid="ignore"):   # no incident power at f: nan
+                    j.R = np.abs(X1r - X0r) / np.abs(X0r)
+                    j.T = np.abs(X1t) / np.abs(X0t)
         t = time.perf_counter() if now is None else now
         for j in fin:
             j.finished = t
