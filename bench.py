@@ -24,6 +24,10 @@ sys.path.insert(0, os.path.join(ROOT, "py-fdtd_pic_amd"))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# Launches stamped with timing events: one in PROFILE_EVERY (an event pair
+# costs the host ~20 us, more than a launch; stamping every launch would make
+# the timed run host-bound).
+PROFILE_EVERY = 16
 METRIC = "Gcell-updates/s + Gparticle-pushes/s at \u22641e-5 rel L2 vs FP64; p50/p99 sweep latency"
 
 
@@ -36,6 +40,7 @@ def parse():
     ap.add_argument("--sim-steps", type=int, default=50000, help="time steps per simulation")
     ap.add_argument("--K", type=int, default=0)
     ap.add_argument("--S", type=int, default=0)
+    ap.add_argument("--K-slab", type=int, default=0, help="cells per lane of slab tiles (K or K/2; 0: default)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU baseline sample length")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-pic", action="store_true", help="skip the config 3 (PIC) measurement")
@@ -53,8 +58,6 @@ def parse():
     return ap.parse_args()
 
 
-# Best measured tilings on MI355X for config 2 (cells per lane, steps per launch).
-DEFAULT_TILING = {"f32": (2, 48), "f64": (2, 48)}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "stencil_tiles_config2_hbm_traffic.txt")
 
 
@@ -69,7 +72,7 @@ def calibrate_copy():
     print(json.dumps({"copy_bytes": n * 4, "copies": 3}), flush=True)
 
 
-def measured_traffic(precision, K, S):
+def measured_traffic(precision, K, S, Ks):
     """HBM bytes per stencil launch from the committed rocprofv3 --pmc summary
     (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by a copy-kernel
     calibration), or None when no summary matches this tiling."""
@@ -77,8 +80,9 @@ def measured_traffic(precision, K, S):
         return None
     for line in open(TRAFFIC_FILE):
         f = line.split()
-        if len(f) == 5 and f[0] == "traffic" and f[1] == precision and f[2] == f"K={K}" and f[3] == f"S={S}":
-            return float(f[4])
+        if len(f) == 6 and f[0] == "traffic" and f[1] == precision and f[2] == f"K={K}" and f[3] == f"Kslab={Ks}" \
+                and f[4] == f"S={S}":
+            return float(f[5])
     return None
 
 
@@ -123,7 +127,7 @@ def pic_bench(a):
     ctx.sync()
     sim.reset()
     ctx.sync()
-    ctx.set_profile(True)
+    ctx.set_profile(True, every=PROFILE_EVERY)
     ctx.timing(reset=True)
     t0 = time.perf_counter()
     ctx.run_all()
@@ -135,8 +139,9 @@ def pic_bench(a):
     pushes, cells = n * T, N * T
     bpc = configs.bytes_per_cell_update(p, "f32")
     bpp = 40.0          # cell int32 + frac + ux, uy, uz (FP32), read and written
-    kern_ms = sum(v[0] for v in kt.values())
-    achieved = (pushes * bpp + cells * bpc) / (kern_ms / 1e3) / 1e9
+    # per-step kernel time from the sampled launches
+    step_ms = sum(v[0] / max(v[1], 1) for v in kt.values())
+    achieved = (n * bpp + N * bpc) / (step_ms / 1e3) / 1e9
     out = {"metric": "Gparticle-pushes/s", "value": round(pushes / wall / 1e9, 3),
            "cells_value": round(cells / wall / 1e9, 3), "ms_per_run": round(wall * 1e3, 2),
            "config": {"workload": f"config3: N={N} cells, {T} steps, config-1 slab, {n} electrons "
@@ -252,15 +257,17 @@ def main():
 
     p = configs.config2(steps=a.sim_steps)
     ctx = engine.Context(a.precision)
-    K = a.K or DEFAULT_TILING[a.precision][0]
-    S = a.S or (DEFAULT_TILING[a.precision][1] if K == DEFAULT_TILING[a.precision][0] else 8 * K)
-    ctx.set_tiling(K, S)
+    K0, S0, Ks0 = configs.TILING_CONFIG2[a.precision]
+    K = a.K or K0
+    Ks = a.K_slab or (Ks0 if K == K0 else K)
+    S = a.S or (S0 if K == K0 else 8 * Ks)
+    ctx.set_tiling(K, S, Ks)
     sim = ctx.add(engine.physical_spec(p))
     for _ in range(a.warmup):
         sim.reset()
         ctx.run_all()
     ctx.sync()
-    ctx.set_profile(True)
+    ctx.set_profile(True, every=PROFILE_EVERY)
     ctx.timing(reset=True)
     t0 = time.perf_counter()
     for _ in range(a.steps):
@@ -268,13 +275,13 @@ def main():
         ctx.run_all()
     ctx.sync()
     wall = time.perf_counter() - t0
-    kern_ms, launches = ctx.timing()
+    kern_ms, sampled = ctx.timing()
     ms_per_step = wall * 1e3 / a.steps
     cells = p["N"] * p["steps"]
     value = cells / (wall / a.steps) / 1e9
     bpc = configs.bytes_per_cell_update(p, a.precision)
-    alg_bytes_total = bpc * cells * a.steps
-    achieved = alg_bytes_total / (kern_ms / 1e3) / 1e9 if kern_ms > 0 else 0.0
+    launch_ms = kern_ms / max(sampled, 1)                 # average over the sampled launches
+    achieved = bpc * p["N"] * S / (launch_ms / 1e3) / 1e9 if kern_ms > 0 else 0.0
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -289,13 +296,14 @@ def main():
         "dtype": "fp32+fp64-kerr" if a.precision == "f32" else "f64",
         "data": "synthetic (seeded config 2 parameters)",
         "config": {"workload": "config2: N=65536 cells, 50000 steps, Lorentz+Kerr slab [20000,40000)",
-                   "sim_steps": p["steps"], "K": K, "S": S, "precision": a.precision},
+                   "sim_steps": p["steps"], "K": K, "K_slab": Ks, "S": S, "precision": a.precision},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(a.precision, K, S),
+                     "traffic": measured_traffic(a.precision, K, S, Ks),
                      "algorithmic_bytes_per_launch": round(bpc * p["N"] * S),
                      "bytes_per_cell_update": round(bpc, 2),
-                     "avg_launch_us": round(kern_ms * 1e3 / max(launches, 1), 2), "launches": launches},
+                     "avg_launch_us": round(launch_ms * 1e3, 2), "sampled_launches": sampled,
+                     "launches": a.steps * ((p["steps"] + S - 1) // S)},
     }
     ctx.close()
     if not a.no_cpu:

@@ -114,6 +114,10 @@ int fdtd_ctx_destroy(fdtd_ctx *ctx);
 /* Temporal tiling: K cells per lane (1,2,4,8), S steps per launch (<= 32K-1).
  * S = 0 picks the library default for K. */
 int fdtd_set_tiling(fdtd_ctx *ctx, int K, int S);
+/* Mixed widths in one launch: tiles whose window meets a Lorentz/Kerr slab
+ * hold K_slab = K/2 cells per lane, the others K (K_slab = K: uniform).  The
+ * costly slab cells then get more lanes while plain cells use fewer waves. */
+int fdtd_set_tiling_mixed(fdtd_ctx *ctx, int K, int K_slab, int S);
 
 /* ---- simulations ------------------------------------------------------ */
 /* Uploads coefficients, allocates zeroed state (FieldInit + CPML_FieldInit,
@@ -202,16 +206,20 @@ int fdtd_step_op(fdtd_ctx *ctx, int op, int narr, double *const *arrays, const i
 
 /* ---- diagnostics ------------------------------------------------------ */
 const char *fdtd_last_error(void);
-/* Device time (ms) spent in stencil launches since the last reset, from HIP
- * events stamped by the kernels' own dispatch packets; also the launch count. */
+/* Device time (ms) of the sampled stencil launches since the last reset, from
+ * HIP events stamped by the kernels' own dispatch packets; also how many
+ * launches were sampled. */
 int fdtd_timing(fdtd_ctx *ctx, double *ms, int64_t *launches, int reset);
 /* Device time (ms) and launch count of one kernel kind (FDTD_KERNEL_*) since
  * the last reset, under profiling. */
 int fdtd_kernel_timing(fdtd_ctx *ctx, int kind, double *ms, int64_t *launches);
-/* Profiling: when on, every stencil launch carries start/stop HIP events so
- * fdtd_timing reports the summed kernel durations and the launch count (the
- * average launch duration used for the roofline). */
-int fdtd_set_profile(fdtd_ctx *ctx, int on);
+/* Profiling: every `every`-th launch (0: off, 1: all) carries start/stop HIP
+ * events stamped by the kernels' dispatch packets, so fdtd_timing and
+ * fdtd_kernel_timing report the summed durations and the number of sampled
+ * launches (their ratio is the average launch duration used for the
+ * roofline).  An event pair costs more host time than a launch, so sampling
+ * keeps a long run GPU-bound. */
+int fdtd_set_profile(fdtd_ctx *ctx, int every);
 /* Bytes of device memory the context currently holds. */
 int fdtd_device_bytes(fdtd_ctx *ctx, int64_t *bytes);
 /* Library version string. */

@@ -51,6 +51,7 @@ struct fdtd_ctx {
     int prec = FDTD_F64;
     hipStream_t st = nullptr;
     int K = 2, S = 16;
+    int K_slab = 2;             // cells per lane of tiles meeting a D-form slab (K or K/2)
     int S_run = 16;             // steps per launch of the current tile list (1 with a beam)
     std::vector<SimHost*> sims;
     void* d_sims = nullptr;
@@ -62,7 +63,7 @@ struct fdtd_ctx {
     std::vector<TileGroup> groups;
     long long launch_idx = 0, gstep = 0;
     bool dirty = true;
-    bool profile = false;
+    int profile = 0;            // stamp every profile-th launch with events (0: off)
     std::vector<EvPair> events;   // profiling: start/stop per launch, tagged by kernel kind
     size_t ev_used = 0;
     long long launches = 0;

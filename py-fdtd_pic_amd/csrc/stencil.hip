@@ -394,24 +394,14 @@ __device__ __forceinline__ void run_steps(TILE_PARAMS, const TileCtx<R>& c, cons
     }
 }
 
+// One tile: 64 lanes x K cells, S steps (the whole per-tile program).
 template <typename R, int K, bool KAHAN, int KIND>
-__global__ __launch_bounds__(64) void stencil_tiles(const SimDev<R>* __restrict__ sims,
-                                                    const int2* __restrict__ tiles, int ntiles,
-                                                    long long launch_idx,
-                                                    long long gstep0, int nsteps, int S) {
+__device__ __forceinline__ void tile_body(const SimDev<R>& sd, int out0_, long long launch_idx, long long gstep0,
+                                          int nsteps, int S) {
     constexpr bool DFORM = KIND >= SLAB_LINEAR;
-    // Tile t runs as block t: consecutive tiles go round-robin to the 8 XCDs
-    // (block b runs on XCD b % 8), so the costly slab tiles, which are
-    // contiguous along the grid, spread over every XCD.  (A contiguous run of
-    // tiles per XCD keeps neighbours in one L2 but put all of config 2's slab
-    // on three XCDs: 25.4 us launches against 18.0 us here.)
-    const int tid = blockIdx.x;
-    if (tid >= ntiles) return;
-    const int2 tl = tiles[tid];
-    const SimDev<R>& sd = sims[tl.x];
     const int lane = threadIdx.x;
     const int U = 64 * K - 2 * S;
-    const int out0 = tl.y;
+    const int out0 = out0_;
     const int N = sd.N;
     const int par = (int)((launch_idx - sd.launch_begin) & 1);
     const int F = sd.F, Rr = sd.Rr, Ls = sd.Ls, Rs = sd.Rs;
@@ -570,6 +560,31 @@ __global__ __launch_bounds__(64) void stencil_tiles(const SimDev<R>* __restrict_
             sd.psiE[po][i] = psE[j]; sd.psiH[po][i] = psH[j];
         }
     }
+}
+
+// Tile t runs as block t: consecutive tiles go round-robin to the 8 XCDs
+// (block b runs on XCD b % 8), so the costly slab tiles, which are contiguous
+// along the grid, spread over every XCD.  (A contiguous run of tiles per XCD
+// keeps neighbours in one L2 but put all of config 2's slab on three XCDs:
+// 25.4 us launches against 18.0 us here.)  A tile entry is (sim | narrow << 24,
+// first interior cell): narrow tiles hold KW/2 cells per lane, the others KW,
+// so tiles with costly slab cells can be made narrower than plain ones
+// within one launch.
+template <typename R, int KW, bool KAHAN, int KIND>
+__global__ __launch_bounds__(64) void stencil_tiles(const SimDev<R>* __restrict__ sims,
+                                                    const int2* __restrict__ tiles, int ntiles,
+                                                    long long launch_idx, long long gstep0, int nsteps, int S) {
+    const int tid = blockIdx.x;
+    if (tid >= ntiles) return;
+    const int2 tl = tiles[tid];
+    const SimDev<R>& sd = sims[tl.x & 0xffffff];
+    if constexpr (KW > 1) {
+        if (tl.x >> 24) {
+            tile_body<R, KW / 2, KAHAN, KIND>(sd, tl.y, launch_idx, gstep0, nsteps, S);
+            return;
+        }
+    }
+    tile_body<R, KW, KAHAN, KIND>(sd, tl.y, launch_idx, gstep0, nsteps, S);
 }
 
 // Host-side launcher (called from host.hip).

@@ -33,6 +33,12 @@ E_CHARGE = 1.602176634e-19
 M_ELECTRON = 9.1093837139e-31      # CODATA 2022
 
 
+# Tiling of the config-2 loop measured best on MI355X: (K, S, K_slab) =
+# cells per lane of plain tiles, steps per launch, cells per lane of tiles
+# that meet the slab.
+TILING_CONFIG2 = {"f32": (2, 48, 2), "f64": (2, 48, 2)}
+
+
 def base(N, steps, **kw):
     p = dict(N=int(N), steps=int(steps), Sc=SC, dz=DZ, pml_width=W, pml_order=3, pml_R0=1e-8,
              slab_start=int(N) // 2, slab_stop=int(N) // 2, eps_inf=1.5, d_eps=2.0,

@@ -166,8 +166,10 @@ class Context:
         if K is not None:
             check(self.lib.fdtd_set_tiling(self.h, int(K), int(S or 0)))
 
-    def set_tiling(self, K, S=0):
-        check(self.lib.fdtd_set_tiling(self.h, int(K), int(S)))
+    def set_tiling(self, K, S=0, K_slab=None):
+        """K cells per lane, S steps per launch; K_slab = K/2 makes tiles that
+        meet a Lorentz/Kerr slab narrower than plain ones."""
+        check(self.lib.fdtd_set_tiling_mixed(self.h, int(K), int(K if K_slab is None else K_slab), int(S)))
 
     def add(self, spec, beam=None):
         """Add a simulation; beam (optional) is a Beam whose current couples to it."""
@@ -194,8 +196,9 @@ class Context:
     def sync(self):
         check(self.lib.fdtd_sync(self.h))
 
-    def set_profile(self, on=True):
-        check(self.lib.fdtd_set_profile(self.h, 1 if on else 0))
+    def set_profile(self, on=True, every=1):
+        """Stamp every `every`-th launch with timing events (off: on=False)."""
+        check(self.lib.fdtd_set_profile(self.h, int(every) if on else 0))
 
     def timing(self, reset=False):
         ms, n = C.c_double(), C.c_int64()

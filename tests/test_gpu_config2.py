@@ -31,8 +31,10 @@ def oracle_run():
 
 
 def _device(p, prec):
-    from fdtd_pic_amd import engine
+    from fdtd_pic_amd import configs, engine
+    K, S, Ks = configs.TILING_CONFIG2[prec]          # the tiling bench.py measures
     with engine.Context(prec) as ctx:
+        ctx.set_tiling(K, S, Ks)
         sim = ctx.add(engine.physical_spec(p))
         ctx.run_all()
         return {k: sim.read(k) for k in ("E", "H", "D", "P")} | {"probes": sim.probes()}

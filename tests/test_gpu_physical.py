@@ -25,11 +25,11 @@ def _spec(**kw):
     return ph.default_spec(**base)
 
 
-def _device(spec, prec, K=None, S=0, with_slab=True):
+def _device(spec, prec, K=None, S=0, with_slab=True, K_slab=None):
     from fdtd_pic_amd import engine
     with engine.Context(prec) as ctx:
         if K:
-            ctx.set_tiling(K, S)
+            ctx.set_tiling(K, S, K_slab)
         sim = ctx.add(engine.physical_spec(spec, with_slab=with_slab))
         ctx.run_all()
         out = {k: sim.read(k) for k in ("E", "H", "PSIE", "PSIH")}
@@ -73,13 +73,15 @@ def test_fp32_within_tolerance():
         assert v <= 1e-5, (k, v)
 
 
-@pytest.mark.parametrize("K,S", [(1, 3), (1, 31), (2, 16), (4, 8), (4, 100), (8, 64), (8, 255)])
-def test_tilings_agree(K, S):
+@pytest.mark.parametrize("K,S,Ks", [(1, 3, 1), (1, 31, 1), (2, 16, 2), (4, 8, 4), (4, 100, 4), (8, 64, 8),
+                                    (8, 255, 8), (4, 48, 2), (2, 20, 1), (8, 100, 4)])
+def test_tilings_agree(K, S, Ks):
+    """Uniform and mixed (narrow slab tiles) tilings give the same fields."""
     spec = _spec(chi3=1e-3, src_amp=2.0, steps=900)
     base = _device(spec, "f64", K=2, S=1)
-    d = _device(spec, "f64", K=K, S=S)
+    d = _device(spec, "f64", K=K, S=S, K_slab=Ks)
     errs = {k: rel_l2(d[k], base[k]) for k in ("E", "H", "D", "P", "PSIE", "PSIH", "probes")}
-    print(K, S, {k: f"{v:.1e}" for k, v in errs.items()}, flush=True)
+    print(K, S, Ks, {k: f"{v:.1e}" for k, v in errs.items()}, flush=True)
     for k, v in errs.items():
         assert v <= 1e-13, (K, S, k, v)
 
