@@ -217,8 +217,11 @@ __global__ __launch_bounds__(64 * kPushWaves) void pic_push_count(const SimDev<R
             Sl[k][0] = Sl[k][1] = Sl[k][2] = Sl[k][3] = 0.0;
         }
         wave_sync();
-        const int plo = clampi(st[g0]);
-        const int phi = max(plo, clampi(st[g1]));
+        // particles of live cells only: the graveyard cells 1 and N-2 can hold
+        // most of a beam once it has drifted out, and their particles neither
+        // move nor deposit (their cells' counts stay 0 below)
+        const int plo = clampi(st[max(g0, 2)]);
+        const int phi = max(plo, clampi(st[min(g1, N - 2)]));
         double acc[4] = {0.0, 0.0, 0.0, 0.0};
         for (int bb = plo; bb < phi; bb += 64 * kBatch) {
             int cc[kBatch];

@@ -142,7 +142,8 @@ def emulate_pic_step(cell, d_live, off_live, a0, a1, start, crange, urange, G, N
             g1 = min(g0 + G, hi_cell + 1)
             cnt = np.zeros((G, 3), dtype=np.int64)
             Sl = np.zeros((G, 4))
-            plo, phi = start[g0], start[g1]
+            plo, phi = start[max(g0, 2)], start[min(g1, N - 2)]
+            phi = max(plo, phi)
             for base in range(plo, phi, 64):
                 i, valid, c, fl, last = _chunk(cell, base, phi)
                 ii = np.minimum(i, n - 1)

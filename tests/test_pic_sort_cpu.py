@@ -26,6 +26,7 @@ def _moves(rng, cell, N):
     (64, 2000, 2, 62, 64, 3),        # one group covers everything
     (500, 40, 2, 498, 16, 4),        # sparse: chunks span many cells
     (120, 900, 2, 6, 3, 6),          # drifting into the left graveyard
+    (60, 3000, 54, 58, 1, 6),        # most of the beam parked in the right graveyard
 ])
 def test_pic_step_indices(N, n, lo, hi, G, steps):
     rng = np.random.default_rng(N * 7 + n + G)
