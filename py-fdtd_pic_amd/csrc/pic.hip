@@ -114,19 +114,38 @@ __device__ __forceinline__ gptr<T> gp(T* p) { return (gptr<T>)p; }
 template <typename T>
 __device__ __forceinline__ gptr<const T> gp(const T* p) { return (gptr<const T>)p; }
 
+// Loads and stores at an unsigned 32-bit element index: the address is the
+// (scalar) base plus a zero-extended 32-bit byte offset, which the global
+// instructions take directly (no 64-bit address per lane).
+template <typename T>
+__device__ __forceinline__ T ldo(gptr<const T> p, unsigned i) {
+    return *(gptr<const T>)((gptr<const char>)p + i * (unsigned)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void sto(gptr<T> p, unsigned i, T v) {
+    *(gptr<T>)((gptr<char>)p + i * (unsigned)sizeof(T)) = v;
+}
+
 // 1/x: the hardware reciprocal in FP32 (1 ulp; the FP32 mode's tolerance is
 // 1e-5), an IEEE division in FP64.
 __device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double recip(double x) { return 1.0 / x; }
 
 // Boris push of one live particle; returns d and the CIC weights (node offset off = m0 + 1).
+// What the push reads of the descriptors, fetched once per wave before the
+// kernel's first store (scalar loads are only legal ahead of it).
 template <typename R>
-__device__ __forceinline__ void push_one(const BeamDev<R>& b, const SimDev<R>& sd, int par, int& c, R& f, R& ux,
-                                         R& uy, R& uz, int& d, int& off, double& a0, double& a1) {
-    const int N = b.N;
-    const gptr<const R> E = gp((const R*)sd.E[par]);
-    const gptr<const R> Hc = gp((const R*)sd.H[par]);
-    const gptr<const R> Hp = gp((const R*)sd.H[par ^ 1]);
+struct PushArgs {
+    gptr<const R> E, Hc, Hp;     // E^n, H^{n+1/2}, H^{n-1/2}
+    R a, jscale, Sc;
+    int N;
+};
+
+template <typename R>
+__device__ __forceinline__ void push_one(const PushArgs<R>& pa, int& c, R& f, R& ux, R& uy, R& uz, int& d,
+                                         int& off, double& a0, double& a1) {
+    const int N = pa.N;
+    const gptr<const R> E = pa.E, Hc = pa.Hc, Hp = pa.Hp;
     const R Eg = (R(1) - f) * E[c] + f * E[c + 1];
     int j;
     R g;
@@ -135,10 +154,10 @@ __device__ __forceinline__ void push_one(const BeamDev<R>& b, const SimDev<R>& s
     const R h1 = R(0.5) * (Hp[j + 1] + Hc[j + 1]);
     const R Hg = (R(1) - g) * h0 + g * h1;
     // Boris: E = (Eg, 0, 0), B = (0, -Hg/c, 0)
-    const R aE = b.a * Eg;
+    const R aE = pa.a * Eg;
     const R umx = ux + aE, umy = uy, umz = uz;
     const R gm = sqrt(R(1) + umx * umx + umy * umy + umz * umz);
-    const R ty = -b.a * Hg * recip(gm);
+    const R ty = -pa.a * Hg * recip(gm);
     const R sy = R(2) * ty / (R(1) + ty * ty);
     const R upx = umx - umz * ty;
     const R upz = umz + umx * ty;
@@ -147,12 +166,12 @@ __device__ __forceinline__ void push_one(const BeamDev<R>& b, const SimDev<R>& s
     uy = umy;
     const R rg = recip(sqrt(R(1) + ux * ux + uy * uy + uz * uz));
     const R bx = ux * rg, bz = uz * rg;
-    const R dx = b.Sc * bz;
+    const R dx = pa.Sc * bz;
     // CIC of J' at x^{n+1/2}, relative to cell c: xh in (-1/4, 5/4)
     const R xh = f + R(0.5) * dx;
     const int m0 = xh < R(0) ? -1 : (xh >= R(1) ? 1 : 0);
     const R w1 = xh - (R)m0;
-    const R q = b.jscale * bx;
+    const R q = pa.jscale * bx;
     a0 = (double)(q * (R(1) - w1));
     a1 = (double)(q * w1);
     off = m0 + 1;
@@ -179,7 +198,9 @@ __global__ __launch_bounds__(64 * kPushWaves) void pic_push_count(const SimDev<R
                                                                   long long launch_idx) {
     __shared__ int cnt_all[kPushWaves][kPicGMax][3];    // down, stay, up so far per group cell (rank carry)
     __shared__ double S_all[kPushWaves][kPicGMax][4];   // J' sums onto nodes c-1 .. c+2
-    const int wv = threadIdx.x >> 6;
+    // wave-uniform by construction; readfirstlane lets the compiler see it, so
+    // the item and the descriptors come in through scalar loads
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     int (&cnt)[kPicGMax][3] = cnt_all[wv];
     double (&Sl)[kPicGMax][4] = S_all[wv];
     const int2 it = items[blockIdx.x * kPushWaves + wv];
@@ -189,18 +210,13 @@ __global__ __launch_bounds__(64 * kPushWaves) void pic_push_count(const SimDev<R
     const int par = (int)((launch_idx - sd.launch_begin) & 1);
     const int po = par ^ 1;
     const int lane = threadIdx.x & 63;
-    const int N = b.N, np = b.np;
-    if (it.y == 0 && lane == 0) {            // next step's ranges start from here
-        b.range[po][0] = N;
-        b.range[po][1] = -1;
-        b.urange[po][0] = b.urange[par][0];
-        b.urange[po][1] = b.urange[par][1];
-    }
-    const int cmin = b.range[par][0], cmax = b.range[par][1];
-    if (cmin > cmax) return;
+    // every descriptor field is read here, ahead of the range store below
+    const int N = b.N, np = b.np, nwaves = b.nwaves;
     const int G = max(1, min(b.G, kPicGMax));
-    const int lo_cell = max(cmin - 1, 0), hi_cell = min(cmax + 1, N);   // counts cover the range +-1
-    const int ngroups = (hi_cell - lo_cell + G) / G;
+    const PushArgs<R> pa{gp((const R*)sd.E[par]), gp((const R*)sd.H[par]), gp((const R*)sd.H[par ^ 1]),
+                         b.a, b.jscale, b.Sc, N};
+    const gptr<int32_t> rng_p = gp(b.range[par]), rng_o = gp(b.range[po]);
+    const gptr<int32_t> urng_p = gp(b.urange[par]), urng_o = gp(b.urange[po]);
     const gptr<const int32_t> st = gp((const int32_t*)b.start[par]);
     const gptr<const int32_t> cell = gp((const int32_t*)b.cell[par]);
     const gptr<const R> pf = gp((const R*)b.frac[par]), px = gp((const R*)b.ux[par]);
@@ -208,8 +224,18 @@ __global__ __launch_bounds__(64 * kPushWaves) void pic_push_count(const SimDev<R
     const gptr<int32_t> tcell = gp(b.tcell), trank = gp(b.trank), cup = gp(b.cup), cdown = gp(b.cdown);
     const gptr<R> tfrac = gp(b.tfrac), tux = gp(b.tux), tuy = gp(b.tuy), tuz = gp(b.tuz);
     const gptr<double> Sg = gp(b.S);
+    const int cmin = rng_p[0], cmax = rng_p[1];
+    if (it.y == 0 && lane == 0) {            // next step's ranges start from here
+        rng_o[0] = N;
+        rng_o[1] = -1;
+        urng_o[0] = urng_p[0];
+        urng_o[1] = urng_p[1];
+    }
+    if (cmin > cmax) return;
+    const int lo_cell = max(cmin - 1, 0), hi_cell = min(cmax + 1, N);   // counts cover the range +-1
+    const int ngroups = (hi_cell - lo_cell + G) / G;
     auto clampi = [&](int v) { return min(max(v, 0), np); };
-    for (int gi = it.y; gi < ngroups; gi += b.nwaves) {
+    for (int gi = it.y; gi < ngroups; gi += nwaves) {
         const int g0 = lo_cell + gi * G;
         const int g1 = min(g0 + G, hi_cell + 1);
         for (int k = lane; k < G; k += 64) {
@@ -230,11 +256,11 @@ __global__ __launch_bounds__(64 * kPushWaves) void pic_push_count(const SimDev<R
             for (int q = 0; q < kBatch; ++q) {
                 const int i = bb + 64 * q + lane;
                 const bool v = i < phi;
-                cc[q] = v ? cell[i] : 0x7fffffff;
-                ff[q] = v ? pf[i] : R(0);
-                xx[q] = v ? px[i] : R(0);
-                yy[q] = v ? py[i] : R(0);
-                zz[q] = v ? pz[i] : R(0);
+                cc[q] = v ? ldo(cell, i) : 0x7fffffff;
+                ff[q] = v ? ldo(pf, i) : R(0);
+                xx[q] = v ? ldo(px, i) : R(0);
+                yy[q] = v ? ldo(py, i) : R(0);
+                zz[q] = v ? ldo(pz, i) : R(0);
             }
 #pragma unroll
             for (int q = 0; q < kBatch; ++q) {
@@ -247,7 +273,7 @@ __global__ __launch_bounds__(64 * kPushWaves) void pic_push_count(const SimDev<R
                 R f = ff[q], ux = xx[q], uy = yy[q], uz = zz[q];
                 double a0 = 0.0, a1 = 0.0;
                 const bool live = k.valid && c_old >= 2 && c_old <= N - 3;
-                if (live) push_one<R>(b, sd, par, c, f, ux, uy, uz, d, off, a0, a1);
+                if (live) push_one<R>(pa, c, f, ux, uy, uz, d, off, a0, a1);
                 const unsigned long long seg = lanes_le(lane) & ~lanes_lt(k.fl);
                 const unsigned long long bd0 = __ballot(k.valid && d == -1);
                 const unsigned long long bd1 = __ballot(k.valid && d == 0);
@@ -279,12 +305,12 @@ __global__ __launch_bounds__(64 * kPushWaves) void pic_push_count(const SimDev<R
                     }
                 }
                 if (in) {
-                    tcell[i] = c;
-                    tfrac[i] = f;
-                    tux[i] = ux;
-                    tuy[i] = uy;
-                    tuz[i] = uz;
-                    trank[i] = (r << 2) | (d + 1);
+                    sto(tcell, i, c);
+                    sto(tfrac, i, f);
+                    sto(tux, i, ux);
+                    sto(tuy, i, uy);
+                    sto(tuz, i, uz);
+                    sto(trank, i, (r << 2) | (d + 1));
                 }
                 wave_sync();
             }
