@@ -18,8 +18,10 @@
  *     Python layer.
  *   - Host arrays are plain pointers with explicit lengths; values are FP64 on
  *     the host side whatever the device precision.
- *   - A context owns one HIP stream; fdtd_run is asynchronous, fdtd_sync
- *     waits.  Contexts are not thread-safe; use one per host thread.
+ *   - A context owns a compute stream and an upload stream; fdtd_run is
+ *     asynchronous, fdtd_sync waits.  fdtd_add_sim and pic_attach return
+ *     once their uploads are done, without waiting for running launches.
+ *     Contexts are not thread-safe; use one per host thread.
  */
 #ifndef FDTDPIC_H
 #define FDTDPIC_H
@@ -141,6 +143,11 @@ int fdtd_run(fdtd_ctx *ctx, int64_t nsteps);
 /* Advances until every active simulation has run all its steps. */
 int fdtd_run_all(fdtd_ctx *ctx);
 int fdtd_sync(fdtd_ctx *ctx);
+/* *busy = 1 while launches queued by fdtd_run are still running, else 0.
+ * Lets a batching host admit work (fdtd_add_sim, pic_attach upload on a
+ * stream of their own and do not wait for running launches) while the
+ * device steps; no counterpart in the reference, whose loop is blocking. */
+int fdtd_busy(fdtd_ctx *ctx, int *busy);
 
 /* ---- particle beams (PIC) ---------------------------------------------- */
 /* The reference has no particle code: its current sink V.Jx

@@ -65,7 +65,7 @@ struct Layout {
 };
 
 template <typename R>
-static int upload_beam(fdtd_ctx* c, BeamHost& b, int N);
+static int upload_beam(BeamHost& b, int N, hipStream_t st);
 
 template <typename R>
 static void fill_dev(SimHost& s, SimDev<R>& d) {
@@ -156,9 +156,11 @@ static int build_sim(fdtd_ctx* ctx, SimHost& s, const fdtd_sim_desc_t* q) {
     }
     put(12, q->srcE, T);
     put(13, q->srcH, T);
-    HIPCHK(hipMemsetAsync(s.block, 0, s.bytes, ctx->st));
-    HIPCHK(hipMemcpyAsync(s.block, img.data(), coef_end, hipMemcpyHostToDevice, ctx->st));
-    HIPCHK(hipStreamSynchronize(ctx->st));
+    // a new block is not referenced by queued launches: upload on the upload
+    // stream, so adding a simulation does not wait for running ones
+    HIPCHK(hipMemsetAsync(s.block, 0, s.bytes, ctx->up));
+    HIPCHK(hipMemcpyAsync(s.block, img.data(), coef_end, hipMemcpyHostToDevice, ctx->up));
+    HIPCHK(hipStreamSynchronize(ctx->up));
     return FDTD_OK;
 }
 
@@ -213,7 +215,13 @@ int fdtd_ctx_create(int precision, fdtd_ctx** out) {
         delete c;
         return fdtd_fail(FDTD_E_HIP, "hipStreamCreate failed");
     }
+    if (hipStreamCreateWithFlags(&c->up, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipStreamDestroy(c->st);
+        delete c;
+        return fdtd_fail(FDTD_E_HIP, "hipStreamCreate failed");
+    }
     if (hipMalloc(&c->d_err, sizeof(int32_t)) != hipSuccess || hipMemset(c->d_err, 0, sizeof(int32_t)) != hipSuccess) {
+        (void)hipStreamDestroy(c->up);
         (void)hipStreamDestroy(c->st);
         delete c;
         return fdtd_fail(FDTD_E_HIP, "allocation of the error flag failed");
@@ -226,6 +234,7 @@ int fdtd_ctx_destroy(fdtd_ctx* c) {
     if (!c) return FDTD_OK;
     // best effort: teardown errors are not actionable here
     (void)hipStreamSynchronize(c->st);
+    (void)hipStreamSynchronize(c->up);
     for (SimHost* s : c->sims)
         if (s) { if (s->block) (void)hipFree(s->block); delete s; }
     for (BeamHost* b : c->beams)
@@ -236,6 +245,7 @@ int fdtd_ctx_destroy(fdtd_ctx* c) {
     if (c->d_tiles) (void)hipFree(c->d_tiles);
     for (auto& e : c->events) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
     for (hipEvent_t e : c->throttle) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(c->up);
     (void)hipStreamDestroy(c->st);
     delete c;
     return FDTD_OK;
@@ -299,8 +309,9 @@ int fdtd_reset_sim(fdtd_ctx* c, int id) {
     const size_t state0 = s->lay_off[14];   // first state sub-array (after coefficients and sources)
     HIPCHK(hipMemsetAsync((char*)s->block + state0, 0, s->bytes - state0, c->st));
     if (id < (int)c->beams.size() && c->beams[id]) {
-        int rc = c->prec == FDTD_F64 ? upload_beam<double>(c, *c->beams[id], s->d.N)
-                                     : upload_beam<float>(c, *c->beams[id], s->d.N);
+        // ordered after the simulation's queued launches
+        int rc = c->prec == FDTD_F64 ? upload_beam<double>(*c->beams[id], s->d.N, c->st)
+                                     : upload_beam<float>(*c->beams[id], s->d.N, c->st);
         if (rc) return rc;
     }
     s->steps_done = 0;
@@ -370,7 +381,7 @@ static BeamDev<R> fill_beam_dev(BeamHost& b, int32_t* err) {
 
 // Upload the initial (sorted) particle state into buffer 0 and start[] into both.
 template <typename R>
-static int upload_beam(fdtd_ctx* c, BeamHost& b, int N) {
+static int upload_beam(BeamHost& b, int N, hipStream_t st) {
     const int64_t n = b.np;
     char* blk = (char*)b.block;
     std::vector<R> fr(n), ux(n), uy(n), uz(n);
@@ -379,18 +390,18 @@ static int upload_beam(fdtd_ctx* c, BeamHost& b, int N) {
         ux[i] = (R)b.u0[3 * i]; uy[i] = (R)b.u0[3 * i + 1]; uz[i] = (R)b.u0[3 * i + 2];
     }
     const size_t rn = sizeof(R) * n, in = sizeof(int32_t) * n, sn = sizeof(int32_t) * (N + 2);
-    HIPCHK(hipMemcpyAsync(blk + b.off[BO_CELL], b.cell0.data(), in, hipMemcpyHostToDevice, c->st));
-    HIPCHK(hipMemcpyAsync(blk + b.off[BO_FRAC], fr.data(), rn, hipMemcpyHostToDevice, c->st));
-    HIPCHK(hipMemcpyAsync(blk + b.off[BO_UX], ux.data(), rn, hipMemcpyHostToDevice, c->st));
-    HIPCHK(hipMemcpyAsync(blk + b.off[BO_UY], uy.data(), rn, hipMemcpyHostToDevice, c->st));
-    HIPCHK(hipMemcpyAsync(blk + b.off[BO_UZ], uz.data(), rn, hipMemcpyHostToDevice, c->st));
-    HIPCHK(hipMemcpyAsync(blk + b.off[BO_ID], b.id0.data(), in, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(blk + b.off[BO_CELL], b.cell0.data(), in, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(blk + b.off[BO_FRAC], fr.data(), rn, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(blk + b.off[BO_UX], ux.data(), rn, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(blk + b.off[BO_UY], uy.data(), rn, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(blk + b.off[BO_UZ], uz.data(), rn, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(blk + b.off[BO_ID], b.id0.data(), in, hipMemcpyHostToDevice, st));
     for (int k = 0; k < 2; ++k) {
-        HIPCHK(hipMemcpyAsync(blk + b.off[BO_START + k], b.start0.data(), sn, hipMemcpyHostToDevice, c->st));
-        HIPCHK(hipMemcpyAsync(blk + b.off[BO_RANGE + k], b.range0, sizeof(b.range0), hipMemcpyHostToDevice, c->st));
-        HIPCHK(hipMemcpyAsync(blk + b.off[BO_URANGE + k], b.range0, sizeof(b.range0), hipMemcpyHostToDevice, c->st));
+        HIPCHK(hipMemcpyAsync(blk + b.off[BO_START + k], b.start0.data(), sn, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(blk + b.off[BO_RANGE + k], b.range0, sizeof(b.range0), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(blk + b.off[BO_URANGE + k], b.range0, sizeof(b.range0), hipMemcpyHostToDevice, st));
     }
-    HIPCHK(hipStreamSynchronize(c->st));   // the host vectors above go out of scope
+    HIPCHK(hipStreamSynchronize(st));   // the host vectors above go out of scope
     return FDTD_OK;
 }
 
@@ -475,7 +486,7 @@ static int attach_t(fdtd_ctx* c, int id, SimHost& s, const fdtd_beam_desc_t* bd)
     if (c->beams.size() < c->sims.size()) c->beams.resize(c->sims.size(), nullptr);
     c->beams[id] = b;
     c->dev_bytes += b->bytes;
-    int rc = upload_beam<R>(c, *b, N);
+    int rc = upload_beam<R>(*b, N, c->up);     // a new beam: no queued launch reads it
     if (rc) return rc;
     c->dirty = true;
     return FDTD_OK;
@@ -718,6 +729,15 @@ int fdtd_sync(fdtd_ctx* c) {
     if (!c) return fdtd_fail(FDTD_E_ARG, "null ctx");
     HIPCHK(hipStreamSynchronize(c->st));
     return c->has_pic ? check_pic_flag(c) : FDTD_OK;
+}
+
+int fdtd_busy(fdtd_ctx* c, int* busy) {
+    if (!c || !busy) return fdtd_fail(FDTD_E_ARG, "null argument");
+    const hipError_t e = hipStreamQuery(c->st);
+    if (e == hipErrorNotReady) { *busy = 1; return FDTD_OK; }
+    if (e != hipSuccess) return fdtd_fail(FDTD_E_HIP, std::string("hipStreamQuery: ") + hipGetErrorString(e));
+    *busy = 0;
+    return FDTD_OK;
 }
 
 int fdtd_kernel_timing(fdtd_ctx* c, int kind, double* ms, int64_t* launches) {

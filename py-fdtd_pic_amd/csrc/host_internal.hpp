@@ -50,6 +50,7 @@ struct TileGroup {
 struct fdtd_ctx {
     int prec = FDTD_F64;
     hipStream_t st = nullptr;
+    hipStream_t up = nullptr;                // uploads of new simulations and beams (never waits on st)
     int K = 2, S = 16;
     int K_slab = 2;             // cells per lane of tiles meeting a D-form slab (K or K/2)
     int S_run = 16;             // steps per launch of the current tile list (1 with a beam)

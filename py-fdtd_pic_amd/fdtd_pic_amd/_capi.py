@@ -74,6 +74,7 @@ SIGNATURES = {
     "fdtd_run": (C.c_int, [_vp, C.c_int64]),
     "fdtd_run_all": (C.c_int, [_vp]),
     "fdtd_sync": (C.c_int, [_vp]),
+    "fdtd_busy": (C.c_int, [_vp, C.POINTER(C.c_int)]),
     "fdtd_read_field": (C.c_int, [_vp, C.c_int, C.c_int, _dp, C.c_int64]),
     "fdtd_write_field": (C.c_int, [_vp, C.c_int, C.c_int, _dp, C.c_int64]),
     "fdtd_read_probes": (C.c_int, [_vp, C.c_int, _dp, C.c_int64]),

@@ -196,6 +196,12 @@ class Context:
     def sync(self):
         check(self.lib.fdtd_sync(self.h))
 
+    def busy(self):
+        """True while launches queued by run() are still executing."""
+        b = C.c_int()
+        check(self.lib.fdtd_busy(self.h, C.byref(b)))
+        return bool(b.value)
+
     def set_profile(self, on=True, every=1):
         """Stamp every `every`-th launch with timing events (off: on=False)."""
         check(self.lib.fdtd_set_profile(self.h, int(every) if on else 0))

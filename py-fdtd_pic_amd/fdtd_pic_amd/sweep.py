@@ -147,14 +147,24 @@ class Scheduler:
         return best
 
     # -- stepping --------------------------------------------------------
-    def step(self):
-        """Advance both contexts by one slice (asynchronously, on their own streams)."""
+    def launch(self):
+        """Queue one slice on each context that has active jobs (asynchronous)."""
         if any(not j.has_beam for j in self.active):
             self.field.run(self.S * self.slice_launches)
         if any(j.has_beam for j in self.active):
             self.beam.run(self.beam_slice_steps)
+
+    def wait(self):
         self.field.sync()
         self.beam.sync()
+
+    def busy(self):
+        return self.field.busy() or self.beam.busy()
+
+    def step(self):
+        """Advance both contexts by one slice (each on its own stream)."""
+        self.launch()
+        self.wait()
 
     def collect(self, now=None):
         """Finish jobs whose two passes are complete: spectra on the device, then free."""
@@ -186,15 +196,30 @@ class Scheduler:
             self.done.append(j)
         return fin
 
-    def run_batch(self, jobs):
-        """Admit everything that fits, tune, run to completion, admitting the rest as room frees."""
+    def run_batch(self, jobs, first_chunk=64):
+        """Run jobs to completion.  Admission (parameter construction, uploads,
+        particle sort) is host work: after a first chunk it proceeds while the
+        device runs the queued slice, one job at least per slice, as long as
+        the HBM budget allows; finished jobs free their room at slice ends."""
         pending = list(jobs)
+
+        def admissible():
+            return pending and (self.fits(pending[0]) or not self.active)
+
+        n = 0
+        while admissible() and n < first_chunk:
+            self.admit(pending.pop(0))
+            n += 1
+        if self.autotune_enabled and not self.tuned:
+            self.autotune()
         while pending or self.active:
-            while pending and (self.fits(pending[0]) or not self.active):
+            floor = 1 if self.active else first_chunk     # idle device: admit a chunk before stepping
+            self.launch()
+            n = 0
+            while admissible() and (n < floor or self.busy()):
                 self.admit(pending.pop(0))
-            if self.autotune_enabled and not self.tuned:
-                self.autotune()
-            self.step()
+                n += 1
+            self.wait()
             self.collect()
         return self.done
 

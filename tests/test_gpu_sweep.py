@@ -110,6 +110,24 @@ def test_alone_equals_batch_bitwise(batch_jobs):
             assert np.array_equal(a.X[k], j.X[k]), (j.id, k)
 
 
+def test_admission_while_running_bitwise():
+    """Jobs admitted while slices of others run (upload stream, first chunk of
+    one job) give the same spectra, bitwise, as each job alone."""
+    from fdtd_pic_amd import sweep
+    ps = _small_instances(count=6, steps=2000, seed=13, beams=(1, 4), n_particles=2000)
+    sch = sweep.Scheduler("f32", K=2, S=48, freqs=FREQS, max_slots=3)
+    jobs = [sweep.Job(i, p) for i, p in enumerate(ps)]
+    try:
+        sch.run_batch(jobs, first_chunk=1)
+    finally:
+        sch.close()
+    assert all(j.X is not None for j in jobs)
+    for p, j in zip(ps, jobs):
+        a = sweep.run_alone(p, "f32", K=2, S=48, freqs=FREQS)
+        for k in range(4):
+            assert np.array_equal(a.X[k], j.X[k]), (j.id, k)
+
+
 def test_served_trace_alone_vs_mix():
     from fdtd_pic_amd import serve, sweep
     ps = _small_instances(count=5, steps=3000, seed=7, beams=(0, 3))
