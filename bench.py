@@ -52,6 +52,8 @@ def parse():
                     help="config4: batched sweep vs the per-simulation loop; config5: served trace replay")
     ap.add_argument("--c4-count", type=int, default=4096, help="config 4 instances")
     ap.add_argument("--c4-sample", type=int, default=24, help="instances timed in the per-simulation loop")
+    ap.add_argument("--c4-order", default="arrival", choices=["costly", "arrival"],
+                    help="batched admission order: costly jobs first, or as listed")
     ap.add_argument("--c5-requests", type=int, default=8, help="requests of the config 5 trace replayed")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 256 MiB device copy (FETCH_SIZE/WRITE_SIZE calibration under rocprofv3 --pmc)")
@@ -181,7 +183,7 @@ def config4_bench(a):
     jobs = [sweep.Job(i, p) for i, p in enumerate(inst)]
     sch = sweep.Scheduler("f32", budget_bytes=200 << 30, autotune=True)
     t0 = time.perf_counter()
-    sch.run_batch(jobs)
+    sch.run_batch(jobs, costly_first=a.c4_order == "costly")
     wall = time.perf_counter() - t0
     tiling = (sch.K, sch.S)
     sch.close()
@@ -213,6 +215,7 @@ def config4_bench(a):
             "per_sim_loop_sample": f"{len(sample)} jobs, half with beams, seeded; scaled per stratum by work",
             "per_sim_loop_sample_s": round(alone, 3),
             "per_sim_loop_est_s": round(est, 3), "speedup_vs_per_sim_loop": round(est / wall, 2),
+            "admission_order": a.c4_order,
             "spectra": "R(f), T(f) at 256 frequencies per job, FP64 DFT on the device"}
 
 

@@ -419,25 +419,38 @@ static int attach_t(fdtd_ctx* c, int id, SimHost& s, const fdtd_beam_desc_t* bd)
     b->Sc = c0 * bd->dt / bd->dz;
     b->a = bd->q * bd->dt / (2.0 * bd->m * c0);
     b->jscale = z0 * bd->q * bd->w * c0;
-    // sort by (cell, frac, ux, uy, uz, input index): a total order of the input
-    std::vector<int64_t> ord(n);
+    // sort by (cell, frac, ux, uy, uz, input index), a total order of the
+    // input: a counting sort by cell, then each cell's run by the rest
+    // (contiguous (frac, index) keys; u and the index only break frac ties)
     std::vector<int32_t> cl(n);
     std::vector<double> fr(n);
     for (int64_t i = 0; i < n; ++i) {
-        ord[i] = i;
         const double x = bd->x[i];
         const double fl = std::floor(x);
         cl[i] = (int32_t)fl;
         fr[i] = x - fl;
     }
     const double* u = bd->u;
-    std::sort(ord.begin(), ord.end(), [&](int64_t p, int64_t q) {
-        if (cl[p] != cl[q]) return cl[p] < cl[q];
-        if (fr[p] != fr[q]) return fr[p] < fr[q];
-        for (int k = 0; k < 3; ++k)
-            if (u[3 * p + k] != u[3 * q + k]) return u[3 * p + k] < u[3 * q + k];
-        return p < q;
-    });
+    std::vector<int64_t> first(N + 3, 0);
+    for (int64_t i = 0; i < n; ++i) first[cl[i] + 1]++;
+    for (int cc = 0; cc <= N + 1; ++cc) first[cc + 1] += first[cc];
+    struct Key { double f; int64_t i; };
+    std::vector<Key> keys(n);
+    {
+        std::vector<int64_t> pos(first.begin(), first.end());
+        for (int64_t i = 0; i < n; ++i) keys[pos[cl[i]]++] = Key{fr[i], i};
+    }
+    for (int cc = 0; cc <= N + 1; ++cc) {
+        if (first[cc + 1] - first[cc] < 2) continue;
+        std::sort(keys.begin() + first[cc], keys.begin() + first[cc + 1], [&](const Key& p, const Key& q) {
+            if (p.f != q.f) return p.f < q.f;
+            for (int k = 0; k < 3; ++k)
+                if (u[3 * p.i + k] != u[3 * q.i + k]) return u[3 * p.i + k] < u[3 * q.i + k];
+            return p.i < q.i;
+        });
+    }
+    std::vector<int64_t> ord(n);
+    for (int64_t k = 0; k < n; ++k) ord[k] = keys[k].i;
     b->cell0.resize(n); b->id0.resize(n); b->frac0.resize(n); b->u0.resize(3 * n);
     b->start0.assign(N + 2, 0);
     std::vector<int64_t> occ(N + 2, 0);
