@@ -32,12 +32,6 @@ __device__ __forceinline__ R pick(const R (&a)[K], int j) {
     for (int i = 1; i < K; ++i) v = (i == j) ? a[i] : v;
     return v;
 }
-template <typename R, int K>
-__device__ __forceinline__ void add_at(R (&a)[K], int j, R x) {
-#pragma unroll
-    for (int i = 0; i < K; ++i)
-        if (i == j) a[i] += x;
-}
 
 // Wave-uniform lane broadcast / insert (v_readlane / v_writelane).
 __device__ __forceinline__ float lane_read(float v, int lane) {
@@ -277,8 +271,16 @@ __device__ __forceinline__ void run_steps(TILE_PARAMS, const TileCtx<R>& c, cons
     // once after the loop: no memory access on the per-step path
     // (chunk q of 64 steps sits in srcEc[0] while it runs; the arrays rotate
     // at chunk ends so every index is a compile-time constant)
-    R srcEc[kChunks], srcHc[kChunks], pacc[2];
+    // The source and TFSF cells enter through 0/1 masks per lane and cell:
+    // E + 1*v is E + v exactly and E + 0*v is E, so the injection needs no
+    // branch on the lane or the cell.
+    R srcEc[kChunks], srcHc[kChunks], pacc[2], ms[K], mt[K];
     if constexpr (SRC) {
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            ms[j] = (c.src_lane >= 0 && c.lane == c.src_lane && j == c.src_j) ? R(1) : R(0);
+            mt[j] = (c.tf_lane >= 0 && c.lane == c.tf_lane && j == c.tf_j) ? R(1) : R(0);
+        }
 #pragma unroll
         for (int q = 0; q < kChunks; ++q) {
             const int ix = q * 64 + c.lane;
@@ -325,15 +327,13 @@ __device__ __forceinline__ void run_steps(TILE_PARAMS, const TileCtx<R>& c, cons
             }
         }
         if constexpr (SRC) {
-            if (c.src_lane >= 0) {
-                const R sv = lane_read(srcEc[0], ix & 63);
-                if (c.lane == c.src_lane) add_at<R, K>(E, c.src_j, sv);
-            }
+            const R sv = lane_read(srcEc[0], ix & 63);
+#pragma unroll
+            for (int j = 0; j < K; ++j) E[j] = E[j] + ms[j] * sv;
             // ---------------- H half (TFSF correction first) ----------------
-            if (c.tf_lane >= 0) {
-                const R hv = lane_read(srcHc[0], ix & 63);
-                if (c.lane == c.tf_lane) add_at<R, K>(H, c.tf_j, -hv);
-            }
+            const R hv = lane_read(srcHc[0], ix & 63);
+#pragma unroll
+            for (int j = 0; j < K; ++j) H[j] = H[j] - mt[j] * hv;
         }
         const R Er = shl1(E[0]);
 #pragma unroll
