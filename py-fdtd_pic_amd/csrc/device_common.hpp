@@ -53,9 +53,8 @@ struct BeamDev {
     int32_t* start[2];   // N+2
     int32_t* range[2];   // [cmin, cmax] of the live particles' cells (empty: cmin > cmax)
     int32_t* urange[2];  // union of the live ranges so far (where J and start[] can change)
-    // per-step records at the old (sorted) index, push/count -> scatter
-    int32_t* tcell;
-    R *tfrac, *tux, *tuy, *tuz;
+    // per-step record at the old (sorted) index, push/count -> scatter (the
+    // scatter re-runs the push for the new state, bitwise the same)
     int32_t* trank;      // rank << 2 | (d + 1), d = new cell - old cell, rank among (cell, d)
     // per-cell results of the push/count pass, valid on the live range +-1
     int32_t *cup, *cdown;  // particles moving up / down out of the cell

@@ -345,7 +345,7 @@ static int upload_grow(fdtd_ctx* c, void** buf, size_t* cap, const void* src, si
 // particle beams
 // ------------------------------------------------------------------------
 enum { BO_CELL = 0, BO_FRAC = 2, BO_UX = 4, BO_UY = 6, BO_UZ = 8, BO_ID = 10, BO_START = 12, BO_RANGE = 14,
-       BO_URANGE = 16, BO_TCELL = 18, BO_TFRAC, BO_TUX, BO_TUY, BO_TUZ, BO_TRANK, BO_CUP, BO_CDOWN, BO_S, BO_END };
+       BO_URANGE = 16, BO_TRANK = 18, BO_CUP, BO_CDOWN, BO_S, BO_END };
 
 template <typename R>
 static BeamDev<R> fill_beam_dev(BeamHost& b, int32_t* err) {
@@ -366,11 +366,6 @@ static BeamDev<R> fill_beam_dev(BeamHost& b, int32_t* err) {
         d.range[k] = (int32_t*)(blk + b.off[BO_RANGE + k]);
         d.urange[k] = (int32_t*)(blk + b.off[BO_URANGE + k]);
     }
-    d.tcell = (int32_t*)(blk + b.off[BO_TCELL]);
-    d.tfrac = (R*)(blk + b.off[BO_TFRAC]);
-    d.tux = (R*)(blk + b.off[BO_TUX]);
-    d.tuy = (R*)(blk + b.off[BO_TUY]);
-    d.tuz = (R*)(blk + b.off[BO_TUZ]);
     d.trank = (int32_t*)(blk + b.off[BO_TRANK]);
     d.cup = (int32_t*)(blk + b.off[BO_CUP]);
     d.cdown = (int32_t*)(blk + b.off[BO_CDOWN]);
@@ -485,8 +480,6 @@ static int attach_t(fdtd_ctx* c, int id, SimHost& s, const fdtd_beam_desc_t* bd)
     for (int k = 0; k < 2; ++k) L.add(in);                       // id
     for (int k = 0; k < 2; ++k) L.add(cn);                       // start
     for (int k = 0; k < 4; ++k) L.add(2 * sizeof(int32_t));      // range x2, urange x2
-    L.add(in);                                                   // tcell
-    for (int k = 0; k < 4; ++k) L.add(rn);                       // tfrac tux tuy tuz
     L.add(in);                                                   // trank
     L.add(cn); L.add(cn);                                        // cup cdown
     L.add(4 * sizeof(double) * (N + 2));                         // S

@@ -16,7 +16,8 @@
 //                   particle's stable rank among (cell, d), and four FP64
 //                   sums of the CIC weights of J' at x^{n+1/2} onto nodes
 //                   c-1 .. c+2 (segmented DPP scans in a fixed order).
-//   pic_scatter     one thread per particle: the new index
+//   pic_scatter     one thread per particle: the push again (bitwise the
+//                   same: no contraction in push_one), then the new index
 //                      d=-1: start[c] - up(c-1) + r
 //                      d= 0: start[c] + down(c) + r
 //                      d=+1: start[c+1] - up(c) + down(c+1) + r
@@ -141,9 +142,14 @@ struct PushArgs {
     int N;
 };
 
+// Both PIC kernels run it on the same inputs (push/count for the moves, ranks
+// and current; scatter for the new state) and must agree bit for bit, so no
+// FMA contraction here: a contraction decision can depend on the caller
+// (a product with two uses in one kernel and one in the other).
 template <typename R>
 __device__ __forceinline__ void push_one(const PushArgs<R>& pa, int& c, R& f, R& ux, R& uy, R& uz, int& d,
                                          int& off, double& a0, double& a1) {
+#pragma clang fp contract(off)
     const int N = pa.N;
     const gptr<const R> E = pa.E, Hc = pa.Hc, Hp = pa.Hp;
     const R Eg = (R(1) - f) * E[c] + f * E[c + 1];
@@ -221,8 +227,7 @@ __global__ __launch_bounds__(64 * kPushWaves) void pic_push_count(const SimDev<R
     const gptr<const int32_t> cell = gp((const int32_t*)b.cell[par]);
     const gptr<const R> pf = gp((const R*)b.frac[par]), px = gp((const R*)b.ux[par]);
     const gptr<const R> py = gp((const R*)b.uy[par]), pz = gp((const R*)b.uz[par]);
-    const gptr<int32_t> tcell = gp(b.tcell), trank = gp(b.trank), cup = gp(b.cup), cdown = gp(b.cdown);
-    const gptr<R> tfrac = gp(b.tfrac), tux = gp(b.tux), tuy = gp(b.tuy), tuz = gp(b.tuz);
+    const gptr<int32_t> trank = gp(b.trank), cup = gp(b.cup), cdown = gp(b.cdown);
     const gptr<double> Sg = gp(b.S);
     const int cmin = rng_p[0], cmax = rng_p[1];
     if (it.y == 0 && lane == 0) {            // next step's ranges start from here
@@ -304,14 +309,7 @@ __global__ __launch_bounds__(64 * kPushWaves) void pic_push_count(const SimDev<R
                         for (int w = 0; w < 4; ++w) Sl[lk][w] += v[w];
                     }
                 }
-                if (in) {
-                    sto(tcell, i, c);
-                    sto(tfrac, i, f);
-                    sto(tux, i, ux);
-                    sto(tuy, i, uy);
-                    sto(tuz, i, uz);
-                    sto(trank, i, (r << 2) | (d + 1));
-                }
+                if (in) sto(trank, i, (r << 2) | (d + 1));
                 wave_sync();
             }
         }
@@ -343,66 +341,81 @@ __global__ __launch_bounds__(256) void pic_scatter(const SimDev<R>* __restrict__
     const SimDev<R>& sd = sims[b.sim];
     const int par = (int)((launch_idx - sd.launch_begin) & 1);
     const int po = par ^ 1;
-    const int N = b.N, np = b.np;
-    const int32_t* st = b.start[par];
-    const int cmin = b.range[par][0], cmax = b.range[par][1];
+    // every descriptor field is read here, ahead of the kernel's first store
+    // (scalar loads), as in pic_push_count
+    const int N = b.N, np = b.np, nnode = b.nnode;
+    const PushArgs<R> pa{gp((const R*)sd.E[par]), gp((const R*)sd.H[par]), gp((const R*)sd.H[par ^ 1]),
+                         b.a, b.jscale, b.Sc, N};
+    const gptr<const int32_t> st = gp((const int32_t*)b.start[par]);
+    const gptr<const int32_t> cup = gp((const int32_t*)b.cup), cdown = gp((const int32_t*)b.cdown);
+    const gptr<const int32_t> trank = gp((const int32_t*)b.trank);
+    const gptr<const int32_t> cell = gp((const int32_t*)b.cell[par]), idp = gp((const int32_t*)b.id[par]);
+    const gptr<const R> pf = gp((const R*)b.frac[par]), px = gp((const R*)b.ux[par]);
+    const gptr<const R> py = gp((const R*)b.uy[par]), pz = gp((const R*)b.uz[par]);
+    const gptr<int32_t> cello = gp(b.cell[po]), ido = gp(b.id[po]), st_o = gp(b.start[po]);
+    const gptr<R> pfo = gp(b.frac[po]), pxo = gp(b.ux[po]), pyo = gp(b.uy[po]), pzo = gp(b.uz[po]);
+    const gptr<const double> Sg = gp((const double*)b.S);
+    const gptr<int32_t> err = gp(b.err);
+    const gptr<const int32_t> rng_p = gp((const int32_t*)b.range[par]), urng_p = gp((const int32_t*)b.urange[par]);
+    const gptr<int32_t> rng_o = gp(b.range[po]), urng_o = gp(b.urange[po]);
+    const gptr<R> J = gp(const_cast<R*>(sd.J));
+    const int cmin = rng_p[0], cmax = rng_p[1];
     // counts and sums exist on [cmin-1, cmax+1] this step; zero elsewhere
     auto inr = [&](int c) { return c >= cmin - 1 && c <= cmax + 1 && cmin <= cmax; };
-    auto up = [&](int c) { return inr(c) ? b.cup[c] : 0; };
-    auto down = [&](int c) { return inr(c) ? b.cdown[c] : 0; };
+    auto up = [&](int c) { return inr(c) ? cup[c] : 0; };
+    auto down = [&](int c) { return inr(c) ? cdown[c] : 0; };
     if (bl.y >= 0) {
         const int i = bl.y + (int)threadIdx.x;
         if (i >= np) return;
-        const int c = b.cell[par][i];
-        if (c >= 2 && c <= N - 3) {
-            const int tr = b.trank[i];
+        const int c0 = ldo(cell, i);
+        int c = c0;
+        R f = ldo(pf, i), ux = ldo(px, i), uy = ldo(py, i), uz = ldo(pz, i);
+        const int id = ldo(idp, i);
+        int ni = i;                       // parked: unchanged, same index
+        if (c0 >= 2 && c0 <= N - 3) {
+            const int tr = ldo(trank, i);
             const int d = (tr & 3) - 1, r = tr >> 2;
-            int ni;
-            if (d < 0) ni = st[c] - up(c - 1) + r;
-            else if (d == 0) ni = st[c] + down(c) + r;
-            else ni = st[c + 1] - up(c) + down(c + 1) + r;
-            if (ni >= 0 && ni < np) {
-                b.cell[po][ni] = b.tcell[i];
-                b.frac[po][ni] = b.tfrac[i];
-                b.ux[po][ni] = b.tux[i];
-                b.uy[po][ni] = b.tuy[i];
-                b.uz[po][ni] = b.tuz[i];
-                b.id[po][ni] = b.id[par][i];
-            } else {
-                b.err[0] = 1;
+            int dn, off;
+            double a0, a1;
+            push_one<R>(pa, c, f, ux, uy, uz, dn, off, a0, a1);    // the push of pic_push_count, again
+            if (d < 0) ni = st[c0] - up(c0 - 1) + r;
+            else if (d == 0) ni = st[c0] + down(c0) + r;
+            else ni = st[c0 + 1] - up(c0) + down(c0 + 1) + r;
+            if (dn != d || ni < 0 || ni >= np) {
+                err[0] = 1;
+                return;
             }
-        } else {        // parked: unchanged, same index
-            b.cell[po][i] = c;
-            b.frac[po][i] = b.frac[par][i];
-            b.ux[po][i] = b.ux[par][i];
-            b.uy[po][i] = b.uy[par][i];
-            b.uz[po][i] = b.uz[par][i];
-            b.id[po][i] = b.id[par][i];
         }
+        sto(cello, ni, c);
+        sto(pfo, ni, f);
+        sto(pxo, ni, ux);
+        sto(pyo, ni, uy);
+        sto(pzo, ni, uz);
+        sto(ido, ni, id);
         return;
     }
     // node threads: J', new start[] and the next live range over the union range
     const int k = -bl.y - 1;
-    const int e0 = max(b.urange[par][0] - 3, 0), e1 = min(b.urange[par][1] + 4, N);
+    const int u0 = urng_p[0], u1 = urng_p[1];
+    const int e0 = max(u0 - 3, 0), e1 = min(u1 + 4, N);
     const int nlo = st[2] + down(2);                 // new start of cell 2 (cell 1 sends nothing up)
     const int nhi = st[N - 2] - up(N - 3);           // new start of cell N-2
-    R* J = const_cast<R*>(sd.J);
-    for (int m = e0 + k * 256 + (int)threadIdx.x; m <= e1; m += b.nnode * 256) {
-        auto Sq = [&](int c, int w) { return inr(c) ? b.S[4 * (size_t)c + w] : 0.0; };
+    for (int m = e0 + k * 256 + (int)threadIdx.x; m <= e1; m += nnode * 256) {
+        auto Sq = [&](int c, int w) { return inr(c) ? Sg[4 * (size_t)c + w] : 0.0; };
         const double j = Sq(m - 2, 3) + Sq(m - 1, 2) + Sq(m, 1) + Sq(m + 1, 0);
         J[m] = (R)j;
         const int ns = st[m] - up(m - 1) + down(m);
-        b.start[po][m] = ns;
+        st_o[m] = ns;
         if (m >= 2 && m <= N - 3) {
             const int ns1 = st[m + 1] - up(m) + down(m + 1);
             if (ns1 > ns) {
                 if (ns == nlo) {
-                    b.range[po][0] = m;
-                    b.urange[po][0] = min(b.urange[par][0], m);
+                    rng_o[0] = m;
+                    urng_o[0] = min(u0, m);
                 }
                 if (ns1 == nhi) {
-                    b.range[po][1] = m;
-                    b.urange[po][1] = max(b.urange[par][1], m);
+                    rng_o[1] = m;
+                    urng_o[1] = max(u1, m);
                 }
             }
         }

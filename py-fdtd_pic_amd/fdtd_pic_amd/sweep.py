@@ -56,8 +56,8 @@ def estimate_bytes(p, precision="f32"):
     total = slab_pass + vac_pass
     if p.get("beam"):
         n = p["beam"]["n"]
-        # (cell, frac, ux, uy, uz, id, start) x 2, records (cell, frac, u, rank), per-cell counts and sums
-        beam = 2 * (4 * n + 4 * w * n + 4 * n + 4 * (N + 2)) + 8 * n + 4 * w * n + 40 * (N + 2) + 16 * 256
+        # (cell, frac, ux, uy, uz, id, start) x 2, the rank record, per-cell counts and sums
+        beam = 2 * (4 * n + 4 * w * n + 4 * n + 4 * (N + 2)) + 4 * n + 40 * (N + 2) + 16 * 256
         total += 2 * (beam + w * (N + 1))
     return int(total)
 
