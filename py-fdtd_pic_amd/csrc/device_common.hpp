@@ -29,6 +29,10 @@ struct SimDev {
     R *probe_out;                               // nprobe x T
     R *snap;                                    // snap_rows x (N+1)
     const R *J;                                 // N+1 (PIC current, read every step)
+    // with a beam attached the field step forms J' itself from the beam's
+    // per-cell CIC sums (j_from_sums), so it does not wait for the scatter
+    const double* pS;                           // 4 per cell, or null
+    const int32_t* prange[2];                   // the beam's live range per parity
 };
 
 enum SlabKind { SLAB_NONE = 0, SLAB_DIEL = 1, SLAB_LINEAR = 2, SLAB_KERR_REF = 3, SLAB_KERR_E = 4 };
@@ -41,6 +45,7 @@ enum SlabKind { SLAB_NONE = 0, SLAB_DIEL = 1, SLAB_LINEAR = 2, SLAB_KERR_REF = 3
 constexpr int kPicGMax = 64;        // cells per push/count group (LDS tables)
 template <typename R>
 struct BeamDev {
+    int64_t launch_begin;  // the simulation's, copied: the parity without a load of the sim descriptor
     int32_t sim, N, np, G;
     int32_t nwaves;      // push/count waves of this beam (grid-stride over cell groups)
     int32_t nnode;       // scatter node blocks of this beam (grid-stride over nodes)
@@ -56,11 +61,24 @@ struct BeamDev {
     // per-step record at the old (sorted) index, push/count -> scatter (the
     // scatter re-runs the push for the new state, bitwise the same)
     int32_t* trank;      // rank << 2 | (d + 1), d = new cell - old cell, rank among (cell, d)
+    R* hold;             // H^{n-1/2} on the live range +-1, copied by push/count for the scatter's
+                         // push (the field step of the same launch may overwrite that buffer)
     // per-cell results of the push/count pass, valid on the live range +-1
     int32_t *cup, *cdown;  // particles moving up / down out of the cell
     double* S;             // 4 per cell: J' sums onto nodes c-1, c, c+1, c+2
     int32_t* err;        // set when an index check fails (never expected)
 };
+
+// J'[m] of a beam from its per-cell CIC sums S (4 per cell: onto nodes
+// c-1 .. c+2), which exist on the live range [cmin, cmax] +-1 this step:
+// J'[m] = S+2(m-2) + S+1(m-1) + S0(m) + S-1(m+1), summed in this order.
+template <typename P>
+__device__ __forceinline__ double j_from_sums(P S, int cmin, int cmax, int m) {
+    auto Sq = [&](int c, int w) {
+        return (cmin <= cmax && c >= cmin - 1 && c <= cmax + 1) ? S[4 * (size_t)c + w] : 0.0;
+    };
+    return Sq(m - 2, 3) + Sq(m - 1, 2) + Sq(m, 1) + Sq(m + 1, 0);
+}
 
 // ---- cross-lane shifts inside one wavefront (DPP, GFX9 wave_shr/wave_shl) ----
 // shr1: lane i receives lane i-1 (lane 0 receives 0); shl1: lane i receives lane i+1.

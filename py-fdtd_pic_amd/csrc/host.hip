@@ -27,9 +27,11 @@ template <typename R>
 hipError_t launch_dft(const R* const* series, const int* lens, int nseries, const double* freqs, int nf, double dt,
                       double* out, hipStream_t st);
 template <typename R>
-hipError_t launch_pic(const SimDev<R>* sims, const BeamDev<R>* beams, const int2* push_blocks, int npush,
-                      const int2* items, int nitems, long long launch_idx, hipStream_t st, hipEvent_t e0,
-                      hipEvent_t e1, hipEvent_t e2, hipEvent_t e3);
+hipError_t launch_push(const SimDev<R>* sims, const BeamDev<R>* beams, const int2* push_items, int npush,
+                       long long launch_idx, hipStream_t st, hipEvent_t e0, hipEvent_t e1);
+template <typename R>
+hipError_t launch_scatter(const SimDev<R>* sims, const BeamDev<R>* beams, const int2* scatter_blocks,
+                          int nscatter, long long launch_idx, hipStream_t st, hipEvent_t e0, hipEvent_t e1);
 }
 
 using namespace fdtd;
@@ -345,13 +347,14 @@ static int upload_grow(fdtd_ctx* c, void** buf, size_t* cap, const void* src, si
 // particle beams
 // ------------------------------------------------------------------------
 enum { BO_CELL = 0, BO_FRAC = 2, BO_UX = 4, BO_UY = 6, BO_UZ = 8, BO_ID = 10, BO_START = 12, BO_RANGE = 14,
-       BO_URANGE = 16, BO_TRANK = 18, BO_CUP, BO_CDOWN, BO_S, BO_END };
+       BO_URANGE = 16, BO_TRANK = 18, BO_CUP, BO_CDOWN, BO_S, BO_HOLD, BO_END };
 
 template <typename R>
-static BeamDev<R> fill_beam_dev(BeamHost& b, int32_t* err) {
+static BeamDev<R> fill_beam_dev(BeamHost& b, int32_t* err, int64_t launch_begin) {
     BeamDev<R> d;
     std::memset(&d, 0, sizeof(d));
     char* blk = (char*)b.block;
+    d.launch_begin = launch_begin;
     d.sim = b.sim; d.N = b.N; d.np = (int32_t)b.np; d.G = b.G;
     d.nwaves = b.nwaves; d.nnode = b.nnode;
     d.a = (R)b.a; d.jscale = (R)b.jscale; d.Sc = (R)b.Sc;
@@ -367,6 +370,7 @@ static BeamDev<R> fill_beam_dev(BeamHost& b, int32_t* err) {
         d.urange[k] = (int32_t*)(blk + b.off[BO_URANGE + k]);
     }
     d.trank = (int32_t*)(blk + b.off[BO_TRANK]);
+    d.hold = (R*)(blk + b.off[BO_HOLD]);
     d.cup = (int32_t*)(blk + b.off[BO_CUP]);
     d.cdown = (int32_t*)(blk + b.off[BO_CDOWN]);
     d.S = (double*)(blk + b.off[BO_S]);
@@ -483,6 +487,7 @@ static int attach_t(fdtd_ctx* c, int id, SimHost& s, const fdtd_beam_desc_t* bd)
     L.add(in);                                                   // trank
     L.add(cn); L.add(cn);                                        // cup cdown
     L.add(4 * sizeof(double) * (N + 2));                         // S
+    L.add(sizeof(R) * (N + 2));                                  // hold
     for (int k = 0; k < L.n; ++k) b->off[k] = L.off[k];
     b->bytes = L.total;
     if (hipMalloc(&b->block, b->bytes) != hipSuccess) {
@@ -517,6 +522,12 @@ static int rebuild_t(fdtd_ctx* c) {
         SimHost* s = c->sims[i];
         if (!s) continue;
         fill_dev<R>(*s, devs[i]);
+        if (i < (int)c->beams.size() && c->beams[i]) {     // J' formed in the field step from the CIC sums
+            const BeamHost& bh = *c->beams[i];
+            char* bb = (char*)bh.block;
+            devs[i].pS = (const double*)(bb + bh.off[BO_S]);
+            for (int k = 0; k < 2; ++k) devs[i].prange[k] = (const int32_t*)(bb + bh.off[BO_RANGE + k]);
+        }
         if (s->steps_done >= s->d.steps) continue;
         c->tile_sims.push_back(i);
         const int kind = s->d.slab_kind == FDTD_SLAB_NONE ? FDTD_SLAB_DIEL : s->d.slab_kind;
@@ -536,7 +547,7 @@ static int rebuild_t(fdtd_ctx* c) {
         BeamHost* bh = i < (int)c->beams.size() ? c->beams[i] : nullptr;
         if (!bh) continue;
         const int bi = (int)bdev.size();
-        bdev.push_back(fill_beam_dev<R>(*bh, c->d_err));
+        bdev.push_back(fill_beam_dev<R>(*bh, c->d_err, c->sims[i]->launch_begin));
         for (int w = 0; w < bh->nwaves; ++w) pitems.push_back(make_int2(bi, w));
         for (int64_t p = 0; p < bh->np; p += 256) pblocks.push_back(make_int2(bi, (int)p));
         for (int k = 0; k < bh->nnode; ++k) pblocks.push_back(make_int2(bi, -(k + 1)));
@@ -630,7 +641,7 @@ static EvPair* next_event(fdtd_ctx* c, int kind) {
 
 // ev (profiling): ev->first is stamped at the start of the first group's
 // kernel, ev->second at the end of the last one.
-static int launch_once(fdtd_ctx* c, int n, EvPair* ev) {
+static int launch_once(fdtd_ctx* c, int n, EvPair* ev, hipStream_t st) {
     const size_t ng = c->groups.size();
     for (size_t gi = 0; gi < ng; ++gi) {
         const TileGroup& g = c->groups[gi];
@@ -639,10 +650,10 @@ static int launch_once(fdtd_ctx* c, int n, EvPair* ev) {
         hipError_t e;
         if (c->prec == FDTD_F64)
             e = launch_stencil<double, false>((const SimDev<double>*)c->d_sims, c->d_tiles + g.offset, g.count,
-                                              c->launch_idx, c->gstep, n, c->K, c->S_run, g.kind, c->st, e0, e1);
+                                              c->launch_idx, c->gstep, n, c->K, c->S_run, g.kind, st, e0, e1);
         else
             e = launch_stencil<float, true>((const SimDev<float>*)c->d_sims, c->d_tiles + g.offset, g.count,
-                                            c->launch_idx, c->gstep, n, c->K, c->S_run, g.kind, c->st, e0, e1);
+                                            c->launch_idx, c->gstep, n, c->K, c->S_run, g.kind, st, e0, e1);
         if (e != hipSuccess) return fdtd_fail(FDTD_E_HIP, std::string("stencil launch: ") + hipGetErrorString(e));
     }
     return FDTD_OK;
@@ -665,9 +676,9 @@ int fdtd_run(fdtd_ctx* c, int64_t nsteps) {
         int n = (int)std::min<int64_t>(remaining, c->S_run);
         // profiling samples every profile-th launch (events cost host time)
         const bool prof = c->profile > 0 && (c->launches % c->profile) == 0;
+        // event handles are copied out at once: next_event may grow (move) the vector
+        hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
         if (c->has_pic) {
-            // event handles are copied out at once: next_event may grow (move) the vector
-            hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
             if (prof) {
                 const EvPair* pe = next_event(c, FDTD_KERNEL_PUSH);
                 if (!pe) return FDTD_E_HIP;
@@ -681,21 +692,28 @@ int fdtd_run(fdtd_ctx* c, int64_t nsteps) {
                 return fdtd_fail(FDTD_E_HIP, std::string("error pending before the pic launch at launch ") +
                                                  std::to_string(c->launch_idx) + ": " + hipGetErrorString(stale));
             hipError_t e = c->prec == FDTD_F64
-                ? launch_pic<double>((const SimDev<double>*)c->d_sims, (const BeamDev<double>*)c->d_beams,
-                                     c->d_pitems, c->npitems, c->d_pblocks, c->npblocks, c->launch_idx, c->st,
-                                     e0, e1, e2, e3)
-                : launch_pic<float>((const SimDev<float>*)c->d_sims, (const BeamDev<float>*)c->d_beams,
-                                    c->d_pitems, c->npitems, c->d_pblocks, c->npblocks, c->launch_idx, c->st,
-                                    e0, e1, e2, e3);
+                ? launch_push<double>((const SimDev<double>*)c->d_sims, (const BeamDev<double>*)c->d_beams,
+                                      c->d_pitems, c->npitems, c->launch_idx, c->st, e0, e1)
+                : launch_push<float>((const SimDev<float>*)c->d_sims, (const BeamDev<float>*)c->d_beams,
+                                     c->d_pitems, c->npitems, c->launch_idx, c->st, e0, e1);
             if (e != hipSuccess)
-                return fdtd_fail(FDTD_E_HIP, std::string("pic launch ") + std::to_string(c->launch_idx) + " (" +
-                                                 std::to_string(c->npitems) + " push waves, " +
-                                                 std::to_string(c->npblocks) + " scatter blocks): " + hipGetErrorString(e));
+                return fdtd_fail(FDTD_E_HIP, std::string("push/count launch ") + std::to_string(c->launch_idx) +
+                                                 " (" + std::to_string(c->npitems) + " waves): " + hipGetErrorString(e));
         }
         EvPair* ev = prof ? next_event(c, FDTD_KERNEL_STENCIL) : nullptr;
         if (prof && !ev) return FDTD_E_HIP;
-        int rc = launch_once(c, n, ev);
+        int rc = launch_once(c, n, ev, c->st);
         if (rc) return rc;
+        if (c->has_pic) {
+            hipError_t e = c->prec == FDTD_F64
+                ? launch_scatter<double>((const SimDev<double>*)c->d_sims, (const BeamDev<double>*)c->d_beams,
+                                         c->d_pblocks, c->npblocks, c->launch_idx, c->st, e2, e3)
+                : launch_scatter<float>((const SimDev<float>*)c->d_sims, (const BeamDev<float>*)c->d_beams,
+                                        c->d_pblocks, c->npblocks, c->launch_idx, c->st, e2, e3);
+            if (e != hipSuccess)
+                return fdtd_fail(FDTD_E_HIP, std::string("scatter launch ") + std::to_string(c->launch_idx) + " (" +
+                                                 std::to_string(c->npblocks) + " blocks): " + hipGetErrorString(e));
+        }
         rc = throttle(c);
         if (rc) return rc;
         c->launches++;

@@ -7,7 +7,8 @@
 // Units are the normalised ones of the field kernel: E, H' = Z0*Hy, and the
 // deposit writes J' = Z0*Jx (J_x*dz in the reference's units).
 //
-// Step n, before the field launch of step n (which consumes J^{n+1/2}):
+// Step n: push/count, then the field launch of step n (which forms
+// J^{n+1/2} from the CIC sums) and the scatter, which need not wait for it:
 //   pic_push_count  one wave per group of G cells of the live range (grid-
 //                   stride): per particle, gather E^n at x^n and
 //                   B^n = -(H^{n-1/2} + H^{n+1/2})/(2c) at x^n - 1/2, Boris,
@@ -213,7 +214,7 @@ __global__ __launch_bounds__(64 * kPushWaves) void pic_push_count(const SimDev<R
     if (it.x < 0) return;
     const BeamDev<R>& b = beams[it.x];
     const SimDev<R>& sd = sims[b.sim];
-    const int par = (int)((launch_idx - sd.launch_begin) & 1);
+    const int par = (int)((launch_idx - b.launch_begin) & 1);
     const int po = par ^ 1;
     const int lane = threadIdx.x & 63;
     // every descriptor field is read here, ahead of the range store below
@@ -228,6 +229,7 @@ __global__ __launch_bounds__(64 * kPushWaves) void pic_push_count(const SimDev<R
     const gptr<const R> pf = gp((const R*)b.frac[par]), px = gp((const R*)b.ux[par]);
     const gptr<const R> py = gp((const R*)b.uy[par]), pz = gp((const R*)b.uz[par]);
     const gptr<int32_t> trank = gp(b.trank), cup = gp(b.cup), cdown = gp(b.cdown);
+    const gptr<R> hold = gp(b.hold);
     const gptr<double> Sg = gp(b.S);
     const int cmin = rng_p[0], cmax = rng_p[1];
     if (it.y == 0 && lane == 0) {            // next step's ranges start from here
@@ -247,6 +249,9 @@ __global__ __launch_bounds__(64 * kPushWaves) void pic_push_count(const SimDev<R
             cnt[k][0] = cnt[k][1] = cnt[k][2] = 0;
             Sl[k][0] = Sl[k][1] = Sl[k][2] = Sl[k][3] = 0.0;
         }
+        // H^{n-1/2} of the group's cells for the scatter's push: the groups tile
+        // [cmin-1, cmax+1], the H nodes the live particles' gathers touch
+        for (int k = lane; k < g1 - g0; k += 64) hold[g0 + k] = pa.Hp[g0 + k];
         wave_sync();
         // particles of live cells only: the graveyard cells 1 and N-2 can hold
         // most of a beam once it has drifted out, and their particles neither
@@ -339,12 +344,14 @@ __global__ __launch_bounds__(256) void pic_scatter(const SimDev<R>* __restrict__
     const int2 bl = blocks[blockIdx.x];
     const BeamDev<R>& b = beams[bl.x];
     const SimDev<R>& sd = sims[b.sim];
-    const int par = (int)((launch_idx - sd.launch_begin) & 1);
+    const int par = (int)((launch_idx - b.launch_begin) & 1);
     const int po = par ^ 1;
     // every descriptor field is read here, ahead of the kernel's first store
     // (scalar loads), as in pic_push_count
     const int N = b.N, np = b.np, nnode = b.nnode;
-    const PushArgs<R> pa{gp((const R*)sd.E[par]), gp((const R*)sd.H[par]), gp((const R*)sd.H[par ^ 1]),
+    // H^{n-1/2} from push/count's copy: the field step of this launch may run
+    // alongside and overwrite H[par ^ 1]
+    const PushArgs<R> pa{gp((const R*)sd.E[par]), gp((const R*)sd.H[par]), gp((const R*)b.hold),
                          b.a, b.jscale, b.Sc, N};
     const gptr<const int32_t> st = gp((const int32_t*)b.start[par]);
     const gptr<const int32_t> cup = gp((const int32_t*)b.cup), cdown = gp((const int32_t*)b.cdown);
@@ -401,9 +408,7 @@ __global__ __launch_bounds__(256) void pic_scatter(const SimDev<R>* __restrict__
     const int nlo = st[2] + down(2);                 // new start of cell 2 (cell 1 sends nothing up)
     const int nhi = st[N - 2] - up(N - 3);           // new start of cell N-2
     for (int m = e0 + k * 256 + (int)threadIdx.x; m <= e1; m += nnode * 256) {
-        auto Sq = [&](int c, int w) { return inr(c) ? Sg[4 * (size_t)c + w] : 0.0; };
-        const double j = Sq(m - 2, 3) + Sq(m - 1, 2) + Sq(m, 1) + Sq(m + 1, 0);
-        J[m] = (R)j;
+        J[m] = (R)j_from_sums(Sg, cmin, cmax, m);     // the value the field step forms
         const int ns = st[m] - up(m - 1) + down(m);
         st_o[m] = ns;
         if (m >= 2 && m <= N - 3) {
@@ -425,38 +430,42 @@ __global__ __launch_bounds__(256) void pic_scatter(const SimDev<R>* __restrict__
 // With profiling events the kernels go through hipExtLaunchKernelGGL, which
 // stamps the events from their own dispatch packets.
 template <typename R>
-hipError_t launch_pic(const SimDev<R>* sims, const BeamDev<R>* beams, const int2* push_items, int npush,
-                      const int2* scatter_blocks, int nscatter, long long launch_idx, hipStream_t st, hipEvent_t e0,
-                      hipEvent_t e1, hipEvent_t e2, hipEvent_t e3) {
-    hipError_t e = hipSuccess;
-    if (npush > 0) {
-        const int nb = (npush + kPushWaves - 1) / kPushWaves;    // items are padded to whole blocks
-        if (e0 || e1)
-            hipExtLaunchKernelGGL((pic_push_count<R>), dim3(nb), dim3(64 * kPushWaves), 0, st, e0, e1, 0, sims,
-                                  beams, push_items, launch_idx);
-        else
-            hipLaunchKernelGGL((pic_push_count<R>), dim3(nb), dim3(64 * kPushWaves), 0, st, sims, beams,
-                               push_items, launch_idx);
-        e = hipGetLastError();
-        if (e != hipSuccess) return e;
-    }
-    if (nscatter > 0) {
-        if (e2 || e3)
-            hipExtLaunchKernelGGL((pic_scatter<R>), dim3(nscatter), dim3(256), 0, st, e2, e3, 0, sims, beams,
-                                  scatter_blocks, launch_idx);
-        else
-            hipLaunchKernelGGL((pic_scatter<R>), dim3(nscatter), dim3(256), 0, st, sims, beams, scatter_blocks,
-                               launch_idx);
-        e = hipGetLastError();
-    }
-    return e;
+hipError_t launch_push(const SimDev<R>* sims, const BeamDev<R>* beams, const int2* push_items, int npush,
+                       long long launch_idx, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
+    if (npush <= 0) return hipSuccess;
+    const int nb = (npush + kPushWaves - 1) / kPushWaves;    // items are padded to whole blocks
+    if (e0 || e1)
+        hipExtLaunchKernelGGL((pic_push_count<R>), dim3(nb), dim3(64 * kPushWaves), 0, st, e0, e1, 0, sims, beams,
+                              push_items, launch_idx);
+    else
+        hipLaunchKernelGGL((pic_push_count<R>), dim3(nb), dim3(64 * kPushWaves), 0, st, sims, beams, push_items,
+                           launch_idx);
+    return hipGetLastError();
 }
 
-template hipError_t launch_pic<double>(const SimDev<double>*, const BeamDev<double>*, const int2*, int,
-                                       const int2*, int, long long, hipStream_t, hipEvent_t, hipEvent_t,
-                                       hipEvent_t, hipEvent_t);
-template hipError_t launch_pic<float>(const SimDev<float>*, const BeamDev<float>*, const int2*, int,
-                                      const int2*, int, long long, hipStream_t, hipEvent_t, hipEvent_t,
-                                      hipEvent_t, hipEvent_t);
+// The scatter reads nothing the field step of the same launch writes
+// (H^{n-1/2} comes from the copy in hold, and both form J' from the CIC
+// sums), so it may follow that step or run beside it on another stream.
+template <typename R>
+hipError_t launch_scatter(const SimDev<R>* sims, const BeamDev<R>* beams, const int2* scatter_blocks,
+                          int nscatter, long long launch_idx, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
+    if (nscatter <= 0) return hipSuccess;
+    if (e0 || e1)
+        hipExtLaunchKernelGGL((pic_scatter<R>), dim3(nscatter), dim3(256), 0, st, e0, e1, 0, sims, beams,
+                              scatter_blocks, launch_idx);
+    else
+        hipLaunchKernelGGL((pic_scatter<R>), dim3(nscatter), dim3(256), 0, st, sims, beams, scatter_blocks,
+                           launch_idx);
+    return hipGetLastError();
+}
+
+template hipError_t launch_push<double>(const SimDev<double>*, const BeamDev<double>*, const int2*, int, long long,
+                                        hipStream_t, hipEvent_t, hipEvent_t);
+template hipError_t launch_push<float>(const SimDev<float>*, const BeamDev<float>*, const int2*, int, long long,
+                                       hipStream_t, hipEvent_t, hipEvent_t);
+template hipError_t launch_scatter<double>(const SimDev<double>*, const BeamDev<double>*, const int2*, int,
+                                           long long, hipStream_t, hipEvent_t, hipEvent_t);
+template hipError_t launch_scatter<float>(const SimDev<float>*, const BeamDev<float>*, const int2*, int, long long,
+                                          hipStream_t, hipEvent_t, hipEvent_t);
 
 }  // namespace fdtd

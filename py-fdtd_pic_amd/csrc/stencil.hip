@@ -183,6 +183,8 @@ struct TileCtx {
     R kD, jD;
     double chi3;
     const R *srcE, *srcH, *Jc;
+    const double* pS;       // beam CIC sums (J' formed here), or null: J' read from Jc
+    int pcmin, pcmax;
     R *probe_out, *snap;
 };
 
@@ -298,7 +300,13 @@ __device__ __forceinline__ void run_steps(TILE_PARAMS, const TileCtx<R>& c, cons
 #pragma unroll
         for (int j = 0; j < K; ++j) Jv[j] = R(0);
         if constexpr (GEN) {
-            if (c.Jc) {
+            if (c.pS) {
+#pragma unroll
+                for (int j = 0; j < K; ++j) {
+                    const int n = c.base + j;
+                    if (n >= 0 && n <= c.N) Jv[j] = (R)j_from_sums(c.pS, c.pcmin, c.pcmax, n);
+                }
+            } else if (c.Jc) {
 #pragma unroll
                 for (int j = 0; j < K; ++j) {
                     const int n = c.base + j;
@@ -420,6 +428,13 @@ __device__ __forceinline__ void tile_body(const SimDev<R>& sd, int out0_, long l
     c.kD = sd.kD; c.jD = sd.jD; c.chi3 = sd.kerr_chi3;
     c.nprobe = sd.nprobe; c.snap_interval = sd.snap_interval; c.snap_rows = sd.snap_rows;
     c.srcE = sd.srcE; c.srcH = sd.srcH; c.Jc = sd.J; c.probe_out = sd.probe_out; c.snap = sd.snap;
+    c.pS = sd.pS;
+    c.pcmin = 1;
+    c.pcmax = 0;
+    if (c.pS) {
+        c.pcmin = sd.prange[par][0];
+        c.pcmax = sd.prange[par][1];
+    }
     // valid local steps of this launch: t = gstep0 + s - t_begin in [0, T)
     c.s_lo = (int)max(0LL, c.t_begin - gstep0);
     c.s_hi = (int)min((long long)nsteps, (long long)c.T + c.t_begin - gstep0);
