@@ -21,6 +21,8 @@
 //   probes / snapshots (probeSim :16-54, vidMake :57-68)
 #include <hip/hip_ext.h>
 
+#include <type_traits>
+
 #include "device_common.hpp"
 
 namespace fdtd {
@@ -92,13 +94,19 @@ __device__ __forceinline__ float kerr_delta_f32(float z) {
     t = fmaf(t, -z, 1.0f);
     return -z * t;
 }
+// z of the small-z test: FP32 in the FP32 mode, where z only selects the
+// branch (the 1e-5 bar leaves no need for an FP64 copy).
 template <typename R>
-__device__ __forceinline__ R kerr_series_e(double x, double kz, double& z) {
+using KerrZ = typename std::conditional<sizeof(R) == 4, float, double>::type;
+
+template <typename R>
+__device__ __forceinline__ R kerr_series_e(KerrZ<R> x, double kz, KerrZ<R>& z) {
     if constexpr (sizeof(R) == 4) {
-        const float xf = (float)x;
-        const float zf = (float)kz * xf * xf;
-        z = (double)zf;
-        return (R)fma(x, (double)kerr_delta_f32(zf), x);
+        // E = x (1 + delta) rounded once in FP32: E is stored in FP32 and
+        // formed again from D - P every step, so nothing accumulates
+        const float zf = (float)kz * x * x;
+        z = zf;
+        return fmaf(x, kerr_delta_f32(zf), x);
     } else {
         z = kz * x * x;
         return (R)(x * kerr_series<R>(z));
@@ -110,8 +118,8 @@ __device__ __forceinline__ R kerr_series_e(double x, double kz, double& z) {
 // (ADE_PolarisationCurrent_Ex :620-631); FP32 keeps the velocity form
 // V' = -B V - (1-A-B) P + C E, P += V' (sA = -B, sB = 1-A-B).
 template <typename R, bool KAHAN>
-__device__ __forceinline__ double slab_u(R kD, R jD, R dH, R Jv, R E, R& D, R& cD, R& P, R& Pm, R& cP,
-                                         R sA, R sB, R sC) {
+__device__ __forceinline__ KerrZ<R> slab_u(R kD, R jD, R dH, R Jv, R E, R& D, R& cD, R& P, R& Pm, R& cP,
+                                           R sA, R sB, R sC) {
     if constexpr (sizeof(R) == 4) {
         const R Vn = sA * Pm - sB * P + sC * E;
         Pm = Vn;
@@ -123,11 +131,12 @@ __device__ __forceinline__ double slab_u(R kD, R jD, R dH, R Jv, R E, R& D, R& c
     }
     const R inc = (dH - jD * Jv) * kD;
     if (KAHAN) kahan_add(D, cD, inc); else D = D + inc;
-    // FP32: u = (D - cD) - (P - cP) from the compensated values, the hi part
-    // formed in FP32 (exact when D and P are within a factor 2, else rounded
-    // like the stored E), the compensation added in FP64.  FP64: cD = cP = 0.
+    // FP32: u = (D - cD) - (P - cP) from the compensated values, in FP32.
+    // D - P is exact when D and P are within a factor 2 (the cancelling case)
+    // and otherwise rounded relative to a u of their size, so the rounded sum
+    // is within ~1e-7 of u itself.  FP64: cD = cP = 0.
     if constexpr (sizeof(R) == 4)
-        return KAHAN ? (double)(D - P) + (double)(cP - cD) : (double)(D - P);
+        return KAHAN ? (D - P) + (cP - cD) : (D - P);
     else
         return D - P;
 }
@@ -147,15 +156,17 @@ __device__ __forceinline__ R slab_e_simple(const SimDev<R>& sd, double u, R D, R
 // Cells whose z reached the series cut take the closed form; one wave-level
 // test per step keeps the common path free of per-cell branches.
 template <typename R, int K>
-__device__ __forceinline__ void kerr_fixup(R (&E)[K], const double (&x)[K], const double (&z)[K],
+__device__ __forceinline__ void kerr_fixup(R (&E)[K], const KerrZ<R> (&x)[K], const KerrZ<R> (&z)[K],
                                            const double (&ied)[K], double chi3) {
-    double zmax = z[0];
+    using Z = KerrZ<R>;
+    constexpr Z cut = (Z)kKerrSeriesCut;
+    Z zmax = z[0];
 #pragma unroll
-    for (int j = 1; j < K; ++j) zmax = fmax(zmax, z[j]);
-    if (__any(zmax >= kKerrSeriesCut)) {
+    for (int j = 1; j < K; ++j) zmax = zmax > z[j] ? zmax : z[j];
+    if (__any(zmax >= cut)) {
 #pragma unroll
         for (int j = 0; j < K; ++j)
-            if (z[j] >= kKerrSeriesCut) E[j] = (R)kerr_root_e(x[j] / ied[j], 1.0 / ied[j], chi3);
+            if (z[j] >= cut) E[j] = (R)kerr_root_e((double)x[j] / ied[j], 1.0 / ied[j], chi3);
     }
 }
 
@@ -203,16 +214,16 @@ enum : int {
 template <typename R, int K, bool KAHAN, int KIND>
 __device__ __forceinline__ void e_pure_slab(TILE_PARAMS, const TileCtx<R>& c, const SimDev<R>& sd,
                                             const R (&dH)[K], const R (&Jv)[K]) {
-    double x[K], z[K];
+    KerrZ<R> x[K], z[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) {
-        const double u = slab_u<R, KAHAN>(c.kD, c.jD, dH[j], Jv[j], E[j], D[j], cD[j], P[j], Pm[j],
-                                          cP[j], sA[j], sB[j], sC[j]);
+        const KerrZ<R> u = slab_u<R, KAHAN>(c.kD, c.jD, dH[j], Jv[j], E[j], D[j], cD[j], P[j], Pm[j],
+                                            cP[j], sA[j], sB[j], sC[j]);
         if constexpr (KIND == SLAB_KERR_E) {
-            x[j] = u * ied[j];
+            x[j] = u * ie[j];          // ie = 1/eps_inf in R (FP64: the same value as ied)
             E[j] = kerr_series_e<R>(x[j], kz[j], z[j]);
         } else {
-            E[j] = slab_e_simple<R, KIND>(sd, u, D[j], ie[j]);
+            E[j] = slab_e_simple<R, KIND>(sd, (double)u, D[j], ie[j]);
         }
     }
     if constexpr (KIND == SLAB_KERR_E) kerr_fixup<R, K>(E, x, z, ied, c.chi3);
@@ -234,20 +245,21 @@ __device__ __forceinline__ void e_mixed(TILE_PARAMS, const TileCtx<R>& c, const 
             Ev[j] = Ev[j] - CbE[j] * psE[j];
         }
     }
-    double x[K], z[K];
+    KerrZ<R> x[K], z[K];
     R Es[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) {
         const bool sl = (slab_mask >> j) & 1u;
         R Dn = D[j], cDn = cD[j], Pn = P[j], Pmn = Pm[j], cPn = cP[j];
-        const double u = slab_u<R, KAHAN>(c.kD, c.jD, dH[j], Jv[j], E[j], Dn, cDn, Pn, Pmn, cPn, sA[j],
-                                          sB[j], sC[j]);
+        const KerrZ<R> u = slab_u<R, KAHAN>(c.kD, c.jD, dH[j], Jv[j], E[j], Dn, cDn, Pn, Pmn, cPn, sA[j],
+                                            sB[j], sC[j]);
         if constexpr (KIND == SLAB_KERR_E) {
-            x[j] = u * ied[j];
+            x[j] = u * ie[j];
             Es[j] = kerr_series_e<R>(x[j], sl ? kz[j] : 0.0, z[j]);
         } else {
-            x[j] = z[j] = 0.0;
-            Es[j] = slab_e_simple<R, KIND>(sd, u, Dn, ie[j]);
+            x[j] = 0;
+            z[j] = 0;
+            Es[j] = slab_e_simple<R, KIND>(sd, (double)u, Dn, ie[j]);
         }
         D[j] = sl ? Dn : D[j]; cD[j] = sl ? cDn : cD[j];
         P[j] = sl ? Pn : P[j]; Pm[j] = sl ? Pmn : Pm[j]; cP[j] = sl ? cPn : cP[j];

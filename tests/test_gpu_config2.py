@@ -2,7 +2,8 @@
 per-cell eps_inf and w0, chi3 = 1e-19, 1.8e8 V/m source) against the FP64
 C oracle (oracle/fdtd_cpu.c, bitwise equal to oracle/physical.py).
 
-FP32 (Kahan-compensated D and P, FP64 Kerr root): relative L2 <= 1e-5 on the
+FP32 (Kahan-compensated D and P; u = D - P, the small-z Kerr correction and
+E in FP32, the closed-form Kerr root in FP64): relative L2 <= 1e-5 on the
 fields E, H and the probe series.  At 200 THz most slab cells have eps < 0, so
 at the end of the run the slab holds only an evanescent remnant (~3e-5 of the
 field scale): D' and P' (V/m, same units as E) are compared relative to the
