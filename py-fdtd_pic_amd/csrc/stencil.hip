@@ -306,7 +306,9 @@ __device__ __forceinline__ void run_steps(TILE_PARAMS, const TileCtx<R>& c, cons
     if constexpr (PROBE) {
         pacc[0] = pacc[1] = R(0);
     }
-    for (int s = c.s_lo; s < c.s_hi; ++s) {
+    // the step body, called twice per loop trip: with two copies the
+    // loop-carried registers alternate instead of being moved back each step
+    auto step = [&](const int s) {
         const int ix = s - c.s_lo;
         R Jv[K];
 #pragma unroll
@@ -411,7 +413,13 @@ __device__ __forceinline__ void run_steps(TILE_PARAMS, const TileCtx<R>& c, cons
                 }
             }
         }
+    };
+    int s = c.s_lo;
+    for (; s + 1 < c.s_hi; s += 2) {
+        step(s);
+        step(s + 1);
     }
+    if (s < c.s_hi) step(s);
 }
 
 // One tile: 64 lanes x K cells, S steps (the whole per-tile program).
