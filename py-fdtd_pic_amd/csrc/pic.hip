@@ -73,6 +73,23 @@ __device__ __forceinline__ void seg_scan4(double (&v)[4], int lane, int fl) {
 #undef FDTD_SEG_STEP
 }
 
+// Inclusive scan of four FP64 values over the whole wave (the segmented scan
+// with every lane in one segment, without its selects: out-of-row DPP
+// sources read 0 and masked rows receive 0); lane 63 holds the totals.
+__device__ __forceinline__ void wave_scan4(double (&v)[4]) {
+#define FDTD_SCAN_STEP(CTRL, RM)                                                      \
+    {                                                                                 \
+        _Pragma("unroll") for (int q = 0; q < 4; ++q) v[q] += dpp_f64<CTRL, RM>(v[q]); \
+    }
+    FDTD_SCAN_STEP(0x111, 0xf)
+    FDTD_SCAN_STEP(0x112, 0xf)
+    FDTD_SCAN_STEP(0x114, 0xf)
+    FDTD_SCAN_STEP(0x118, 0xf)
+    FDTD_SCAN_STEP(0x142, 0xa)
+    FDTD_SCAN_STEP(0x143, 0xc)
+#undef FDTD_SCAN_STEP
+}
+
 // The sort/deposit workgroup is a single wave: LDS accesses then execute in
 // program order, so a wavefront-scope fence (a compiler barrier, no wait on
 // outstanding global loads) orders the cross-lane LDS traffic; a workgroup
@@ -128,10 +145,15 @@ __device__ __forceinline__ void sto(gptr<T> p, unsigned i, T v) {
     *(gptr<T>)((gptr<char>)p + i * (unsigned)sizeof(T)) = v;
 }
 
-// 1/x: the hardware reciprocal in FP32 (1 ulp; the FP32 mode's tolerance is
-// 1e-5), an IEEE division in FP64.
+// 1/x and 1/sqrt(x): the hardware reciprocal and reciprocal square root in
+// FP32 (~1 ulp; the FP32 mode's tolerance is 1e-5), IEEE operations in FP64.
 __device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double recip(double x) { return 1.0 / x; }
+__device__ __forceinline__ float rsqrt_r(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ double rsqrt_r(double x) { return 1.0 / sqrt(x); }
+// 2t / (1 + t^2) of the Boris rotation: a reciprocal in FP32, a division in FP64
+__device__ __forceinline__ float boris_s(float t) { return 2.0f * t * __builtin_amdgcn_rcpf(1.0f + t * t); }
+__device__ __forceinline__ double boris_s(double t) { return 2.0 * t / (1.0 + t * t); }
 
 // Boris push of one live particle; returns d and the CIC weights (node offset off = m0 + 1).
 // What the push reads of the descriptors, fetched once per wave before the
@@ -149,29 +171,29 @@ struct PushArgs {
 // (a product with two uses in one kernel and one in the other).
 template <typename R>
 __device__ __forceinline__ void push_one(const PushArgs<R>& pa, int& c, R& f, R& ux, R& uy, R& uz, int& d,
-                                         int& off, double& a0, double& a1) {
+                                         int& off, R& a0, R& a1) {
 #pragma clang fp contract(off)
     const int N = pa.N;
     const gptr<const R> E = pa.E, Hc = pa.Hc, Hp = pa.Hp;
-    const R Eg = (R(1) - f) * E[c] + f * E[c + 1];
+    const R Eg = (R(1) - f) * ldo(E, c) + f * ldo(E, c + 1);
     int j;
     R g;
     if (f >= R(0.5)) { j = c; g = f - R(0.5); } else { j = c - 1; g = f + R(0.5); }
-    const R h0 = R(0.5) * (Hp[j] + Hc[j]);
-    const R h1 = R(0.5) * (Hp[j + 1] + Hc[j + 1]);
+    const R h0 = R(0.5) * (ldo(Hp, j) + ldo(Hc, j));
+    const R h1 = R(0.5) * (ldo(Hp, j + 1) + ldo(Hc, j + 1));
     const R Hg = (R(1) - g) * h0 + g * h1;
     // Boris: E = (Eg, 0, 0), B = (0, -Hg/c, 0)
     const R aE = pa.a * Eg;
     const R umx = ux + aE, umy = uy, umz = uz;
-    const R gm = sqrt(R(1) + umx * umx + umy * umy + umz * umz);
-    const R ty = -pa.a * Hg * recip(gm);
-    const R sy = R(2) * ty / (R(1) + ty * ty);
+    const R igm = rsqrt_r(R(1) + umx * umx + umy * umy + umz * umz);     // 1 / gamma^-
+    const R ty = -pa.a * Hg * igm;
+    const R sy = boris_s(ty);
     const R upx = umx - umz * ty;
     const R upz = umz + umx * ty;
     ux = umx - upz * sy + aE;
     uz = umz + upx * sy;
     uy = umy;
-    const R rg = recip(sqrt(R(1) + ux * ux + uy * uy + uz * uz));
+    const R rg = rsqrt_r(R(1) + ux * ux + uy * uy + uz * uz);
     const R bx = ux * rg, bz = uz * rg;
     const R dx = pa.Sc * bz;
     // CIC of J' at x^{n+1/2}, relative to cell c: xh in (-1/4, 5/4)
@@ -179,8 +201,8 @@ __device__ __forceinline__ void push_one(const PushArgs<R>& pa, int& c, R& f, R&
     const int m0 = xh < R(0) ? -1 : (xh >= R(1) ? 1 : 0);
     const R w1 = xh - (R)m0;
     const R q = pa.jscale * bx;
-    a0 = (double)(q * (R(1) - w1));
-    a1 = (double)(q * w1);
+    a0 = q * (R(1) - w1);
+    a1 = q * w1;
     off = m0 + 1;
     // new position
     R nf = f + dx;
@@ -281,7 +303,7 @@ __global__ __launch_bounds__(64 * kPushWaves) void pic_push_count(const SimDev<R
                 const Chunk k = chunk_info(c_old, i, phi, lane);
                 int c = c_old, d = 0, off = 1;
                 R f = ff[q], ux = xx[q], uy = yy[q], uz = zz[q];
-                double a0 = 0.0, a1 = 0.0;
+                R a0 = R(0), a1 = R(0);
                 const bool live = k.valid && c_old >= 2 && c_old <= N - 3;
                 if (live) push_one<R>(pa, c, f, ux, uy, uz, d, off, a0, a1);
                 const unsigned long long seg = lanes_le(lane) & ~lanes_lt(k.fl);
@@ -290,14 +312,22 @@ __global__ __launch_bounds__(64 * kPushWaves) void pic_push_count(const SimDev<R
                 const unsigned long long bd2 = __ballot(k.valid && d == 1);
                 const unsigned long long mine = d < 0 ? bd0 : (d == 0 ? bd1 : bd2);
                 double v[4];
-#pragma unroll
-                for (int w = 0; w < 4; ++w) v[w] = (w == off ? a0 : 0.0) + (w == off + 1 ? a1 : 0.0);
                 if (G == 1) {
                     // one cell per group: each lane accumulates its particles
-                    // in chunk order, one fixed-order reduction after the group
-#pragma unroll
-                    for (int w = 0; w < 4; ++w) acc[w] += v[w];
+                    // in chunk order (weights onto nodes c-1+off, c+off), one
+                    // fixed-order reduction after the group
+                    const R s0 = off == 0 ? a0 : R(0);
+                    const R s1 = off == 0 ? a1 : (off == 1 ? a0 : R(0));
+                    const R s2 = off == 1 ? a1 : (off == 2 ? a0 : R(0));
+                    const R s3 = off == 2 ? a1 : R(0);
+                    acc[0] += (double)s0;
+                    acc[1] += (double)s1;
+                    acc[2] += (double)s2;
+                    acc[3] += (double)s3;
                 } else {
+#pragma unroll
+                    for (int w = 0; w < 4; ++w)
+                        v[w] = (w == off ? (double)a0 : 0.0) + (w == off + 1 ? (double)a1 : 0.0);
                     seg_scan4(v, lane, k.fl);
                 }
                 const int lk = k.valid ? c_old - g0 : -1;
@@ -319,7 +349,7 @@ __global__ __launch_bounds__(64 * kPushWaves) void pic_push_count(const SimDev<R
             }
         }
         if (G == 1) {
-            seg_scan4(acc, lane, 0);        // lane 63 holds the wave sums
+            wave_scan4(acc);                // lane 63 holds the wave sums
             if (lane == 63) {
 #pragma unroll
                 for (int w = 0; w < 4; ++w) Sl[0][w] = acc[w];
@@ -383,7 +413,7 @@ __global__ __launch_bounds__(256) void pic_scatter(const SimDev<R>* __restrict__
             const int tr = ldo(trank, i);
             const int d = (tr & 3) - 1, r = tr >> 2;
             int dn, off;
-            double a0, a1;
+            R a0, a1;
             push_one<R>(pa, c, f, ux, uy, uz, dn, off, a0, a1);    // the push of pic_push_count, again
             if (d < 0) ni = st[c0] - up(c0 - 1) + r;
             else if (d == 0) ni = st[c0] + down(c0) + r;
