@@ -267,32 +267,35 @@ __global__ __launch_bounds__(64 * kPushWaves) void pic_push_count(const SimDev<R
     for (int gi = it.y; gi < ngroups; gi += nwaves) {
         const int g0 = lo_cell + gi * G;
         const int g1 = min(g0 + G, hi_cell + 1);
+        // particles of live cells only: the graveyard cells 1 and N-2 can hold
+        // most of a beam once it has drifted out, and their particles neither
+        // move nor deposit (their cells' counts stay 0 below).  Both bounds
+        // come in with one load (lane 0 and lane 1), read back with readlane.
+        const int sb = ldo(st, (unsigned)(lane == 0 ? max(g0, 2) : min(g1, N - 2)));
         for (int k = lane; k < G; k += 64) {
             cnt[k][0] = cnt[k][1] = cnt[k][2] = 0;
             Sl[k][0] = Sl[k][1] = Sl[k][2] = Sl[k][3] = 0.0;
         }
-        // H^{n-1/2} of the group's cells for the scatter's push: the groups tile
-        // [cmin-1, cmax+1], the H nodes the live particles' gathers touch
-        for (int k = lane; k < g1 - g0; k += 64) hold[g0 + k] = pa.Hp[g0 + k];
         wave_sync();
-        // particles of live cells only: the graveyard cells 1 and N-2 can hold
-        // most of a beam once it has drifted out, and their particles neither
-        // move nor deposit (their cells' counts stay 0 below)
-        const int plo = clampi(st[max(g0, 2)]);
-        const int phi = max(plo, clampi(st[min(g1, N - 2)]));
+        const int plo = clampi(__builtin_amdgcn_readlane(sb, 0));
+        const int phi = max(plo, clampi(__builtin_amdgcn_readlane(sb, 1)));
         double acc[4] = {0.0, 0.0, 0.0, 0.0};
         for (int bb = plo; bb < phi; bb += 64 * kBatch) {
             int cc[kBatch];
             R ff[kBatch], xx[kBatch], yy[kBatch], zz[kBatch];
 #pragma unroll
             for (int q = 0; q < kBatch; ++q) {
+                // unconditional loads at an in-range index (np >= 1), then selects
                 const int i = bb + 64 * q + lane;
                 const bool v = i < phi;
-                cc[q] = v ? ldo(cell, i) : 0x7fffffff;
-                ff[q] = v ? ldo(pf, i) : R(0);
-                xx[q] = v ? ldo(px, i) : R(0);
-                yy[q] = v ? ldo(py, i) : R(0);
-                zz[q] = v ? ldo(pz, i) : R(0);
+                const unsigned ic = (unsigned)min(i, np - 1);
+                const int c_ = ldo(cell, ic);
+                const R f_ = ldo(pf, ic), x_ = ldo(px, ic), y_ = ldo(py, ic), z_ = ldo(pz, ic);
+                cc[q] = v ? c_ : 0x7fffffff;
+                ff[q] = v ? f_ : R(0);
+                xx[q] = v ? x_ : R(0);
+                yy[q] = v ? y_ : R(0);
+                zz[q] = v ? z_ : R(0);
             }
 #pragma unroll
             for (int q = 0; q < kBatch; ++q) {
@@ -362,6 +365,9 @@ __global__ __launch_bounds__(64 * kPushWaves) void pic_push_count(const SimDev<R
             cdown[c] = cnt[k][0];
 #pragma unroll
             for (int w = 0; w < 4; ++w) Sg[4 * (size_t)c + w] = Sl[k][w];
+            // H^{n-1/2} of the group's cells for the scatter's push: the groups
+            // tile [cmin-1, cmax+1], the H nodes the live particles' gathers touch
+            hold[c] = pa.Hp[c];
         }
         wave_sync();
     }
