@@ -16,7 +16,9 @@
 //                   moving down / up (|d| <= 1 since Sc*beta < 1), each
 //                   particle's stable rank among (cell, d), and four FP64
 //                   sums of the CIC weights of J' at x^{n+1/2} onto nodes
-//                   c-1 .. c+2 (segmented DPP scans in a fixed order).
+//                   c-1 .. c+2 (one-cell groups: per-lane sums and a wave
+//                   scan; larger groups: segmented DPP scans; a fixed order
+//                   either way).
 //   pic_scatter     one thread per particle: the push again (bitwise the
 //                   same: no contraction in push_one), then the new index
 //                      d=-1: start[c] - up(c-1) + r
@@ -90,10 +92,11 @@ __device__ __forceinline__ void wave_scan4(double (&v)[4]) {
 #undef FDTD_SCAN_STEP
 }
 
-// The sort/deposit workgroup is a single wave: LDS accesses then execute in
-// program order, so a wavefront-scope fence (a compiler barrier, no wait on
-// outstanding global loads) orders the cross-lane LDS traffic; a workgroup
-// barrier would also drain the prefetched loads of the next chunk.
+// Each push/count wave owns its own LDS tables (the workgroup's waves never
+// share them), and a wave's LDS accesses execute in program order, so a
+// wavefront-scope fence (a compiler barrier, no wait on outstanding global
+// loads) orders the cross-lane LDS traffic; a workgroup barrier would also
+// drain the prefetched loads of the next chunk.
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
