@@ -275,6 +275,72 @@ __global__ __launch_bounds__(64 * kPushWaves) void pic_push_count(const SimDev<R
         // move nor deposit (their cells' counts stay 0 below).  Both bounds
         // come in with one load (lane 0 and lane 1), read back with readlane.
         const int sb = ldo(st, (unsigned)(lane == 0 ? max(g0, 2) : min(g1, N - 2)));
+        if (G == 1) {
+            // One cell: every particle of the range is in cell g0, so the
+            // per-class carries are wave-uniform scalars, a particle's rank is
+            // its class carry plus the earlier lanes of its class (mbcnt), and
+            // no LDS table is needed.  Same ranks, counts and sums (same order)
+            // as the general path below.
+            const int plo1 = clampi(__builtin_amdgcn_readlane(sb, 0));
+            const int phi1 = max(plo1, clampi(__builtin_amdgcn_readlane(sb, 1)));
+            int n0 = 0, n1 = 0, n2 = 0;        // particles moving down / staying / up so far
+            double acc1[4] = {0.0, 0.0, 0.0, 0.0};
+            for (int bb = plo1; bb < phi1; bb += 64 * kBatch) {
+                int cc[kBatch];
+                R ff[kBatch], xx[kBatch], yy[kBatch], zz[kBatch];
+#pragma unroll
+                for (int q = 0; q < kBatch; ++q) {
+                    const unsigned ic = (unsigned)min(bb + 64 * q + lane, np - 1);
+                    cc[q] = ldo(cell, ic);
+                    ff[q] = ldo(pf, ic);
+                    xx[q] = ldo(px, ic);
+                    yy[q] = ldo(py, ic);
+                    zz[q] = ldo(pz, ic);
+                }
+#pragma unroll
+                for (int q = 0; q < kBatch; ++q) {
+                    const int base = bb + 64 * q;
+                    if (base >= phi1) break;
+                    const int i = base + lane;
+                    const bool valid = i < phi1;
+                    int c = cc[q], d = 0, off = 1;
+                    R f = ff[q], ux = xx[q], uy = yy[q], uz = zz[q];
+                    R a0 = R(0), a1 = R(0);
+                    if (valid) push_one<R>(pa, c, f, ux, uy, uz, d, off, a0, a1);
+                    const unsigned long long bd0 = __ballot(valid && d == -1);
+                    const unsigned long long bd1 = __ballot(valid && d == 0);
+                    const unsigned long long bd2 = __ballot(valid && d == 1);
+                    const unsigned long long mine = d < 0 ? bd0 : (d == 0 ? bd1 : bd2);
+                    const int carry = d < 0 ? n0 : (d == 0 ? n1 : n2);
+                    const int r = carry + (int)__builtin_amdgcn_mbcnt_hi(
+                                              (unsigned)(mine >> 32),
+                                              __builtin_amdgcn_mbcnt_lo((unsigned)mine, 0u));
+                    if (valid) sto(trank, i, (r << 2) | (d + 1));
+                    const R s0 = off == 0 ? a0 : R(0);
+                    const R s1 = off == 0 ? a1 : (off == 1 ? a0 : R(0));
+                    const R s2 = off == 1 ? a1 : (off == 2 ? a0 : R(0));
+                    const R s3 = off == 2 ? a1 : R(0);
+                    acc1[0] += (double)s0;
+                    acc1[1] += (double)s1;
+                    acc1[2] += (double)s2;
+                    acc1[3] += (double)s3;
+                    n0 += __popcll(bd0);
+                    n1 += __popcll(bd1);
+                    n2 += __popcll(bd2);
+                }
+            }
+            wave_scan4(acc1);                   // lane 63 holds the cell's sums
+            if (lane == 63) {
+#pragma unroll
+                for (int w = 0; w < 4; ++w) Sg[4 * (size_t)g0 + w] = acc1[w];
+            }
+            if (lane == 0) {
+                cup[g0] = n2;
+                cdown[g0] = n0;
+                hold[g0] = pa.Hp[g0];
+            }
+            continue;
+        }
         for (int k = lane; k < G; k += 64) {
             cnt[k][0] = cnt[k][1] = cnt[k][2] = 0;
             Sl[k][0] = Sl[k][1] = Sl[k][2] = Sl[k][3] = 0.0;
