@@ -110,6 +110,18 @@ def test_alone_equals_batch_bitwise(batch_jobs):
             assert np.array_equal(a.X[k], j.X[k]), (j.id, k)
 
 
+def test_busy_reports_queued_work():
+    """fdtd_busy: set while queued launches run, clear after fdtd_sync."""
+    from fdtd_pic_amd import configs, engine
+    p = configs.config2(steps=9600)   # 200 launches of ~15 us each: the device lags the host
+    with engine.Context("f32", K=2, S=48) as ctx:
+        ctx.add(engine.physical_spec(p))
+        ctx.run(9600)
+        seen = ctx.busy()
+        ctx.sync()
+        assert seen and not ctx.busy()
+
+
 def test_admission_while_running_bitwise():
     """Jobs admitted while slices of others run (upload stream, first chunk of
     one job) give the same spectra, bitwise, as each job alone."""
