@@ -54,7 +54,7 @@ def parse():
     ap.add_argument("--c4-sample", type=int, default=24, help="instances timed in the per-simulation loop")
     ap.add_argument("--c4-order", default="arrival", choices=["costly", "arrival"],
                     help="batched admission order: costly jobs first, or as listed")
-    ap.add_argument("--c5-requests", type=int, default=8, help="requests of the config 5 trace replayed")
+    ap.add_argument("--c5-requests", type=int, default=32, help="requests of the config 5 trace replayed")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 256 MiB device copy (FETCH_SIZE/WRITE_SIZE calibration under rocprofv3 --pmc)")
     return ap.parse_args()

@@ -61,6 +61,8 @@ struct BeamDev {
     // per-step record at the old (sorted) index, push/count -> scatter (the
     // scatter re-runs the push for the new state, bitwise the same)
     int32_t* trank;      // rank << 2 | (d + 1), d = new cell - old cell, rank among (cell, d)
+    int32_t* park;       // [2]: the previous step's start[2] and start[N-2] (push/count -> scatter):
+                         // particles below / at or above them are parked and in both buffers already
     R* hold;             // H^{n-1/2} on the live range +-1, copied by push/count for the scatter's
                          // push (the field step of the same launch may overwrite that buffer)
     // per-cell results of the push/count pass, valid on the live range +-1

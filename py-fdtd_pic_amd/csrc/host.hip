@@ -347,7 +347,7 @@ static int upload_grow(fdtd_ctx* c, void** buf, size_t* cap, const void* src, si
 // particle beams
 // ------------------------------------------------------------------------
 enum { BO_CELL = 0, BO_FRAC = 2, BO_UX = 4, BO_UY = 6, BO_UZ = 8, BO_ID = 10, BO_START = 12, BO_RANGE = 14,
-       BO_URANGE = 16, BO_TRANK = 18, BO_CUP, BO_CDOWN, BO_S, BO_HOLD, BO_END };
+       BO_URANGE = 16, BO_TRANK = 18, BO_CUP, BO_CDOWN, BO_S, BO_HOLD, BO_PARK, BO_END };
 
 template <typename R>
 static BeamDev<R> fill_beam_dev(BeamHost& b, int32_t* err, int64_t launch_begin) {
@@ -371,6 +371,7 @@ static BeamDev<R> fill_beam_dev(BeamHost& b, int32_t* err, int64_t launch_begin)
     }
     d.trank = (int32_t*)(blk + b.off[BO_TRANK]);
     d.hold = (R*)(blk + b.off[BO_HOLD]);
+    d.park = (int32_t*)(blk + b.off[BO_PARK]);
     d.cup = (int32_t*)(blk + b.off[BO_CUP]);
     d.cdown = (int32_t*)(blk + b.off[BO_CDOWN]);
     d.S = (double*)(blk + b.off[BO_S]);
@@ -488,6 +489,7 @@ static int attach_t(fdtd_ctx* c, int id, SimHost& s, const fdtd_beam_desc_t* bd)
     L.add(cn); L.add(cn);                                        // cup cdown
     L.add(4 * sizeof(double) * (N + 2));                         // S
     L.add(sizeof(R) * (N + 2));                                  // hold
+    L.add(2 * sizeof(int32_t));                                  // park
     for (int k = 0; k < L.n; ++k) b->off[k] = L.off[k];
     b->bytes = L.total;
     if (hipMalloc(&b->block, b->bytes) != hipSuccess) {

@@ -255,6 +255,8 @@ __global__ __launch_bounds__(64 * kPushWaves) void pic_push_count(const SimDev<R
     const gptr<const R> py = gp((const R*)b.uy[par]), pz = gp((const R*)b.uz[par]);
     const gptr<int32_t> trank = gp(b.trank), cup = gp(b.cup), cdown = gp(b.cdown);
     const gptr<R> hold = gp(b.hold);
+    const gptr<int32_t> park = gp(b.park);
+    const gptr<const int32_t> st_prev = gp((const int32_t*)b.start[po]);
     const gptr<double> Sg = gp(b.S);
     const int cmin = rng_p[0], cmax = rng_p[1];
     if (it.y == 0 && lane == 0) {            // next step's ranges start from here
@@ -262,6 +264,11 @@ __global__ __launch_bounds__(64 * kPushWaves) void pic_push_count(const SimDev<R
         rng_o[1] = -1;
         urng_o[0] = urng_p[0];
         urng_o[1] = urng_p[1];
+        // the previous step's graveyard bounds (start[po] still holds that
+        // step's start[]; this step's scatter overwrites it): a particle
+        // parked then was copied to the other buffer then and keeps its index
+        park[0] = st_prev[2];
+        park[1] = st_prev[N - 2];
     }
     if (cmin > cmax) return;
     const int lo_cell = max(cmin - 1, 0), hi_cell = min(cmax + 1, N);   // counts cover the range +-1
@@ -468,6 +475,7 @@ __global__ __launch_bounds__(256) void pic_scatter(const SimDev<R>* __restrict__
     const gptr<R> pfo = gp(b.frac[po]), pxo = gp(b.ux[po]), pyo = gp(b.uy[po]), pzo = gp(b.uz[po]);
     const gptr<const double> Sg = gp((const double*)b.S);
     const gptr<int32_t> err = gp(b.err);
+    const gptr<const int32_t> park = gp((const int32_t*)b.park);
     const gptr<const int32_t> rng_p = gp((const int32_t*)b.range[par]), urng_p = gp((const int32_t*)b.urange[par]);
     const gptr<int32_t> rng_o = gp(b.range[po]), urng_o = gp(b.urange[po]);
     const gptr<R> J = gp(const_cast<R*>(sd.J));
@@ -479,6 +487,8 @@ __global__ __launch_bounds__(256) void pic_scatter(const SimDev<R>* __restrict__
     if (bl.y >= 0) {
         const int i = bl.y + (int)threadIdx.x;
         if (i >= np) return;
+        // parked since the previous step: already in both buffers at index i
+        if (i < park[0] || i >= park[1]) return;
         const int c0 = ldo(cell, i);
         int c = c0;
         R f = ldo(pf, i), ux = ldo(px, i), uy = ldo(py, i), uz = ldo(pz, i);

@@ -4,6 +4,7 @@ Cases (config-1 slab, thermal electron beam drifting at 0.1c):
   beam_only  no source: every field is radiated by the beam current
   driven     a 1e9 V/m pulse crosses the beam before reaching the slab
   absorbed   a beam drifting into the left boundary (particles parked)
+  absorbed_right  the same at the right boundary
 FP64: fields, probes, particle displacement and u within 1e-10 relative L2;
 FP32: within 1e-5.  Also: bitwise invariance of the run under a permutation
 of the input particles, agreement between sort/deposit wave widths, and a
@@ -30,6 +31,10 @@ def _params(kind, N=4096, steps=600, n=20000):
         p["src_amp"] = 0.0
         p["steps"] = 250               # about half of the beam crosses cell 2
         lo, hi, vd = 3, 60, -0.3 * C0
+    elif kind == "absorbed_right":
+        p["src_amp"] = 0.0
+        p["steps"] = 250               # about half of the beam crosses cell N-3
+        lo, hi, vd = N - 60, N - 3, 0.3 * C0
     p["beam"] = dict(seed=17, n=n, cell_lo=lo, cell_hi=hi, v_drift=vd, v_th=1e-3 * C0, density=1e16)
     return p
 
@@ -65,10 +70,10 @@ def _device(p, prec, group=0, perm=None):
 
 @pytest.fixture(scope="module")
 def oracles():
-    return {k: (_params(k), _oracle(_params(k))) for k in ("beam_only", "driven", "absorbed")}
+    return {k: (_params(k), _oracle(_params(k))) for k in ("beam_only", "driven", "absorbed", "absorbed_right")}
 
 
-@pytest.mark.parametrize("kind", ["beam_only", "driven", "absorbed"])
+@pytest.mark.parametrize("kind", ["beam_only", "driven", "absorbed", "absorbed_right"])
 @pytest.mark.parametrize("prec,tol", [("f64", 1e-10), ("f32", 1e-5)])
 def test_pic_parity(oracles, kind, prec, tol):
     p, o = oracles[kind]
@@ -79,7 +84,7 @@ def test_pic_parity(oracles, kind, prec, tol):
     print(kind, prec, {k: f"{v:.2e}" for k, v in errs.items()}, "absorbed", int((~o["alive"]).sum()), flush=True)
     assert np.abs(o["E"]).max() > 0
     assert np.array_equal(d["alive"], o["alive"])
-    if kind == "absorbed":
+    if kind.startswith("absorbed"):
         assert 100 < (~o["alive"]).sum() < len(o["alive"]) - 100
     for k, v in errs.items():
         assert v <= tol, (kind, prec, k, v)
