@@ -247,6 +247,8 @@ int fdtd_ctx_destroy(fdtd_ctx* c) {
     if (c->d_tiles) (void)hipFree(c->d_tiles);
     for (auto& e : c->events) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
     for (hipEvent_t e : c->throttle) (void)hipEventDestroy(e);
+    if (c->stage) (void)hipHostFree(c->stage);
+    if (c->stage_ev) (void)hipEventDestroy(c->stage_ev);
     (void)hipStreamDestroy(c->up);
     (void)hipStreamDestroy(c->st);
     delete c;
@@ -333,13 +335,58 @@ int fdtd_sim_progress(fdtd_ctx* c, int id, int64_t* done) {
 }  // extern "C"
 
 // Grow-only device buffer, then an async upload on the context stream.
-static int upload_grow(fdtd_ctx* c, void** buf, size_t* cap, const void* src, size_t bytes) {
-    if (bytes > *cap) {
-        if (*buf) HIPCHK(hipFree(*buf));
-        HIPCHK(hipMalloc(buf, bytes * 2));
-        *cap = bytes * 2;
+// Device arrays rebuilt from host vectors (descriptors, tile and work lists).
+struct Upload {
+    void** dst;
+    size_t* cap;         // bytes allocated at *dst
+    const void* src;
+    size_t bytes;
+};
+
+// Copies the sources into the context's pinned staging buffer and queues
+// the uploads on the compute stream, after the launches already queued
+// (which keep reading the old contents until then).  The staging buffer is
+// reused only after the previous upload has run; a device array that must
+// grow is reallocated after the queued launches have finished with it.
+static int staged_upload(fdtd_ctx* c, const std::vector<Upload>& ups) {
+    auto pad = [](size_t b) { return (b + 255) & ~size_t(255); };
+    size_t total = 0;
+    bool grow = false;
+    for (const Upload& u : ups) {
+        total += pad(u.bytes);
+        grow = grow || u.bytes > *u.cap;
     }
-    if (bytes) HIPCHK(hipMemcpyAsync(*buf, src, bytes, hipMemcpyHostToDevice, c->st));
+    if (c->stage_pending) {
+        HIPCHK(hipEventSynchronize(c->stage_ev));
+        c->stage_pending = false;
+    }
+    if (total > c->stage_cap) {
+        if (c->stage) HIPCHK(hipHostFree(c->stage));
+        c->stage = nullptr;
+        c->stage_cap = 0;
+        HIPCHK(hipHostMalloc(&c->stage, total * 2, hipHostMallocDefault));
+        c->stage_cap = total * 2;
+    }
+    if (grow) HIPCHK(hipStreamSynchronize(c->st));
+    size_t off = 0;
+    for (const Upload& u : ups) {
+        if (u.bytes > *u.cap) {
+            if (*u.dst) HIPCHK(hipFree(*u.dst));
+            *u.dst = nullptr;
+            *u.cap = 0;
+            HIPCHK(hipMalloc(u.dst, u.bytes * 2));
+            *u.cap = u.bytes * 2;
+        }
+        if (u.bytes) {
+            char* s = (char*)c->stage + off;
+            std::memcpy(s, u.src, u.bytes);
+            HIPCHK(hipMemcpyAsync(*u.dst, s, u.bytes, hipMemcpyHostToDevice, c->st));
+        }
+        off += pad(u.bytes);
+    }
+    if (!c->stage_ev) HIPCHK(hipEventCreateWithFlags(&c->stage_ev, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(c->stage_ev, c->st));
+    c->stage_pending = true;
     return FDTD_OK;
 }
 
@@ -558,11 +605,11 @@ static int rebuild_t(fdtd_ctx* c) {
     c->has_pic = !bdev.empty();
     c->npblocks = (int)pblocks.size();
     c->npitems = (int)pitems.size();
+    std::vector<Upload> ups;
     if (c->has_pic) {
-        int rc = upload_grow(c, &c->d_beams, &c->d_beams_cap, bdev.data(), sizeof(BeamDev<R>) * bdev.size());
-        if (!rc) rc = upload_grow(c, (void**)&c->d_pblocks, &c->d_pblocks_cap, pblocks.data(), sizeof(int2) * pblocks.size());
-        if (!rc) rc = upload_grow(c, (void**)&c->d_pitems, &c->d_pitems_cap, pitems.data(), sizeof(int2) * pitems.size());
-        if (rc) return rc;
+        ups.push_back({&c->d_beams, &c->d_beams_cap, bdev.data(), sizeof(BeamDev<R>) * bdev.size()});
+        ups.push_back({(void**)&c->d_pblocks, &c->d_pblocks_cap, pblocks.data(), sizeof(int2) * pblocks.size()});
+        ups.push_back({(void**)&c->d_pitems, &c->d_pitems_cap, pitems.data(), sizeof(int2) * pitems.size()});
     }
     // Costly tiles first: a tile whose window meets a D-form slab, a CPML
     // strip, the source or a probe is dispatched before the plain ones, so
@@ -590,20 +637,10 @@ static int rebuild_t(fdtd_ctx* c) {
         c->groups.push_back({k, (int)tiles.size(), (int)by_kind[k].size()});
         tiles.insert(tiles.end(), by_kind[k].begin(), by_kind[k].end());
     }
-    const size_t need = sizeof(SimDev<R>) * devs.size();
-    if (need > c->d_sims_cap) {
-        if (c->d_sims) HIPCHK(hipFree(c->d_sims));
-        HIPCHK(hipMalloc(&c->d_sims, need * 2));
-        c->d_sims_cap = need * 2;
-    }
-    HIPCHK(hipMemcpyAsync(c->d_sims, devs.data(), need, hipMemcpyHostToDevice, c->st));
-    if (tiles.size() > c->d_tiles_cap) {
-        if (c->d_tiles) HIPCHK(hipFree(c->d_tiles));
-        HIPCHK(hipMalloc(&c->d_tiles, sizeof(int2) * tiles.size() * 2));
-        c->d_tiles_cap = tiles.size() * 2;
-    }
-    if (!tiles.empty())
-        HIPCHK(hipMemcpyAsync(c->d_tiles, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice, c->st));
+    ups.push_back({&c->d_sims, &c->d_sims_cap, devs.data(), sizeof(SimDev<R>) * devs.size()});
+    ups.push_back({(void**)&c->d_tiles, &c->d_tiles_cap, tiles.data(), sizeof(int2) * tiles.size()});
+    int rc = staged_upload(c, ups);
+    if (rc) return rc;
     c->ntiles = (int)tiles.size();
     c->dirty = false;
     return FDTD_OK;

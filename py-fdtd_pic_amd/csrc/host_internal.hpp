@@ -51,6 +51,12 @@ struct fdtd_ctx {
     int prec = FDTD_F64;
     hipStream_t st = nullptr;
     hipStream_t up = nullptr;                // uploads of new simulations and beams (never waits on st)
+    // pinned staging of the descriptor / tile / work-list uploads; reused
+    // once the event recorded after the last upload has completed
+    void* stage = nullptr;
+    size_t stage_cap = 0;
+    hipEvent_t stage_ev = nullptr;
+    bool stage_pending = false;
     int K = 2, S = 16;
     int K_slab = 2;             // cells per lane of tiles meeting a D-form slab (K or K/2)
     int S_run = 16;             // steps per launch of the current tile list (1 with a beam)
@@ -58,7 +64,7 @@ struct fdtd_ctx {
     void* d_sims = nullptr;
     size_t d_sims_cap = 0;
     int2* d_tiles = nullptr;
-    size_t d_tiles_cap = 0;
+    size_t d_tiles_cap = 0;     // bytes (all *_cap members)
     int ntiles = 0;
     std::vector<int> tile_sims;  // simulations covered by the current tile list
     std::vector<TileGroup> groups;
