@@ -118,7 +118,9 @@ __device__ __forceinline__ R kerr_series_e(KerrZ<R> x, double kz, KerrZ<R>& z) {
 // zero coefficients).  FP64 keeps the reference form P' = A P + B Pm + C E
 // (ADE_PolarisationCurrent_Ex :620-631); FP32 keeps the velocity form
 // V' = -B V - (1-A-B) P + C E, P += V' (sA = -B, sB = 1-A-B).
-template <typename R, bool KAHAN>
+// HASJ: the step carries a current (general tiles); otherwise Jv is zero and
+// the jD * Jv term is left out (jD * 0 is not folded under IEEE rules).
+template <typename R, bool KAHAN, bool HASJ>
 __device__ __forceinline__ KerrZ<R> slab_u(R kD, R jD, R dH, R Jv, R E, R& D, R& cD, R& P, R& Pm, R& cP,
                                            R sA, R sB, R sC) {
     if constexpr (sizeof(R) == 4) {
@@ -130,7 +132,7 @@ __device__ __forceinline__ KerrZ<R> slab_u(R kD, R jD, R dH, R Jv, R E, R& D, R&
         Pm = P;
         P = Pn;
     }
-    const R inc = (dH - jD * Jv) * kD;
+    const R inc = (HASJ ? dH - jD * Jv : dH) * kD;
     if (KAHAN) kahan_add(D, cD, inc); else D = D + inc;
     // FP32: u = (D - cD) - (P - cP) from the compensated values, in FP32.
     // D - P is exact when D and P are within a factor 2 (the cancelling case)
@@ -212,13 +214,13 @@ enum : int {
 
 // Slab E half for all K cells of a pure-slab lane (straight-line: the K FP64
 // chains interleave).
-template <typename R, int K, bool KAHAN, int KIND>
+template <typename R, int K, bool KAHAN, int KIND, bool HASJ>
 __device__ __forceinline__ void e_pure_slab(TILE_PARAMS, const TileCtx<R>& c, const SimDev<R>& sd,
                                             const R (&dH)[K], const R (&Jv)[K]) {
     KerrZ<R> x[K], z[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) {
-        const KerrZ<R> u = slab_u<R, KAHAN>(c.kD, c.jD, dH[j], Jv[j], E[j], D[j], cD[j], P[j], Pm[j],
+        const KerrZ<R> u = slab_u<R, KAHAN, HASJ>(c.kD, c.jD, dH[j], Jv[j], E[j], D[j], cD[j], P[j], Pm[j],
                                             cP[j], sA[j], sB[j], sC[j]);
         if constexpr (KIND == SLAB_KERR_E) {
             x[j] = u * ie[j];          // ie = 1/eps_inf in R (FP64: the same value as ied)
@@ -232,7 +234,7 @@ __device__ __forceinline__ void e_pure_slab(TILE_PARAMS, const TileCtx<R>& c, co
 
 // Mixed tile: every cell takes the interior update (+ CPML psi), slab cells
 // then take E from D - P through selects.
-template <typename R, int K, bool KAHAN, int KIND, bool STRIP>
+template <typename R, int K, bool KAHAN, int KIND, bool STRIP, bool HASJ>
 __device__ __forceinline__ void e_mixed(TILE_PARAMS, const TileCtx<R>& c, const SimDev<R>& sd,
                                         const R (&dH)[K], const R (&Jv)[K]) {
     R Ev[K], cEv[K];
@@ -252,7 +254,7 @@ __device__ __forceinline__ void e_mixed(TILE_PARAMS, const TileCtx<R>& c, const 
     for (int j = 0; j < K; ++j) {
         const bool sl = (slab_mask >> j) & 1u;
         R Dn = D[j], cDn = cD[j], Pn = P[j], Pmn = Pm[j], cPn = cP[j];
-        const KerrZ<R> u = slab_u<R, KAHAN>(c.kD, c.jD, dH[j], Jv[j], E[j], Dn, cDn, Pn, Pmn, cPn, sA[j],
+        const KerrZ<R> u = slab_u<R, KAHAN, HASJ>(c.kD, c.jD, dH[j], Jv[j], E[j], Dn, cDn, Pn, Pmn, cPn, sA[j],
                                             sB[j], sC[j]);
         if constexpr (KIND == SLAB_KERR_E) {
             x[j] = u * ie[j];
@@ -335,9 +337,9 @@ __device__ __forceinline__ void run_steps(TILE_PARAMS, const TileCtx<R>& c, cons
 #pragma unroll
         for (int j = 0; j < K; ++j) dH[j] = H[j] - ((j == 0) ? Hl : H[j - 1]);
         if constexpr (SLAB_CODE && (MODE & M_PURE_SLAB) && !STRIP) {
-            e_pure_slab<R, K, KAHAN, KIND>(TILE_ARGS, c, sd, dH, Jv);
+            e_pure_slab<R, K, KAHAN, KIND, GEN>(TILE_ARGS, c, sd, dH, Jv);
         } else if constexpr (SLAB_CODE) {
-            e_mixed<R, K, KAHAN, KIND, STRIP>(TILE_ARGS, c, sd, dH, Jv);
+            e_mixed<R, K, KAHAN, KIND, STRIP, GEN>(TILE_ARGS, c, sd, dH, Jv);
         } else {
 #pragma unroll
             for (int j = 0; j < K; ++j) {
