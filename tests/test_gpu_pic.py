@@ -123,3 +123,33 @@ def test_reduced_config3_f32():
     print("config3 reduced f32", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
     for k, v in errs.items():
         assert v <= 1e-5, (k, v)
+
+
+def test_config3_full_size_invariants():
+    """Config 3 at full size (1e6 electrons, 20,000 steps, FP64), checked by
+    properties that need no oracle run: no particle is lost (the beam stays
+    clear of the boundaries), the deposited current sums to the particles'
+    q w beta_x (CIC weights of a particle sum to one), and a second run is
+    bitwise identical."""
+    from fdtd_pic_amd import configs
+
+    def run():
+        p = configs.config3()
+        d = _device(p, "f64")
+        return p, d
+
+    p, a = run()
+    x, u, alive = a["x"], a["u"], a["alive"]
+    N = p["N"]
+    assert alive.all() and np.all((x >= 2) & (x < N - 2))
+    # J' = Z0 q w c * sum of CIC weights of beta_x; the last deposit used the final u
+    _, _, q, _, w = configs.beam_inputs(p)
+    jscale = 376.730313668 * q * w * configs.C0         # Z0 q w c, as the library forms it
+    bx = u[:, 0] / np.sqrt(1.0 + np.sum(u * u, axis=1))
+    total, ref, mag = a["J"].sum(), jscale * bx.sum(), abs(jscale) * np.abs(bx).sum()
+    print("sum J'", total, "q w c Z0 sum beta_x", ref, "scale", mag, flush=True)
+    assert abs(total - ref) <= 1e-12 * mag
+    _, b = run()
+    for k in ("E", "H", "J", "x", "u"):
+        assert np.array_equal(a[k], b[k]), k
+
