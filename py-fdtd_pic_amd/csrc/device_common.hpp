@@ -43,6 +43,7 @@ enum SlabKind { SLAB_NONE = 0, SLAB_DIEL = 1, SLAB_LINEAR = 2, SLAB_KERR_REF = 3
 // particles occupy cells [2, N-3]; cells 1 and N-2 park absorbed ones, which
 // keep their index from then on.
 constexpr int kPicGMax = 64;        // cells per push/count group (LDS tables)
+constexpr int kScatterPer = 2;      // particles per scatter thread (a block covers 256 * kScatterPer)
 template <typename R>
 struct BeamDev {
     int64_t launch_begin;  // the simulation's, copied: the parity without a load of the sim descriptor

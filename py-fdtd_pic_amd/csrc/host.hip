@@ -598,7 +598,7 @@ static int rebuild_t(fdtd_ctx* c) {
         const int bi = (int)bdev.size();
         bdev.push_back(fill_beam_dev<R>(*bh, c->d_err, c->sims[i]->launch_begin));
         for (int w = 0; w < bh->nwaves; ++w) pitems.push_back(make_int2(bi, w));
-        for (int64_t p = 0; p < bh->np; p += 256) pblocks.push_back(make_int2(bi, (int)p));
+        for (int64_t p = 0; p < bh->np; p += 256 * kScatterPer) pblocks.push_back(make_int2(bi, (int)p));
         for (int k = 0; k < bh->nnode; ++k) pblocks.push_back(make_int2(bi, -(k + 1)));
     }
     while (pitems.size() % 4) pitems.push_back(make_int2(-1, 0));   // whole 4-wave push/count blocks

@@ -485,35 +485,53 @@ __global__ __launch_bounds__(256) void pic_scatter(const SimDev<R>* __restrict__
     auto up = [&](int c) { return inr(c) ? cup[c] : 0; };
     auto down = [&](int c) { return inr(c) ? cdown[c] : 0; };
     if (bl.y >= 0) {
-        const int i = bl.y + (int)threadIdx.x;
-        if (i >= np) return;
-        // parked since the previous step: already in both buffers at index i
-        if (i < park[0] || i >= park[1]) return;
-        const int c0 = ldo(cell, i);
-        int c = c0;
-        R f = ldo(pf, i), ux = ldo(px, i), uy = ldo(py, i), uz = ldo(pz, i);
-        const int id = ldo(idp, i);
-        int ni = i;                       // parked: unchanged, same index
-        if (c0 >= 2 && c0 <= N - 3) {
-            const int tr = ldo(trank, i);
-            const int d = (tr & 3) - 1, r = tr >> 2;
-            int dn, off;
-            R a0, a1;
-            push_one<R>(pa, c, f, ux, uy, uz, dn, off, a0, a1);    // the push of pic_push_count, again
-            if (d < 0) ni = st[c0] - up(c0 - 1) + r;
-            else if (d == 0) ni = st[c0] + down(c0) + r;
-            else ni = st[c0 + 1] - up(c0) + down(c0 + 1) + r;
-            if (dn != d || ni < 0 || ni >= np) {
-                err[0] = 1;
-                return;
-            }
+        // kScatterPer particles per thread, 256 apart: every load of all of
+        // them is issued before the first store (the stores go to the other
+        // buffer, which the compiler cannot tell apart from this one)
+        const int p0 = park[0], p1 = park[1];
+        int ii[kScatterPer], c0[kScatterPer], id[kScatterPer], tr[kScatterPer];
+        R f[kScatterPer], ux[kScatterPer], uy[kScatterPer], uz[kScatterPer];
+        bool todo[kScatterPer];
+#pragma unroll
+        for (int k = 0; k < kScatterPer; ++k) {
+            const int i = bl.y + 256 * k + (int)threadIdx.x;
+            // parked since the previous step: already in both buffers at index i
+            todo[k] = i < np && i >= p0 && i < p1;
+            const unsigned ic = (unsigned)min(i, np - 1);
+            ii[k] = i;
+            c0[k] = ldo(cell, ic);
+            f[k] = ldo(pf, ic);
+            ux[k] = ldo(px, ic);
+            uy[k] = ldo(py, ic);
+            uz[k] = ldo(pz, ic);
+            id[k] = ldo(idp, ic);
+            tr[k] = ldo(trank, ic);
         }
-        sto(cello, ni, c);
-        sto(pfo, ni, f);
-        sto(pxo, ni, ux);
-        sto(pyo, ni, uy);
-        sto(pzo, ni, uz);
-        sto(ido, ni, id);
+#pragma unroll
+        for (int k = 0; k < kScatterPer; ++k) {
+            if (!todo[k]) continue;
+            const int i = ii[k], cc = c0[k];
+            int c = cc, ni = i;           // parked: unchanged, same index
+            if (cc >= 2 && cc <= N - 3) {
+                const int d = (tr[k] & 3) - 1, r = tr[k] >> 2;
+                int dn, off;
+                R a0, a1;
+                push_one<R>(pa, c, f[k], ux[k], uy[k], uz[k], dn, off, a0, a1);   // the push of pic_push_count
+                if (d < 0) ni = st[cc] - up(cc - 1) + r;
+                else if (d == 0) ni = st[cc] + down(cc) + r;
+                else ni = st[cc + 1] - up(cc) + down(cc + 1) + r;
+                if (dn != d || ni < 0 || ni >= np) {
+                    err[0] = 1;
+                    continue;
+                }
+            }
+            sto(cello, ni, c);
+            sto(pfo, ni, f[k]);
+            sto(pxo, ni, ux[k]);
+            sto(pyo, ni, uy[k]);
+            sto(pzo, ni, uz[k]);
+            sto(ido, ni, id[k]);
+        }
         return;
     }
     // node threads: J', new start[] and the next live range over the union range
