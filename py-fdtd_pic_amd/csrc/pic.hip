@@ -51,18 +51,29 @@ __device__ __forceinline__ double dpp_f64(double v) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// Inclusive scan of four FP64 values per lane, segmented at fl (the first lane
-// of the lane's segment): row_shr 1,2,4,8 inside 16-lane rows, then
-// row_bcast:15 (rows 1 and 3 take the last lane of the row below) and
-// row_bcast:31 (rows 2 and 3 take lane 31).  A step adds its source only when
-// the source lane lies in the lane's own segment, so every partial stays a
-// contiguous run of the segment.
-__device__ __forceinline__ void seg_scan4(double (&v)[4], int lane, int fl) {
+// 32-bit DPP move of a float (same lane rules as dpp_f64).
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ float dpp_f32(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWMASK, 0xf, true));
+}
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_val(double v) { return dpp_f64<CTRL, ROWMASK>(v); }
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ float dpp_val(float v) { return dpp_f32<CTRL, ROWMASK>(v); }
+
+// Inclusive scan of four values per lane (FP64, or FP32 in the FP32 mode),
+// segmented at fl (the first lane of the lane's segment): row_shr 1,2,4,8
+// inside 16-lane rows, then row_bcast:15 (rows 1 and 3 take the last lane of
+// the row below) and row_bcast:31 (rows 2 and 3 take lane 31).  A step adds
+// its source only when the source lane lies in the lane's own segment, so
+// every partial stays a contiguous run of the segment.
+template <typename T>
+__device__ __forceinline__ void seg_scan4(T (&v)[4], int lane, int fl) {
 #define FDTD_SEG_STEP(CTRL, RM, COND)                                      \
     {                                                                      \
         const bool ok = (COND);                                            \
         _Pragma("unroll") for (int q = 0; q < 4; ++q) {                    \
-            const double t = dpp_f64<CTRL, RM>(v[q]);                      \
+            const T t = dpp_val<CTRL, RM>(v[q]);                           \
             v[q] = ok ? v[q] + t : v[q];                                   \
         }                                                                  \
     }
@@ -355,7 +366,6 @@ __global__ __launch_bounds__(64 * kPushWaves) void pic_push_count(const SimDev<R
         wave_sync();
         const int plo = clampi(__builtin_amdgcn_readlane(sb, 0));
         const int phi = max(plo, clampi(__builtin_amdgcn_readlane(sb, 1)));
-        double acc[4] = {0.0, 0.0, 0.0, 0.0};
         for (int bb = plo; bb < phi; bb += 64 * kBatch) {
             int cc[kBatch];
             R ff[kBatch], xx[kBatch], yy[kBatch], zz[kBatch];
@@ -390,25 +400,15 @@ __global__ __launch_bounds__(64 * kPushWaves) void pic_push_count(const SimDev<R
                 const unsigned long long bd1 = __ballot(k.valid && d == 0);
                 const unsigned long long bd2 = __ballot(k.valid && d == 1);
                 const unsigned long long mine = d < 0 ? bd0 : (d == 0 ? bd1 : bd2);
-                double v[4];
-                if (G == 1) {
-                    // one cell per group: each lane accumulates its particles
-                    // in chunk order (weights onto nodes c-1+off, c+off), one
-                    // fixed-order reduction after the group
-                    const R s0 = off == 0 ? a0 : R(0);
-                    const R s1 = off == 0 ? a1 : (off == 1 ? a0 : R(0));
-                    const R s2 = off == 1 ? a1 : (off == 2 ? a0 : R(0));
-                    const R s3 = off == 2 ? a1 : R(0);
-                    acc[0] += (double)s0;
-                    acc[1] += (double)s1;
-                    acc[2] += (double)s2;
-                    acc[3] += (double)s3;
-                } else {
-#pragma unroll
-                    for (int w = 0; w < 4; ++w)
-                        v[w] = (w == off ? (double)a0 : 0.0) + (w == off + 1 ? (double)a1 : 0.0);
-                    seg_scan4(v, lane, k.fl);
-                }
+                // the chunk's segment sums of the weights onto nodes c-1+off,
+                // c+off, in R (FP32: at most 64 weights, then added into the
+                // FP64 cell sums in a fixed order)
+                R v[4];
+                v[0] = off == 0 ? a0 : R(0);
+                v[1] = off == 0 ? a1 : (off == 1 ? a0 : R(0));
+                v[2] = off == 1 ? a1 : (off == 2 ? a0 : R(0));
+                v[3] = off == 2 ? a1 : R(0);
+                seg_scan4<R>(v, lane, k.fl);
                 const int lk = k.valid ? c_old - g0 : -1;
                 const bool in = lk >= 0 && lk < G;
                 int r = 0;
@@ -418,22 +418,12 @@ __global__ __launch_bounds__(64 * kPushWaves) void pic_push_count(const SimDev<R
                     cnt[lk][0] += __popcll(bd0 & seg);
                     cnt[lk][1] += __popcll(bd1 & seg);
                     cnt[lk][2] += __popcll(bd2 & seg);
-                    if (G != 1) {
 #pragma unroll
-                        for (int w = 0; w < 4; ++w) Sl[lk][w] += v[w];
-                    }
+                    for (int w = 0; w < 4; ++w) Sl[lk][w] += (double)v[w];
                 }
                 if (in) sto(trank, i, (r << 2) | (d + 1));
                 wave_sync();
             }
-        }
-        if (G == 1) {
-            wave_scan4(acc);                // lane 63 holds the wave sums
-            if (lane == 63) {
-#pragma unroll
-                for (int w = 0; w < 4; ++w) Sl[0][w] = acc[w];
-            }
-            wave_sync();
         }
         for (int k = lane; k < g1 - g0; k += 64) {
             const int c = g0 + k;
