@@ -159,10 +159,8 @@ __device__ __forceinline__ void sto(gptr<T> p, unsigned i, T v) {
     *(gptr<T>)((gptr<char>)p + i * (unsigned)sizeof(T)) = v;
 }
 
-// 1/x and 1/sqrt(x): the hardware reciprocal and reciprocal square root in
-// FP32 (~1 ulp; the FP32 mode's tolerance is 1e-5), IEEE operations in FP64.
-__device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
-__device__ __forceinline__ double recip(double x) { return 1.0 / x; }
+// 1/sqrt(x): the hardware reciprocal square root in FP32 (~1 ulp; the FP32
+// mode's tolerance is 1e-5), IEEE operations in FP64.
 __device__ __forceinline__ float rsqrt_r(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ double rsqrt_r(double x) { return 1.0 / sqrt(x); }
 // 2t / (1 + t^2) of the Boris rotation: a reciprocal in FP32, a division in FP64
