@@ -193,7 +193,19 @@ def config4_bench(a):
     pushes = sum(serve.work(p)[1] for p in inst)
     # per-simulation loop: a seeded sample from each stratum (with / without a
     # beam) run alone one after the other; each stratum's time per unit of
-    # work (cell-updates + 4 x particle-pushes) scales to its total work
+    # work (cell-updates + 4 x particle-pushes) scales to its total work.  A
+    # beam alone gets the push/count group size best for one beam (~96
+    # particles per group: enough waves to fill the GPU), not the larger
+    # groups the library picks by default for a full batched context
+    def lone(p):
+        if not p.get("beam"):
+            return p
+        b = p["beam"]
+        ppc = b["n"] / (b["cell_hi"] - b["cell_lo"])
+        q = dict(p)
+        q["beam"] = dict(b, group=max(1, min(64, int(round(96.0 / ppc)))))
+        return q
+
     import numpy as np
     rng = np.random.default_rng(44)
     w = lambda p: serve.work(p)[0] + 4 * serve.work(p)[1]
@@ -205,7 +217,7 @@ def config4_bench(a):
         pick = sorted(rng.choice(idx, size=min(a.c4_sample // 2, len(idx)), replace=False).tolist())
         t1 = time.perf_counter()
         for i in pick:
-            sweep.run_alone(inst[i], "f32")
+            sweep.run_alone(lone(inst[i]), "f32")
         dt = time.perf_counter() - t1
         alone += dt
         sample += pick

@@ -115,7 +115,9 @@ class Scheduler:
         beam = None
         if job.has_beam:
             x, u, q, m, w = configs.beam_inputs(job.p)
-            beam = engine.Beam(x, u, q, m, w, job.p["Sc"] * job.p["dz"] / configs.C0, job.p["dz"])
+            # cells per push/count group: the beam's own choice, else the library's
+            beam = engine.Beam(x, u, q, m, w, job.p["Sc"] * job.p["dz"] / configs.C0, job.p["dz"],
+                               int(job.p["beam"].get("group", 0)))
         for with_slab in (False, True):
             spec = engine.physical_spec(pass_params(job.p, with_slab))
             job.sims.append(ctx.add(spec, beam=beam))
