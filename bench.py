@@ -48,6 +48,9 @@ def parse():
     ap.add_argument("--pic-particles", type=int, default=1_000_000)
     ap.add_argument("--pic-only", action="store_true", help="print only the config 3 object")
     ap.add_argument("--pic-group", type=int, default=0, help="cells per sort/deposit wave (0: automatic)")
+    ap.add_argument("--pic-K", type=int, default=1, help="cells per lane of the config 3 field tiles")
+    ap.add_argument("--pic-fusion", default="on", choices=["on", "off"],
+                    help="config 3: scatter in the field step's launch (on) or a launch of its own (off)")
     ap.add_argument("--workload", default="config2", choices=["config2", "config4", "config5"],
                     help="config4: batched sweep vs the per-simulation loop; config5: served trace replay")
     ap.add_argument("--c4-count", type=int, default=4096, help="config 4 instances")
@@ -125,7 +128,8 @@ def pic_bench(a):
     beam = engine.Beam(x, u, q, m, w, configs.DT, p["dz"], a.pic_group)
     # one cell per lane: with one step per launch the narrow tiles run the
     # field step fastest on a single simulation (K=1 4.6 us, K=2 5.4 us)
-    ctx = engine.Context("f32", K=1, S=1)
+    ctx = engine.Context("f32", K=a.pic_K, S=1)
+    ctx.set_pic_fusion(a.pic_fusion == "on")
     sim = ctx.add(engine.physical_spec(p), beam=beam)
     ctx.run_all()
     ctx.sync()
@@ -137,7 +141,8 @@ def pic_bench(a):
     ctx.run_all()
     ctx.sync()
     wall = time.perf_counter() - t0
-    kt = {k: ctx.kernel_timing(k) for k in ("stencil", "push", "sort")}
+    kt = {k: ctx.kernel_timing(k) for k in ("stencil", "push", "sort", "field+sort")}
+    kt = {k: v for k, v in kt.items() if v[1] > 0}
     ctx.close()
     n, T, N = p["beam"]["n"], p["steps"], p["N"]
     pushes, cells = n * T, N * T
@@ -149,7 +154,8 @@ def pic_bench(a):
     out = {"metric": "Gparticle-pushes/s", "value": round(pushes / wall / 1e9, 3),
            "cells_value": round(cells / wall / 1e9, 3), "ms_per_run": round(wall * 1e3, 2),
            "config": {"workload": f"config3: N={N} cells, {T} steps, config-1 slab, {n} electrons "
-                                  "(0.1c drift, 1e-3c thermal), FP32 + FP64 deposit sums", "K": 1, "S": 1},
+                                  "(0.1c drift, 1e-3c thermal), FP32 + FP64 deposit sums", "K": a.pic_K, "S": 1,
+                      "fused_field_scatter": a.pic_fusion == "on"},
            "kernels_avg_us": {k: round(v[0] * 1e3 / max(v[1], 1), 2) for k, v in kt.items()},
            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_push": bpp,

@@ -65,6 +65,7 @@ extern "C" {
 #define FDTD_KERNEL_STENCIL 0  /* field stencil (E/H/ADE/Kerr/CPML) */
 #define FDTD_KERNEL_PUSH 1     /* particle gather + Boris push + per-cell counts, ranks, CIC sums */
 #define FDTD_KERNEL_SORT 2     /* particle scatter (stable re-sort) + J', cell starts */
+#define FDTD_KERNEL_FIELD_SORT 3 /* field stencil and particle scatter in one launch (PIC fusion on) */
 
 typedef struct fdtd_ctx fdtd_ctx;
 
@@ -120,6 +121,16 @@ int fdtd_set_tiling(fdtd_ctx *ctx, int K, int S);
  * hold K_slab = K/2 cells per lane, the others K (K_slab = K: uniform).  The
  * costly slab cells then get more lanes while plain cells use fewer waves. */
 int fdtd_set_tiling_mixed(fdtd_ctx *ctx, int K, int K_slab, int S);
+/* PIC step launches: with on = 1 (the default) the particle scatter runs in
+ * the same launch as the field step when that launch is small (K = 1 or 2
+ * and at most 2,048 tiles; with several slab kinds in one context, the last
+ * kind's launch), so a step is two launches instead of three.  A larger
+ * field launch keeps its own, since the fused kernel's register count (the
+ * field tile's) would cut the scatter's occupancy.  0 always launches them
+ * one after the other.  Results are bitwise the same either way.  No
+ * counterpart in the reference (which has no particles); it selects how
+ * this library schedules its own kernels. */
+int fdtd_set_pic_fusion(fdtd_ctx *ctx, int on);
 
 /* ---- simulations ------------------------------------------------------ */
 /* Uploads coefficients, allocates zeroed state (FieldInit + CPML_FieldInit,

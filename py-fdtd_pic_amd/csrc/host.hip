@@ -32,6 +32,10 @@ hipError_t launch_push(const SimDev<R>* sims, const BeamDev<R>* beams, const int
 template <typename R>
 hipError_t launch_scatter(const SimDev<R>* sims, const BeamDev<R>* beams, const int2* scatter_blocks,
                           int nscatter, long long launch_idx, hipStream_t st, hipEvent_t e0, hipEvent_t e1);
+template <typename R, bool KAHAN>
+hipError_t launch_field_scatter(const SimDev<R>* sims, const int2* tiles, int ntiles, long long launch_idx,
+                                long long gstep0, int nsteps, int K, int S, int kind, const BeamDev<R>* beams,
+                                const int2* sblocks, int nsblocks, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1);
 }
 
 using namespace fdtd;
@@ -684,20 +688,40 @@ static EvPair* next_event(fdtd_ctx* c, int kind) {
 }
 
 // ev (profiling): ev->first is stamped at the start of the first group's
-// kernel, ev->second at the end of the last one.
-static int launch_once(fdtd_ctx* c, int n, EvPair* ev, hipStream_t st) {
+// kernel, ev->second at the end of the last one.  With fused, the last group
+// runs in one launch with the particle scatter (stamped by fev instead).
+static int launch_once(fdtd_ctx* c, int n, EvPair* ev, hipStream_t st, bool fused = false,
+                       EvPair* fev = nullptr) {
     const size_t ng = c->groups.size();
+    const size_t nplain = fused ? ng - 1 : ng;
     for (size_t gi = 0; gi < ng; ++gi) {
         const TileGroup& g = c->groups[gi];
-        hipEvent_t e0 = (ev && gi == 0) ? ev->first : nullptr;
-        hipEvent_t e1 = (ev && gi + 1 == ng) ? ev->second : nullptr;
         hipError_t e;
-        if (c->prec == FDTD_F64)
-            e = launch_stencil<double, false>((const SimDev<double>*)c->d_sims, c->d_tiles + g.offset, g.count,
-                                              c->launch_idx, c->gstep, n, c->K, c->S_run, g.kind, st, e0, e1);
-        else
-            e = launch_stencil<float, true>((const SimDev<float>*)c->d_sims, c->d_tiles + g.offset, g.count,
-                                            c->launch_idx, c->gstep, n, c->K, c->S_run, g.kind, st, e0, e1);
+        if (gi < nplain) {
+            hipEvent_t e0 = (ev && gi == 0) ? ev->first : nullptr;
+            hipEvent_t e1 = (ev && gi + 1 == nplain) ? ev->second : nullptr;
+            if (c->prec == FDTD_F64)
+                e = launch_stencil<double, false>((const SimDev<double>*)c->d_sims, c->d_tiles + g.offset, g.count,
+                                                  c->launch_idx, c->gstep, n, c->K, c->S_run, g.kind, st, e0, e1);
+            else
+                e = launch_stencil<float, true>((const SimDev<float>*)c->d_sims, c->d_tiles + g.offset, g.count,
+                                                c->launch_idx, c->gstep, n, c->K, c->S_run, g.kind, st, e0, e1);
+        } else {
+            hipEvent_t e0 = fev ? fev->first : nullptr, e1 = fev ? fev->second : nullptr;
+            if (c->prec == FDTD_F64)
+                e = launch_field_scatter<double, false>(
+                    (const SimDev<double>*)c->d_sims, c->d_tiles + g.offset, g.count, c->launch_idx, c->gstep, n,
+                    c->K, c->S_run, g.kind, (const BeamDev<double>*)c->d_beams, c->d_pblocks, c->npblocks, st, e0, e1);
+            else
+                e = launch_field_scatter<float, true>(
+                    (const SimDev<float>*)c->d_sims, c->d_tiles + g.offset, g.count, c->launch_idx, c->gstep, n,
+                    c->K, c->S_run, g.kind, (const BeamDev<float>*)c->d_beams, c->d_pblocks, c->npblocks, st, e0, e1);
+            if (e != hipSuccess)
+                return fdtd_fail(FDTD_E_HIP, std::string("field + scatter launch ") + std::to_string(c->launch_idx) +
+                                                 " (" + std::to_string(g.count) + " tiles, " +
+                                                 std::to_string(c->npblocks) + " scatter blocks): " + hipGetErrorString(e));
+            continue;
+        }
         if (e != hipSuccess) return fdtd_fail(FDTD_E_HIP, std::string("stencil launch: ") + hipGetErrorString(e));
     }
     return FDTD_OK;
@@ -722,12 +746,20 @@ int fdtd_run(fdtd_ctx* c, int64_t nsteps) {
         const bool prof = c->profile > 0 && (c->launches % c->profile) == 0;
         // event handles are copied out at once: next_event may grow (move) the vector
         hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
+        // the fused launch takes field tiles of 1 or 2 cells per lane, and
+        // only a small field launch: the fused kernel has the field tile's
+        // register count, which cuts the scatter's occupancy (FP32: 8 -> 3
+        // waves per SIMD), so a large field launch keeps its own (config 4's
+        // 2,048 beam passes: 2.11 ms per step fused against 1.74 separate;
+        // config 3's 1,024 tiles: 13.8 us fused against 4.8 + 11.9)
+        const bool fused = c->has_pic && c->fuse_pic && (c->K == 1 || c->K == 2) && !c->groups.empty() &&
+                           c->groups.back().count <= kFuseMaxTiles;
         if (c->has_pic) {
             if (prof) {
                 const EvPair* pe = next_event(c, FDTD_KERNEL_PUSH);
                 if (!pe) return FDTD_E_HIP;
                 e0 = pe->first; e1 = pe->second;
-                const EvPair* se = next_event(c, FDTD_KERNEL_SORT);
+                const EvPair* se = next_event(c, fused ? FDTD_KERNEL_FIELD_SORT : FDTD_KERNEL_SORT);
                 if (!se) return FDTD_E_HIP;
                 e2 = se->first; e3 = se->second;
             }
@@ -744,11 +776,13 @@ int fdtd_run(fdtd_ctx* c, int64_t nsteps) {
                 return fdtd_fail(FDTD_E_HIP, std::string("push/count launch ") + std::to_string(c->launch_idx) +
                                                  " (" + std::to_string(c->npitems) + " waves): " + hipGetErrorString(e));
         }
-        EvPair* ev = prof ? next_event(c, FDTD_KERNEL_STENCIL) : nullptr;
-        if (prof && !ev) return FDTD_E_HIP;
-        int rc = launch_once(c, n, ev, c->st);
+        // a stencil event only when some field launch runs by itself
+        EvPair* ev = (prof && !(fused && c->groups.size() == 1)) ? next_event(c, FDTD_KERNEL_STENCIL) : nullptr;
+        if (prof && !ev && !(fused && c->groups.size() == 1)) return FDTD_E_HIP;
+        EvPair fev{e2, e3, FDTD_KERNEL_FIELD_SORT};
+        int rc = launch_once(c, n, ev, c->st, fused, prof ? &fev : nullptr);
         if (rc) return rc;
-        if (c->has_pic) {
+        if (c->has_pic && !fused) {
             hipError_t e = c->prec == FDTD_F64
                 ? launch_scatter<double>((const SimDev<double>*)c->d_sims, (const BeamDev<double>*)c->d_beams,
                                          c->d_pblocks, c->npblocks, c->launch_idx, c->st, e2, e3)
@@ -810,7 +844,7 @@ int fdtd_busy(fdtd_ctx* c, int* busy) {
 
 int fdtd_kernel_timing(fdtd_ctx* c, int kind, double* ms, int64_t* launches) {
     if (!c) return fdtd_fail(FDTD_E_ARG, "null ctx");
-    if (kind < FDTD_KERNEL_STENCIL || kind > FDTD_KERNEL_SORT) return fdtd_fail(FDTD_E_ARG, "bad kernel kind");
+    if (kind < FDTD_KERNEL_STENCIL || kind > FDTD_KERNEL_FIELD_SORT) return fdtd_fail(FDTD_E_ARG, "bad kernel kind");
     HIPCHK(hipStreamSynchronize(c->st));
     double tot = 0;
     int64_t n = 0;
@@ -838,6 +872,13 @@ int fdtd_set_profile(fdtd_ctx* c, int every) {
     if (every < 0) return fdtd_fail(FDTD_E_ARG, "negative sampling interval");
     c->profile = every;
     c->ev_used = 0;
+    return FDTD_OK;
+}
+
+int fdtd_set_pic_fusion(fdtd_ctx* c, int on) {
+    if (!c) return fdtd_fail(FDTD_E_ARG, "null ctx");
+    if (on != 0 && on != 1) return fdtd_fail(FDTD_E_ARG, "pic fusion flag must be 0 or 1");
+    c->fuse_pic = on != 0;
     return FDTD_OK;
 }
 

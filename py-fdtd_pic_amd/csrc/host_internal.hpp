@@ -85,6 +85,7 @@ struct fdtd_ctx {
     size_t d_pitems_cap = 0;
     int npblocks = 0, npitems = 0;
     bool has_pic = false;                    // some active simulation carries a beam
+    bool fuse_pic = true;                    // scatter in the same launch as the last tile group's field step
     // bound on queued work: an event every kThrottleEvery launches; the host
     // waits on the one recorded kThrottleDepth events back before queueing more
     std::vector<hipEvent_t> throttle;
@@ -96,6 +97,9 @@ struct fdtd_ctx {
 };
 
 int fdtd_fail(int code, const std::string& msg);
+
+// Largest field launch (tiles) that takes the particle scatter into its grid.
+constexpr int kFuseMaxTiles = 2048;
 
 constexpr int kThrottleEvery = 128;
 constexpr size_t kThrottleDepth = 4;

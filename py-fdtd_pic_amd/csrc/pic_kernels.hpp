@@ -1,0 +1,561 @@
+// Particle-in-cell kernels for gfx950: relativistic Boris push with linear
+// gather, deterministic stable re-sort by cell and CIC current deposit with
+// FP64 segmented sums (no floating-point atomics).
+//
+// The reference has no particle code; the current enters the grid through
+// V.Jx in ADE_ExUpdate (BaseFDTD11.py:663-669) as Ex += (dHy - Jx)*UpExMat.
+// Units are the normalised ones of the field kernel: E, H' = Z0*Hy, and the
+// deposit writes J' = Z0*Jx (J_x*dz in the reference's units).
+//
+// Step n: push/count, then the field launch of step n (which forms
+// J^{n+1/2} from the CIC sums) and the scatter, which need not wait for it:
+//   pic_push_count  one wave per group of G cells of the live range (grid-
+//                   stride): per particle, gather E^n at x^n and
+//                   B^n = -(H^{n-1/2} + H^{n+1/2})/(2c) at x^n - 1/2, Boris,
+//                   x^{n+1} = x^n + Sc u_z/gamma; per cell, the particles
+//                   moving down / up (|d| <= 1 since Sc*beta < 1), each
+//                   particle's stable rank among (cell, d), and four FP64
+//                   sums of the CIC weights of J' at x^{n+1/2} onto nodes
+//                   c-1 .. c+2 (one-cell groups: per-lane sums and a wave
+//                   scan; larger groups: segmented DPP scans; a fixed order
+//                   either way).
+//   pic_scatter     one thread per particle: the push again (bitwise the
+//                   same: no contraction in push_one), then the new index
+//                      d=-1: start[c] - up(c-1) + r
+//                      d= 0: start[c] + down(c) + r
+//                      d=+1: start[c+1] - up(c) + down(c+1) + r
+//                   (parked particles keep theirs); node threads form
+//                   J'[m] = S+2(m-2) + S+1(m-1) + S0(m) + S-1(m+1), the new
+//                   start[] and the next live range (a single writer each:
+//                   the first and the last non-empty cell).
+// Absorbed particles stay in the array (cells 1 and N-2), so the particle
+// count is constant and no global scan is needed.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "device_common.hpp"
+
+namespace fdtd {
+
+__device__ __forceinline__ unsigned long long lanes_le(int lane) {
+    return lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+}
+__device__ __forceinline__ unsigned long long lanes_lt(int lane) { return (1ull << lane) - 1ull; }
+
+// 64-bit DPP move (two 32-bit halves); lanes the control does not feed read 0.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, ROWMASK, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWMASK, 0xf, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// 32-bit DPP move of a float (same lane rules as dpp_f64).
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ float dpp_f32(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWMASK, 0xf, true));
+}
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_val(double v) { return dpp_f64<CTRL, ROWMASK>(v); }
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ float dpp_val(float v) { return dpp_f32<CTRL, ROWMASK>(v); }
+
+// Inclusive scan of four values per lane (FP64, or FP32 in the FP32 mode),
+// segmented at fl (the first lane of the lane's segment): row_shr 1,2,4,8
+// inside 16-lane rows, then row_bcast:15 (rows 1 and 3 take the last lane of
+// the row below) and row_bcast:31 (rows 2 and 3 take lane 31).  A step adds
+// its source only when the source lane lies in the lane's own segment, so
+// every partial stays a contiguous run of the segment.
+template <typename T>
+__device__ __forceinline__ void seg_scan4(T (&v)[4], int lane, int fl) {
+#define FDTD_SEG_STEP(CTRL, RM, COND)                                      \
+    {                                                                      \
+        const bool ok = (COND);                                            \
+        _Pragma("unroll") for (int q = 0; q < 4; ++q) {                    \
+            const T t = dpp_val<CTRL, RM>(v[q]);                           \
+            v[q] = ok ? v[q] + t : v[q];                                   \
+        }                                                                  \
+    }
+    FDTD_SEG_STEP(0x111, 0xf, (lane & 15) >= 1 && lane - 1 >= fl)
+    FDTD_SEG_STEP(0x112, 0xf, (lane & 15) >= 2 && lane - 2 >= fl)
+    FDTD_SEG_STEP(0x114, 0xf, (lane & 15) >= 4 && lane - 4 >= fl)
+    FDTD_SEG_STEP(0x118, 0xf, (lane & 15) >= 8 && lane - 8 >= fl)
+    FDTD_SEG_STEP(0x142, 0xa, ((lane >> 4) & 1) && (lane & ~15) - 1 >= fl)
+    FDTD_SEG_STEP(0x143, 0xc, lane >= 32 && 31 >= fl)
+#undef FDTD_SEG_STEP
+}
+
+// Inclusive scan of four FP64 values over the whole wave (the segmented scan
+// with every lane in one segment, without its selects: out-of-row DPP
+// sources read 0 and masked rows receive 0); lane 63 holds the totals.
+__device__ __forceinline__ void wave_scan4(double (&v)[4]) {
+#define FDTD_SCAN_STEP(CTRL, RM)                                                      \
+    {                                                                                 \
+        _Pragma("unroll") for (int q = 0; q < 4; ++q) v[q] += dpp_f64<CTRL, RM>(v[q]); \
+    }
+    FDTD_SCAN_STEP(0x111, 0xf)
+    FDTD_SCAN_STEP(0x112, 0xf)
+    FDTD_SCAN_STEP(0x114, 0xf)
+    FDTD_SCAN_STEP(0x118, 0xf)
+    FDTD_SCAN_STEP(0x142, 0xa)
+    FDTD_SCAN_STEP(0x143, 0xc)
+#undef FDTD_SCAN_STEP
+}
+
+// Each push/count wave owns its own LDS tables (the workgroup's waves never
+// share them), and a wave's LDS accesses execute in program order, so a
+// wavefront-scope fence (a compiler barrier, no wait on outstanding global
+// loads) orders the cross-lane LDS traffic; a workgroup barrier would also
+// drain the prefetched loads of the next chunk.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Segment bookkeeping of one 64-particle chunk of a cell-sorted range.
+struct Chunk {
+    bool valid, last;               // last: final lane of its cell in this chunk
+    int fl;                         // first lane of this lane's cell in the chunk
+};
+
+// c: this lane's old cell (any value when !valid); i: particle index, hi: end of range.
+__device__ __forceinline__ Chunk chunk_info(int c, int i, int hi, int lane) {
+    Chunk k;
+    k.valid = i < hi;
+    const int cp = dpp_shr1_i(c);       // lane-1 (lane 0: unused)
+    const int cn = dpp_shl1_i(c);       // lane+1 (lane 63: unused)
+    const unsigned long long bnd = __ballot(k.valid && (lane == 0 || cp != c));
+    const unsigned long long m = bnd & lanes_le(lane);
+    k.fl = m ? 63 - __clzll((long long)m) : 0;
+    k.last = k.valid && (lane == 63 || i + 1 >= hi || cn != c);
+    return k;
+}
+
+
+constexpr int kBatch = 2;           // chunks of 64 particles fetched per memory round trip
+
+// Beam and field arrays are global memory, but their pointers are read from
+// descriptors, so the compiler would address them as generic (flat) memory;
+// flat accesses share the LDS wait counter with the rank table, which would
+// make every LDS access wait for the particle stores.  These casts keep them
+// global.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+template <typename T>
+__device__ __forceinline__ gptr<T> gp(T* p) { return (gptr<T>)p; }
+template <typename T>
+__device__ __forceinline__ gptr<const T> gp(const T* p) { return (gptr<const T>)p; }
+
+// Loads and stores at an unsigned 32-bit element index: the address is the
+// (scalar) base plus a zero-extended 32-bit byte offset, which the global
+// instructions take directly (no 64-bit address per lane).
+template <typename T>
+__device__ __forceinline__ T ldo(gptr<const T> p, unsigned i) {
+    return *(gptr<const T>)((gptr<const char>)p + i * (unsigned)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void sto(gptr<T> p, unsigned i, T v) {
+    *(gptr<T>)((gptr<char>)p + i * (unsigned)sizeof(T)) = v;
+}
+
+// 1/sqrt(x): the hardware reciprocal square root in FP32 (~1 ulp; the FP32
+// mode's tolerance is 1e-5), IEEE operations in FP64.
+__device__ __forceinline__ float rsqrt_r(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ double rsqrt_r(double x) { return 1.0 / sqrt(x); }
+// 2t / (1 + t^2) of the Boris rotation: a reciprocal in FP32, a division in FP64
+__device__ __forceinline__ float boris_s(float t) { return 2.0f * t * __builtin_amdgcn_rcpf(1.0f + t * t); }
+__device__ __forceinline__ double boris_s(double t) { return 2.0 * t / (1.0 + t * t); }
+
+// Boris push of one live particle; returns d and the CIC weights (node offset off = m0 + 1).
+// What the push reads of the descriptors, fetched once per wave before the
+// kernel's first store (scalar loads are only legal ahead of it).
+template <typename R>
+struct PushArgs {
+    gptr<const R> E, Hc, Hp;     // E^n, H^{n+1/2}, H^{n-1/2}
+    R a, jscale, Sc;
+    int N;
+};
+
+// Both PIC kernels run it on the same inputs (push/count for the moves, ranks
+// and current; scatter for the new state) and must agree bit for bit, so no
+// FMA contraction here: a contraction decision can depend on the caller
+// (a product with two uses in one kernel and one in the other).
+template <typename R>
+__device__ __forceinline__ void push_one(const PushArgs<R>& pa, int& c, R& f, R& ux, R& uy, R& uz, int& d,
+                                         int& off, R& a0, R& a1) {
+#pragma clang fp contract(off)
+    const int N = pa.N;
+    const gptr<const R> E = pa.E, Hc = pa.Hc, Hp = pa.Hp;
+    const R Eg = (R(1) - f) * ldo(E, c) + f * ldo(E, c + 1);
+    int j;
+    R g;
+    if (f >= R(0.5)) { j = c; g = f - R(0.5); } else { j = c - 1; g = f + R(0.5); }
+    const R h0 = R(0.5) * (ldo(Hp, j) + ldo(Hc, j));
+    const R h1 = R(0.5) * (ldo(Hp, j + 1) + ldo(Hc, j + 1));
+    const R Hg = (R(1) - g) * h0 + g * h1;
+    // Boris: E = (Eg, 0, 0), B = (0, -Hg/c, 0)
+    const R aE = pa.a * Eg;
+    const R umx = ux + aE, umy = uy, umz = uz;
+    const R igm = rsqrt_r(R(1) + umx * umx + umy * umy + umz * umz);     // 1 / gamma^-
+    const R ty = -pa.a * Hg * igm;
+    const R sy = boris_s(ty);
+    const R upx = umx - umz * ty;
+    const R upz = umz + umx * ty;
+    ux = umx - upz * sy + aE;
+    uz = umz + upx * sy;
+    uy = umy;
+    const R rg = rsqrt_r(R(1) + ux * ux + uy * uy + uz * uz);
+    const R bx = ux * rg, bz = uz * rg;
+    const R dx = pa.Sc * bz;
+    // CIC of J' at x^{n+1/2}, relative to cell c: xh in (-1/4, 5/4)
+    const R xh = f + R(0.5) * dx;
+    const int m0 = xh < R(0) ? -1 : (xh >= R(1) ? 1 : 0);
+    const R w1 = xh - (R)m0;
+    const R q = pa.jscale * bx;
+    a0 = q * (R(1) - w1);
+    a1 = q * w1;
+    off = m0 + 1;
+    // new position
+    R nf = f + dx;
+    d = nf < R(0) ? -1 : (nf >= R(1) ? 1 : 0);
+    nf -= (R)d;
+    int nc = c + d;
+    if (nc < 2) { nc = 1; nf = R(0.5); }
+    if (nc > N - 3) { nc = N - 2; nf = R(0.5); }
+    c = nc;
+    f = nf;
+}
+
+// kPushWaves independent waves per workgroup (a CU holds a limited number of
+// workgroups; single-wave workgroups would cap the resident waves), each with
+// its own LDS tables; item 4*block + wave, items with x < 0 are padding.
+constexpr int kPushWaves = 4;
+
+template <typename R>
+__global__ __launch_bounds__(64 * kPushWaves) void pic_push_count(const SimDev<R>* __restrict__ sims,
+                                                                  const BeamDev<R>* __restrict__ beams,
+                                                                  const int2* __restrict__ items,
+                                                                  long long launch_idx) {
+    __shared__ int cnt_all[kPushWaves][kPicGMax][3];    // down, stay, up so far per group cell (rank carry)
+    __shared__ double S_all[kPushWaves][kPicGMax][4];   // J' sums onto nodes c-1 .. c+2
+    // wave-uniform by construction; readfirstlane lets the compiler see it, so
+    // the item and the descriptors come in through scalar loads
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    int (&cnt)[kPicGMax][3] = cnt_all[wv];
+    double (&Sl)[kPicGMax][4] = S_all[wv];
+    const int2 it = items[blockIdx.x * kPushWaves + wv];
+    if (it.x < 0) return;
+    const BeamDev<R>& b = beams[it.x];
+    const SimDev<R>& sd = sims[b.sim];
+    const int par = (int)((launch_idx - b.launch_begin) & 1);
+    const int po = par ^ 1;
+    const int lane = threadIdx.x & 63;
+    // every descriptor field is read here, ahead of the range store below
+    const int N = b.N, np = b.np, nwaves = b.nwaves;
+    const int G = max(1, min(b.G, kPicGMax));
+    const PushArgs<R> pa{gp((const R*)sd.E[par]), gp((const R*)sd.H[par]), gp((const R*)sd.H[par ^ 1]),
+                         b.a, b.jscale, b.Sc, N};
+    const gptr<int32_t> rng_p = gp(b.range[par]), rng_o = gp(b.range[po]);
+    const gptr<int32_t> urng_p = gp(b.urange[par]), urng_o = gp(b.urange[po]);
+    const gptr<const int32_t> st = gp((const int32_t*)b.start[par]);
+    const gptr<const int32_t> cell = gp((const int32_t*)b.cell[par]);
+    const gptr<const R> pf = gp((const R*)b.frac[par]), px = gp((const R*)b.ux[par]);
+    const gptr<const R> py = gp((const R*)b.uy[par]), pz = gp((const R*)b.uz[par]);
+    const gptr<int32_t> trank = gp(b.trank), cup = gp(b.cup), cdown = gp(b.cdown);
+    const gptr<R> hold = gp(b.hold);
+    const gptr<int32_t> park = gp(b.park);
+    const gptr<const int32_t> st_prev = gp((const int32_t*)b.start[po]);
+    const gptr<double> Sg = gp(b.S);
+    const int cmin = rng_p[0], cmax = rng_p[1];
+    if (it.y == 0 && lane == 0) {            // next step's ranges start from here
+        rng_o[0] = N;
+        rng_o[1] = -1;
+        urng_o[0] = urng_p[0];
+        urng_o[1] = urng_p[1];
+        // the previous step's graveyard bounds (start[po] still holds that
+        // step's start[]; this step's scatter overwrites it): a particle
+        // parked then was copied to the other buffer then and keeps its index
+        park[0] = st_prev[2];
+        park[1] = st_prev[N - 2];
+    }
+    if (cmin > cmax) return;
+    const int lo_cell = max(cmin - 1, 0), hi_cell = min(cmax + 1, N);   // counts cover the range +-1
+    const int ngroups = (hi_cell - lo_cell + G) / G;
+    auto clampi = [&](int v) { return min(max(v, 0), np); };
+    for (int gi = it.y; gi < ngroups; gi += nwaves) {
+        const int g0 = lo_cell + gi * G;
+        const int g1 = min(g0 + G, hi_cell + 1);
+        // particles of live cells only: the graveyard cells 1 and N-2 can hold
+        // most of a beam once it has drifted out, and their particles neither
+        // move nor deposit (their cells' counts stay 0 below).  Both bounds
+        // come in with one load (lane 0 and lane 1), read back with readlane.
+        const int sb = ldo(st, (unsigned)(lane == 0 ? max(g0, 2) : min(g1, N - 2)));
+        if (G == 1) {
+            // One cell: every particle of the range is in cell g0, so the
+            // per-class carries are wave-uniform scalars, a particle's rank is
+            // its class carry plus the earlier lanes of its class (mbcnt), and
+            // no LDS table is needed.  Same ranks, counts and sums (same order)
+            // as the general path below.
+            const int plo1 = clampi(__builtin_amdgcn_readlane(sb, 0));
+            const int phi1 = max(plo1, clampi(__builtin_amdgcn_readlane(sb, 1)));
+            int n0 = 0, n1 = 0, n2 = 0;        // particles moving down / staying / up so far
+            double acc1[4] = {0.0, 0.0, 0.0, 0.0};
+            for (int bb = plo1; bb < phi1; bb += 64 * kBatch) {
+                int cc[kBatch];
+                R ff[kBatch], xx[kBatch], yy[kBatch], zz[kBatch];
+#pragma unroll
+                for (int q = 0; q < kBatch; ++q) {
+                    const unsigned ic = (unsigned)min(bb + 64 * q + lane, np - 1);
+                    cc[q] = ldo(cell, ic);
+                    ff[q] = ldo(pf, ic);
+                    xx[q] = ldo(px, ic);
+                    yy[q] = ldo(py, ic);
+                    zz[q] = ldo(pz, ic);
+                }
+#pragma unroll
+                for (int q = 0; q < kBatch; ++q) {
+                    const int base = bb + 64 * q;
+                    if (base >= phi1) break;
+                    const int i = base + lane;
+                    const bool valid = i < phi1;
+                    int c = cc[q], d = 0, off = 1;
+                    R f = ff[q], ux = xx[q], uy = yy[q], uz = zz[q];
+                    R a0 = R(0), a1 = R(0);
+                    if (valid) push_one<R>(pa, c, f, ux, uy, uz, d, off, a0, a1);
+                    const unsigned long long bd0 = __ballot(valid && d == -1);
+                    const unsigned long long bd1 = __ballot(valid && d == 0);
+                    const unsigned long long bd2 = __ballot(valid && d == 1);
+                    const unsigned long long mine = d < 0 ? bd0 : (d == 0 ? bd1 : bd2);
+                    const int carry = d < 0 ? n0 : (d == 0 ? n1 : n2);
+                    const int r = carry + (int)__builtin_amdgcn_mbcnt_hi(
+                                              (unsigned)(mine >> 32),
+                                              __builtin_amdgcn_mbcnt_lo((unsigned)mine, 0u));
+                    if (valid) sto(trank, i, (r << 2) | (d + 1));
+                    const R s0 = off == 0 ? a0 : R(0);
+                    const R s1 = off == 0 ? a1 : (off == 1 ? a0 : R(0));
+                    const R s2 = off == 1 ? a1 : (off == 2 ? a0 : R(0));
+                    const R s3 = off == 2 ? a1 : R(0);
+                    acc1[0] += (double)s0;
+                    acc1[1] += (double)s1;
+                    acc1[2] += (double)s2;
+                    acc1[3] += (double)s3;
+                    n0 += __popcll(bd0);
+                    n1 += __popcll(bd1);
+                    n2 += __popcll(bd2);
+                }
+            }
+            wave_scan4(acc1);                   // lane 63 holds the cell's sums
+            if (lane == 63) {
+#pragma unroll
+                for (int w = 0; w < 4; ++w) Sg[4 * (size_t)g0 + w] = acc1[w];
+            }
+            if (lane == 0) {
+                cup[g0] = n2;
+                cdown[g0] = n0;
+                hold[g0] = pa.Hp[g0];
+            }
+            continue;
+        }
+        for (int k = lane; k < G; k += 64) {
+            cnt[k][0] = cnt[k][1] = cnt[k][2] = 0;
+            Sl[k][0] = Sl[k][1] = Sl[k][2] = Sl[k][3] = 0.0;
+        }
+        wave_sync();
+        const int plo = clampi(__builtin_amdgcn_readlane(sb, 0));
+        const int phi = max(plo, clampi(__builtin_amdgcn_readlane(sb, 1)));
+        for (int bb = plo; bb < phi; bb += 64 * kBatch) {
+            int cc[kBatch];
+            R ff[kBatch], xx[kBatch], yy[kBatch], zz[kBatch];
+#pragma unroll
+            for (int q = 0; q < kBatch; ++q) {
+                // unconditional loads at an in-range index (np >= 1), then selects
+                const int i = bb + 64 * q + lane;
+                const bool v = i < phi;
+                const unsigned ic = (unsigned)min(i, np - 1);
+                const int c_ = ldo(cell, ic);
+                const R f_ = ldo(pf, ic), x_ = ldo(px, ic), y_ = ldo(py, ic), z_ = ldo(pz, ic);
+                cc[q] = v ? c_ : 0x7fffffff;
+                ff[q] = v ? f_ : R(0);
+                xx[q] = v ? x_ : R(0);
+                yy[q] = v ? y_ : R(0);
+                zz[q] = v ? z_ : R(0);
+            }
+#pragma unroll
+            for (int q = 0; q < kBatch; ++q) {
+                const int base = bb + 64 * q;
+                if (base >= phi) break;
+                const int i = base + lane;
+                const int c_old = cc[q];
+                const Chunk k = chunk_info(c_old, i, phi, lane);
+                int c = c_old, d = 0, off = 1;
+                R f = ff[q], ux = xx[q], uy = yy[q], uz = zz[q];
+                R a0 = R(0), a1 = R(0);
+                const bool live = k.valid && c_old >= 2 && c_old <= N - 3;
+                if (live) push_one<R>(pa, c, f, ux, uy, uz, d, off, a0, a1);
+                const unsigned long long seg = lanes_le(lane) & ~lanes_lt(k.fl);
+                const unsigned long long bd0 = __ballot(k.valid && d == -1);
+                const unsigned long long bd1 = __ballot(k.valid && d == 0);
+                const unsigned long long bd2 = __ballot(k.valid && d == 1);
+                const unsigned long long mine = d < 0 ? bd0 : (d == 0 ? bd1 : bd2);
+                // the chunk's segment sums of the weights onto nodes c-1+off,
+                // c+off, in R (FP32: at most 64 weights, then added into the
+                // FP64 cell sums in a fixed order)
+                R v[4];
+                v[0] = off == 0 ? a0 : R(0);
+                v[1] = off == 0 ? a1 : (off == 1 ? a0 : R(0));
+                v[2] = off == 1 ? a1 : (off == 2 ? a0 : R(0));
+                v[3] = off == 2 ? a1 : R(0);
+                seg_scan4<R>(v, lane, k.fl);
+                const int lk = k.valid ? c_old - g0 : -1;
+                const bool in = lk >= 0 && lk < G;
+                int r = 0;
+                if (in) r = cnt[lk][d + 1] + __popcll(mine & lanes_lt(lane) & ~lanes_lt(k.fl));
+                wave_sync();
+                if (in && k.last) {
+                    cnt[lk][0] += __popcll(bd0 & seg);
+                    cnt[lk][1] += __popcll(bd1 & seg);
+                    cnt[lk][2] += __popcll(bd2 & seg);
+#pragma unroll
+                    for (int w = 0; w < 4; ++w) Sl[lk][w] += (double)v[w];
+                }
+                if (in) sto(trank, i, (r << 2) | (d + 1));
+                wave_sync();
+            }
+        }
+        for (int k = lane; k < g1 - g0; k += 64) {
+            const int c = g0 + k;
+            cup[c] = cnt[k][2];
+            cdown[c] = cnt[k][0];
+#pragma unroll
+            for (int w = 0; w < 4; ++w) Sg[4 * (size_t)c + w] = Sl[k][w];
+            // H^{n-1/2} of the group's cells for the scatter's push: the groups
+            // tile [cmin-1, cmax+1], the H nodes the live particles' gathers touch
+            hold[c] = pa.Hp[c];
+        }
+        wave_sync();
+    }
+}
+
+// One 256-thread scatter block: entry bl = (beam, first particle) for a
+// particle block of 256 * PER particles, (beam, -(k + 1)) for node block k;
+// t is the thread's index in the block.  Shared by pic_scatter and the fused
+// field + scatter launch.
+template <typename R, int PER>
+__device__ __forceinline__ void scatter_block(const SimDev<R>* __restrict__ sims,
+                                              const BeamDev<R>* __restrict__ beams, const int2 bl,
+                                              long long launch_idx, const int t) {
+    const BeamDev<R>& b = beams[bl.x];
+    const SimDev<R>& sd = sims[b.sim];
+    const int par = (int)((launch_idx - b.launch_begin) & 1);
+    const int po = par ^ 1;
+    // every descriptor field is read here, ahead of the kernel's first store
+    // (scalar loads), as in pic_push_count
+    const int N = b.N, np = b.np, nnode = b.nnode;
+    // H^{n-1/2} from push/count's copy: the field step of this launch may run
+    // alongside and overwrite H[par ^ 1]
+    const PushArgs<R> pa{gp((const R*)sd.E[par]), gp((const R*)sd.H[par]), gp((const R*)b.hold),
+                         b.a, b.jscale, b.Sc, N};
+    const gptr<const int32_t> st = gp((const int32_t*)b.start[par]);
+    const gptr<const int32_t> cup = gp((const int32_t*)b.cup), cdown = gp((const int32_t*)b.cdown);
+    const gptr<const int32_t> trank = gp((const int32_t*)b.trank);
+    const gptr<const int32_t> cell = gp((const int32_t*)b.cell[par]), idp = gp((const int32_t*)b.id[par]);
+    const gptr<const R> pf = gp((const R*)b.frac[par]), px = gp((const R*)b.ux[par]);
+    const gptr<const R> py = gp((const R*)b.uy[par]), pz = gp((const R*)b.uz[par]);
+    const gptr<int32_t> cello = gp(b.cell[po]), ido = gp(b.id[po]), st_o = gp(b.start[po]);
+    const gptr<R> pfo = gp(b.frac[po]), pxo = gp(b.ux[po]), pyo = gp(b.uy[po]), pzo = gp(b.uz[po]);
+    const gptr<const double> Sg = gp((const double*)b.S);
+    const gptr<int32_t> err = gp(b.err);
+    const gptr<const int32_t> park = gp((const int32_t*)b.park);
+    const gptr<const int32_t> rng_p = gp((const int32_t*)b.range[par]), urng_p = gp((const int32_t*)b.urange[par]);
+    const gptr<int32_t> rng_o = gp(b.range[po]), urng_o = gp(b.urange[po]);
+    const gptr<R> J = gp(const_cast<R*>(sd.J));
+    const int cmin = rng_p[0], cmax = rng_p[1];
+    // counts and sums exist on [cmin-1, cmax+1] this step; zero elsewhere
+    auto inr = [&](int c) { return c >= cmin - 1 && c <= cmax + 1 && cmin <= cmax; };
+    auto up = [&](int c) { return inr(c) ? cup[c] : 0; };
+    auto down = [&](int c) { return inr(c) ? cdown[c] : 0; };
+    if (bl.y >= 0) {
+        // PER particles per thread, 256 apart: every load of all of
+        // them is issued before the first store (the stores go to the other
+        // buffer, which the compiler cannot tell apart from this one)
+        const int p0 = park[0], p1 = park[1];
+        int ii[PER], c0[PER], id[PER], tr[PER];
+        R f[PER], ux[PER], uy[PER], uz[PER];
+        bool todo[PER];
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int i = bl.y + 256 * k + t;
+            // parked since the previous step: already in both buffers at index i
+            todo[k] = i < np && i >= p0 && i < p1;
+            const unsigned ic = (unsigned)min(i, np - 1);
+            ii[k] = i;
+            c0[k] = ldo(cell, ic);
+            f[k] = ldo(pf, ic);
+            ux[k] = ldo(px, ic);
+            uy[k] = ldo(py, ic);
+            uz[k] = ldo(pz, ic);
+            id[k] = ldo(idp, ic);
+            tr[k] = ldo(trank, ic);
+        }
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            if (!todo[k]) continue;
+            const int i = ii[k], cc = c0[k];
+            int c = cc, ni = i;           // parked: unchanged, same index
+            if (cc >= 2 && cc <= N - 3) {
+                const int d = (tr[k] & 3) - 1, r = tr[k] >> 2;
+                int dn, off;
+                R a0, a1;
+                push_one<R>(pa, c, f[k], ux[k], uy[k], uz[k], dn, off, a0, a1);   // the push of pic_push_count
+                if (d < 0) ni = st[cc] - up(cc - 1) + r;
+                else if (d == 0) ni = st[cc] + down(cc) + r;
+                else ni = st[cc + 1] - up(cc) + down(cc + 1) + r;
+                if (dn != d || ni < 0 || ni >= np) {
+                    err[0] = 1;
+                    continue;
+                }
+            }
+            sto(cello, ni, c);
+            sto(pfo, ni, f[k]);
+            sto(pxo, ni, ux[k]);
+            sto(pyo, ni, uy[k]);
+            sto(pzo, ni, uz[k]);
+            sto(ido, ni, id[k]);
+        }
+        return;
+    }
+    // node threads: J', new start[] and the next live range over the union range
+    const int k = -bl.y - 1;
+    const int u0 = urng_p[0], u1 = urng_p[1];
+    const int e0 = max(u0 - 3, 0), e1 = min(u1 + 4, N);
+    const int nlo = st[2] + down(2);                 // new start of cell 2 (cell 1 sends nothing up)
+    const int nhi = st[N - 2] - up(N - 3);           // new start of cell N-2
+    for (int m = e0 + k * 256 + t; m <= e1; m += nnode * 256) {
+        J[m] = (R)j_from_sums(Sg, cmin, cmax, m);     // the value the field step forms
+        const int ns = st[m] - up(m - 1) + down(m);
+        st_o[m] = ns;
+        if (m >= 2 && m <= N - 3) {
+            const int ns1 = st[m + 1] - up(m) + down(m + 1);
+            if (ns1 > ns) {
+                if (ns == nlo) {
+                    rng_o[0] = m;
+                    urng_o[0] = min(u0, m);
+                }
+                if (ns1 == nhi) {
+                    rng_o[1] = m;
+                    urng_o[1] = max(u1, m);
+                }
+            }
+        }
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void pic_scatter(const SimDev<R>* __restrict__ sims,
+                                                   const BeamDev<R>* __restrict__ beams,
+                                                   const int2* __restrict__ blocks, long long launch_idx) {
+    scatter_block<R, kScatterPer>(sims, beams, blocks[blockIdx.x], launch_idx, (int)threadIdx.x);
+}
+
+}  // namespace fdtd

@@ -1,6 +1,6 @@
-// Field-step launcher (called from host.hip): dispatch on the slab kind to
+// Field-step launchers (called from host.hip): dispatch on the slab kind to
 // the instances compiled in stencil_<precision>_<kind>.hip.
-#include "stencil_kernels.hpp"
+#include "field_scatter.hpp"
 
 namespace fdtd {
 
@@ -11,6 +11,14 @@ extern template hipError_t launch_kind<double, false, SLAB_DIEL>(const SimDev<do
 extern template hipError_t launch_kind<double, false, SLAB_LINEAR>(const SimDev<double>*, const int2*, int, long long, long long, int, int, int, hipStream_t, hipEvent_t, hipEvent_t);
 extern template hipError_t launch_kind<double, false, SLAB_KERR_E>(const SimDev<double>*, const int2*, int, long long, long long, int, int, int, hipStream_t, hipEvent_t, hipEvent_t);
 extern template hipError_t launch_kind<double, false, SLAB_KERR_REF>(const SimDev<double>*, const int2*, int, long long, long long, int, int, int, hipStream_t, hipEvent_t, hipEvent_t);
+
+extern template hipError_t launch_fused_kind<float, true, SLAB_DIEL>(const SimDev<float>*, const int2*, int, long long, long long, int, int, int, const BeamDev<float>*, const int2*, int, hipStream_t, hipEvent_t, hipEvent_t);
+extern template hipError_t launch_fused_kind<float, true, SLAB_LINEAR>(const SimDev<float>*, const int2*, int, long long, long long, int, int, int, const BeamDev<float>*, const int2*, int, hipStream_t, hipEvent_t, hipEvent_t);
+extern template hipError_t launch_fused_kind<float, true, SLAB_KERR_E>(const SimDev<float>*, const int2*, int, long long, long long, int, int, int, const BeamDev<float>*, const int2*, int, hipStream_t, hipEvent_t, hipEvent_t);
+extern template hipError_t launch_fused_kind<double, false, SLAB_DIEL>(const SimDev<double>*, const int2*, int, long long, long long, int, int, int, const BeamDev<double>*, const int2*, int, hipStream_t, hipEvent_t, hipEvent_t);
+extern template hipError_t launch_fused_kind<double, false, SLAB_LINEAR>(const SimDev<double>*, const int2*, int, long long, long long, int, int, int, const BeamDev<double>*, const int2*, int, hipStream_t, hipEvent_t, hipEvent_t);
+extern template hipError_t launch_fused_kind<double, false, SLAB_KERR_E>(const SimDev<double>*, const int2*, int, long long, long long, int, int, int, const BeamDev<double>*, const int2*, int, hipStream_t, hipEvent_t, hipEvent_t);
+extern template hipError_t launch_fused_kind<double, false, SLAB_KERR_REF>(const SimDev<double>*, const int2*, int, long long, long long, int, int, int, const BeamDev<double>*, const int2*, int, hipStream_t, hipEvent_t, hipEvent_t);
 
 // kind: 0 (no D-form slab: NONE or DIEL share the DIEL kernel), SLAB_LINEAR,
 // SLAB_KERR_REF (FP64 only), SLAB_KERR_E.
@@ -37,5 +45,33 @@ template hipError_t launch_stencil<double, false>(const SimDev<double>*, const i
 template hipError_t launch_stencil<float, true>(const SimDev<float>*, const int2*, int, long long,
                                                 long long, int, int, int, int, hipStream_t, hipEvent_t,
                                                 hipEvent_t);
+
+// The field step of one tile group and the particle scatter in one launch
+// (field_scatter.hpp); same kinds as launch_stencil.
+template <typename R, bool KAHAN>
+hipError_t launch_field_scatter(const SimDev<R>* sims, const int2* tiles, int ntiles, long long launch_idx,
+                                long long gstep0, int nsteps, int K, int S, int kind, const BeamDev<R>* beams,
+                                const int2* sblocks, int nsblocks, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
+#define FDTD_FUSED(KIND) \
+    launch_fused_kind<R, KAHAN, KIND>(sims, tiles, ntiles, launch_idx, gstep0, nsteps, K, S, beams, sblocks, nsblocks, st, ev0, ev1)
+    switch (kind) {
+        case SLAB_NONE:
+        case SLAB_DIEL: return FDTD_FUSED(SLAB_DIEL);
+        case SLAB_LINEAR: return FDTD_FUSED(SLAB_LINEAR);
+        case SLAB_KERR_E: return FDTD_FUSED(SLAB_KERR_E);
+        case SLAB_KERR_REF:
+            if constexpr (sizeof(R) == 8) return FDTD_FUSED(SLAB_KERR_REF);
+            return hipErrorInvalidValue;
+        default: return hipErrorInvalidValue;
+    }
+#undef FDTD_FUSED
+}
+
+template hipError_t launch_field_scatter<double, false>(const SimDev<double>*, const int2*, int, long long, long long,
+                                                        int, int, int, int, const BeamDev<double>*, const int2*, int,
+                                                        hipStream_t, hipEvent_t, hipEvent_t);
+template hipError_t launch_field_scatter<float, true>(const SimDev<float>*, const int2*, int, long long, long long,
+                                                      int, int, int, int, const BeamDev<float>*, const int2*, int,
+                                                      hipStream_t, hipEvent_t, hipEvent_t);
 
 }  // namespace fdtd

@@ -46,10 +46,14 @@ __device__ __forceinline__ double lane_read(double v, int lane) {
     const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
+// Lane of the calling thread in its wave (a tile runs as one wave of a
+// 64-thread block, or as one of the four waves of a fused 256-thread block).
+__device__ __forceinline__ int wave_lane() { return (int)(threadIdx.x & 63u); }
+
 // dst with lane `lane` replaced by the uniform value v (a lane-id select)
 template <typename R>
 __device__ __forceinline__ R lane_write(R v, int lane, R dst) {
-    return ((int)threadIdx.x == lane) ? v : dst;
+    return (wave_lane() == lane) ? v : dst;
 }
 constexpr int kChunks = 4;    // S <= 255: a launch spans at most 4 x 64 steps
 
@@ -430,7 +434,7 @@ template <typename R, int K, bool KAHAN, int KIND>
 __device__ __forceinline__ void tile_body(const SimDev<R>& sd, int out0_, long long launch_idx, long long gstep0,
                                           int nsteps, int S) {
     constexpr bool DFORM = KIND >= SLAB_LINEAR;
-    const int lane = threadIdx.x;
+    const int lane = wave_lane();
     const int U = 64 * K - 2 * S;
     const int out0 = out0_;
     const int N = sd.N;
@@ -609,11 +613,8 @@ __device__ __forceinline__ void tile_body(const SimDev<R>& sd, int out0_, long l
 // so tiles with costly slab cells can be made narrower than plain ones
 // within one launch.
 template <typename R, int KW, bool KAHAN, int KIND>
-__global__ __launch_bounds__(64) void stencil_tiles(const SimDev<R>* __restrict__ sims,
-                                                    const int2* __restrict__ tiles, int ntiles,
-                                                    long long launch_idx, long long gstep0, int nsteps, int S) {
-    const int tid = blockIdx.x;
-    if (tid >= ntiles) return;
+__device__ __forceinline__ void field_tile(const SimDev<R>* __restrict__ sims, const int2* __restrict__ tiles,
+                                           int tid, long long launch_idx, long long gstep0, int nsteps, int S) {
     const int2 tl = tiles[tid];
     const SimDev<R>& sd = sims[tl.x & 0xffffff];
     if constexpr (KW > 1) {
@@ -623,6 +624,15 @@ __global__ __launch_bounds__(64) void stencil_tiles(const SimDev<R>* __restrict_
         }
     }
     tile_body<R, KW, KAHAN, KIND>(sd, tl.y, launch_idx, gstep0, nsteps, S);
+}
+
+template <typename R, int KW, bool KAHAN, int KIND>
+__global__ __launch_bounds__(64) void stencil_tiles(const SimDev<R>* __restrict__ sims,
+                                                    const int2* __restrict__ tiles, int ntiles,
+                                                    long long launch_idx, long long gstep0, int nsteps, int S) {
+    const int tid = blockIdx.x;
+    if (tid >= ntiles) return;
+    field_tile<R, KW, KAHAN, KIND>(sims, tiles, tid, launch_idx, gstep0, nsteps, S);
 }
 
 // Host-side launcher (called from host.hip).

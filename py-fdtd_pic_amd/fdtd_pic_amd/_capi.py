@@ -67,6 +67,7 @@ SIGNATURES = {
     "fdtd_ctx_destroy": (C.c_int, [_vp]),
     "fdtd_set_tiling": (C.c_int, [_vp, C.c_int, C.c_int]),
     "fdtd_set_tiling_mixed": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    "fdtd_set_pic_fusion": (C.c_int, [_vp, C.c_int]),
     "fdtd_add_sim": (C.c_int, [_vp, C.POINTER(SimDesc), C.POINTER(C.c_int)]),
     "fdtd_remove_sim": (C.c_int, [_vp, C.c_int]),
     "fdtd_sim_progress": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int64)]),

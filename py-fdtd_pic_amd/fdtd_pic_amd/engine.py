@@ -171,6 +171,11 @@ class Context:
         meet a Lorentz/Kerr slab narrower than plain ones."""
         check(self.lib.fdtd_set_tiling_mixed(self.h, int(K), int(K if K_slab is None else K_slab), int(S)))
 
+    def set_pic_fusion(self, on=True):
+        """Run the particle scatter in the field step's launch when that
+        launch is small (default on; bitwise the same results either way)."""
+        check(self.lib.fdtd_set_pic_fusion(self.h, 1 if on else 0))
+
     def add(self, spec, beam=None):
         """Add a simulation; beam (optional) is a Beam whose current couples to it."""
         sid = C.c_int()
@@ -212,8 +217,9 @@ class Context:
         return ms.value, n.value
 
     def kernel_timing(self, kind):
-        """(ms, launches) of one kernel kind: "stencil", "push" or "sort"."""
-        k = {"stencil": 0, "push": 1, "sort": 2}[kind]
+        """(ms, launches) of one kernel kind: "stencil", "push", "sort", or
+        "field+sort" (the fused field and scatter launch)."""
+        k = {"stencil": 0, "push": 1, "sort": 2, "field+sort": 3}[kind]
         ms, n = C.c_double(), C.c_int64()
         check(self.lib.fdtd_kernel_timing(self.h, k, C.byref(ms), C.byref(n)))
         return ms.value, n.value

@@ -7,7 +7,8 @@ Cases (config-1 slab, thermal electron beam drifting at 0.1c):
   absorbed_right  the same at the right boundary
 FP64: fields, probes, particle displacement and u within 1e-10 relative L2;
 FP32: within 1e-5.  Also: bitwise invariance of the run under a permutation
-of the input particles, agreement between sort/deposit wave widths, and a
+of the input particles, agreement between sort/deposit wave widths, the
+fused field + scatter launch bitwise equal to separate launches, and a
 reduced config 3 (N = 65,536, 1e5 particles, 300 steps) in FP32.
 """
 import numpy as np
@@ -110,6 +111,43 @@ def test_wave_width_independence():
         d = _device(p, "f64", group=g)
         assert rel_l2(d["E"], ref["E"]) < 1e-13, g
         assert rel_l2(d["x"] - x0, ref["x"] - x0) < 1e-13, g
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("K", [1, 2])
+def test_fused_scatter_bitwise(prec, K):
+    """The scatter run in the field step's launch gives bitwise the same run as
+    separate launches.  Two beams in one context, one with a Lorentz slab and
+    one in vacuum (two tile groups: one field launch alone, one fused), the
+    second absorbed at the right boundary and finishing 50 steps later (the
+    work lists are rebuilt mid-run)."""
+    from fdtd_pic_amd import configs, engine
+    pa = _params("driven", steps=200)
+    pb = _params("absorbed_right")
+    pb["slab_stop"] = pb["slab_start"]
+
+    def run(fuse):
+        print(f"fused scatter {prec} K={K} fuse={fuse}", flush=True)
+        out = []
+        with engine.Context(prec, K=K) as ctx:
+            ctx.set_pic_fusion(fuse)
+            sims = []
+            for p in (pa, pb):
+                x, u, q, mm, w = configs.beam_inputs(p)
+                dt = p["Sc"] * p["dz"] / configs.C0
+                sims.append(ctx.add(engine.physical_spec(p), beam=engine.Beam(x, u, q, mm, w, dt, p["dz"])))
+            ctx.run_all()
+            for s in sims:
+                xo, uo, alive = s.particles()
+                out.append(dict(E=s.read("E"), H=s.read("H"), J=s.read("J"), probes=s.probes(), x=xo, u=uo,
+                                alive=alive))
+        return out
+
+    a, b = run(True), run(False)
+    assert (~b[1]["alive"]).sum() > 100
+    for i in range(2):
+        for k in a[i]:
+            assert np.array_equal(a[i][k], b[i][k]), (i, k)
 
 
 def test_reduced_config3_f32():
