@@ -15,7 +15,7 @@
 
 namespace fdtd {
 
-constexpr int kFusedTileWaves = 4;    // field tiles per 256-thread workgroup
+constexpr int kFusedTileWaves = 4;    // a 256-thread workgroup: four field tiles or one scatter block
 
 template <typename R, int KW, bool KAHAN, int KIND>
 __global__ __launch_bounds__(256) void field_scatter(const SimDev<R>* __restrict__ sims,

@@ -47,7 +47,7 @@ __device__ __forceinline__ double lane_read(double v, int lane) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 // Lane of the calling thread in its wave (a tile runs as one wave of a
-// 64-thread block, or as one of the four waves of a fused 256-thread block).
+// multi-wave workgroup).
 __device__ __forceinline__ int wave_lane() { return (int)(threadIdx.x & 63u); }
 
 // dst with lane `lane` replaced by the uniform value v (a lane-id select)
@@ -604,14 +604,19 @@ __device__ __forceinline__ void tile_body(const SimDev<R>& sd, int out0_, long l
     }
 }
 
-// Tile t runs as block t: consecutive tiles go round-robin to the 8 XCDs
-// (block b runs on XCD b % 8), so the costly slab tiles, which are contiguous
-// along the grid, spread over every XCD.  (A contiguous run of tiles per XCD
-// keeps neighbours in one L2 but put all of config 2's slab on three XCDs:
-// 25.4 us launches against 18.0 us here.)  A tile entry is (sim | narrow << 24,
-// first interior cell): narrow tiles hold KW/2 cells per lane, the others KW,
-// so tiles with costly slab cells can be made narrower than plain ones
-// within one launch.
+// Tile t runs as wave t % kTileWaves of workgroup t / kTileWaves, and
+// consecutive workgroups go round-robin to the 8 XCDs (workgroup b runs on
+// XCD b % 8), so the costly slab tiles, which are contiguous along the grid,
+// spread over every XCD.  (A contiguous run of tiles per XCD keeps
+// neighbours in one L2 but put all of config 2's slab on three XCDs: 25.4 us
+// launches against 18.0 us here.)  Four tiles per workgroup rather than one
+// cut the workgroups to dispatch by four: config 2 235.5 -> 244.0 Gcell/s
+// (FP32), 239.5 -> 243.4 (FP64); two or eight per workgroup measured ~198
+// (FP32) and ~181 (FP64).  A tile entry is (sim | narrow << 24, first
+// interior cell): narrow tiles hold KW/2 cells per lane, the others KW, so
+// tiles with costly slab cells can be made narrower than plain ones within
+// one launch.
+constexpr int kTileWaves = 4;
 template <typename R, int KW, bool KAHAN, int KIND>
 __device__ __forceinline__ void field_tile(const SimDev<R>* __restrict__ sims, const int2* __restrict__ tiles,
                                            int tid, long long launch_idx, long long gstep0, int nsteps, int S) {
@@ -627,10 +632,12 @@ __device__ __forceinline__ void field_tile(const SimDev<R>* __restrict__ sims, c
 }
 
 template <typename R, int KW, bool KAHAN, int KIND>
-__global__ __launch_bounds__(64) void stencil_tiles(const SimDev<R>* __restrict__ sims,
-                                                    const int2* __restrict__ tiles, int ntiles,
-                                                    long long launch_idx, long long gstep0, int nsteps, int S) {
-    const int tid = blockIdx.x;
+__global__ __launch_bounds__(64 * kTileWaves) void stencil_tiles(const SimDev<R>* __restrict__ sims,
+                                                                 const int2* __restrict__ tiles, int ntiles,
+                                                                 long long launch_idx, long long gstep0,
+                                                                 int nsteps, int S) {
+    // wave-uniform tile index, so the descriptors come in through scalar loads
+    const int tid = kTileWaves * (int)blockIdx.x + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     if (tid >= ntiles) return;
     field_tile<R, KW, KAHAN, KIND>(sims, tiles, tid, launch_idx, gstep0, nsteps, S);
 }
@@ -643,13 +650,13 @@ template <typename R, int K, bool KAHAN, int KIND>
 static hipError_t launch_k(const SimDev<R>* sims, const int2* tiles, int ntiles, long long launch_idx,
                            long long gstep0, int nsteps, int S, hipStream_t st, hipEvent_t ev0,
                            hipEvent_t ev1) {
-    const int grid = ntiles;
+    const int grid = (ntiles + kTileWaves - 1) / kTileWaves;
     if (ev0 || ev1)
-        hipExtLaunchKernelGGL((stencil_tiles<R, K, KAHAN, KIND>), dim3(grid), dim3(64), 0, st, ev0, ev1, 0,
-                              sims, tiles, ntiles, launch_idx, gstep0, nsteps, S);
+        hipExtLaunchKernelGGL((stencil_tiles<R, K, KAHAN, KIND>), dim3(grid), dim3(64 * kTileWaves), 0, st, ev0,
+                              ev1, 0, sims, tiles, ntiles, launch_idx, gstep0, nsteps, S);
     else
-        hipLaunchKernelGGL((stencil_tiles<R, K, KAHAN, KIND>), dim3(grid), dim3(64), 0, st, sims, tiles,
-                           ntiles, launch_idx, gstep0, nsteps, S);
+        hipLaunchKernelGGL((stencil_tiles<R, K, KAHAN, KIND>), dim3(grid), dim3(64 * kTileWaves), 0, st, sims,
+                           tiles, ntiles, launch_idx, gstep0, nsteps, S);
     return hipGetLastError();
 }
 
