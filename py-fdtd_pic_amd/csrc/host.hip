@@ -530,7 +530,9 @@ static int attach_t(fdtd_ctx* c, int id, SimHost& s, const fdtd_beam_desc_t* bd)
     }
     b->G = std::min(std::max(G, 1), kPicGMax);
     const int span = b->range0[1] - b->range0[0] + 3;
-    b->nwaves = (int)std::min<int64_t>(32768, std::max<int64_t>(1, (span + b->G - 1) / b->G));
+    // one-cell groups run two cells per wave
+    const int64_t cells_per_wave = b->G == 1 ? 2 : b->G;
+    b->nwaves = (int)std::min<int64_t>(32768, std::max<int64_t>(1, (span + cells_per_wave - 1) / cells_per_wave));
     b->nnode = std::min(64, std::max(1, (span + 8 + 255) / 256 + 1));
     // device block
     Layout L;

@@ -181,19 +181,12 @@ struct PushArgs {
 // and current; scatter for the new state) and must agree bit for bit, so no
 // FMA contraction here: a contraction decision can depend on the caller
 // (a product with two uses in one kernel and one in the other).
+// The push after the gather: Eg, Hg are the fields at the particle.
 template <typename R>
-__device__ __forceinline__ void push_one(const PushArgs<R>& pa, int& c, R& f, R& ux, R& uy, R& uz, int& d,
-                                         int& off, R& a0, R& a1) {
+__device__ __forceinline__ void push_at(const PushArgs<R>& pa, int& c, R& f, R& ux, R& uy, R& uz, R Eg, R Hg,
+                                        int& d, int& off, R& a0, R& a1) {
 #pragma clang fp contract(off)
     const int N = pa.N;
-    const gptr<const R> E = pa.E, Hc = pa.Hc, Hp = pa.Hp;
-    const R Eg = (R(1) - f) * ldo(E, c) + f * ldo(E, c + 1);
-    int j;
-    R g;
-    if (f >= R(0.5)) { j = c; g = f - R(0.5); } else { j = c - 1; g = f + R(0.5); }
-    const R h0 = R(0.5) * (ldo(Hp, j) + ldo(Hc, j));
-    const R h1 = R(0.5) * (ldo(Hp, j + 1) + ldo(Hc, j + 1));
-    const R Hg = (R(1) - g) * h0 + g * h1;
     // Boris: E = (Eg, 0, 0), B = (0, -Hg/c, 0)
     const R aE = pa.a * Eg;
     const R umx = ux + aE, umy = uy, umz = uz;
@@ -225,6 +218,90 @@ __device__ __forceinline__ void push_one(const PushArgs<R>& pa, int& c, R& f, R&
     if (nc > N - 3) { nc = N - 2; nf = R(0.5); }
     c = nc;
     f = nf;
+}
+
+// Linear gather of E at x and of the time-averaged H at x - 1/2.
+template <typename R>
+__device__ __forceinline__ void push_one(const PushArgs<R>& pa, int& c, R& f, R& ux, R& uy, R& uz, int& d,
+                                         int& off, R& a0, R& a1) {
+#pragma clang fp contract(off)
+    const gptr<const R> E = pa.E, Hc = pa.Hc, Hp = pa.Hp;
+    const R Eg = (R(1) - f) * ldo(E, c) + f * ldo(E, c + 1);
+    int j;
+    R g;
+    if (f >= R(0.5)) { j = c; g = f - R(0.5); } else { j = c - 1; g = f + R(0.5); }
+    const R h0 = R(0.5) * (ldo(Hp, j) + ldo(Hc, j));
+    const R h1 = R(0.5) * (ldo(Hp, j + 1) + ldo(Hc, j + 1));
+    const R Hg = (R(1) - g) * h0 + g * h1;
+    push_at<R>(pa, c, f, ux, uy, uz, Eg, Hg, d, off, a0, a1);
+}
+
+// The fields a particle of cell c can gather, wave-uniform when c is: E at
+// c and c+1, the time-averaged H at c-1, c and c+1 (each formed as push_one
+// forms it).  Loaded once per one-cell group, ahead of the particles.
+template <typename R>
+struct CellFields {
+    R E0, E1, hm, h0, hp;
+};
+template <typename R>
+__device__ __forceinline__ CellFields<R> cell_fields(const PushArgs<R>& pa, int c) {
+#pragma clang fp contract(off)
+    const R e0 = ldo(pa.E, c), e1 = ldo(pa.E, c + 1);
+    const R pm = ldo(pa.Hp, c - 1), qm = ldo(pa.Hc, c - 1);
+    const R p0 = ldo(pa.Hp, c), q0 = ldo(pa.Hc, c);
+    const R pp = ldo(pa.Hp, c + 1), qp = ldo(pa.Hc, c + 1);
+    return CellFields<R>{e0, e1, R(0.5) * (pm + qm), R(0.5) * (p0 + q0), R(0.5) * (pp + qp)};
+}
+
+// push_one for a particle of the cell cf was loaded for: the same values
+// and operations (bitwise the same result), no per-particle gather.
+template <typename R>
+__device__ __forceinline__ void push_cell(const PushArgs<R>& pa, const CellFields<R>& cf, int& c, R& f, R& ux,
+                                          R& uy, R& uz, int& d, int& off, R& a0, R& a1) {
+#pragma clang fp contract(off)
+    const R Eg = (R(1) - f) * cf.E0 + f * cf.E1;
+    const bool hi = f >= R(0.5);
+    const R g = hi ? f - R(0.5) : f + R(0.5);
+    const R h0 = hi ? cf.h0 : cf.hm;
+    const R h1 = hi ? cf.hp : cf.h0;
+    const R Hg = (R(1) - g) * h0 + g * h1;
+    push_at<R>(pa, c, f, ux, uy, uz, Eg, Hg, d, off, a0, a1);
+}
+
+// One 64-particle chunk [base, base + 64) of a one-cell group whose
+// particles are [.., phi) (lane: the particle's offset; c, f, u: its state):
+// the push, its move class d, its rank among the cell's class d so far
+// (scalar carries n0, n1, n2 plus the earlier lanes of the class), the rank
+// record, and its CIC weights added to the lane's FP64 sums onto nodes
+// c-1 .. c+2.
+template <typename R>
+__device__ __forceinline__ void g1_chunk(const PushArgs<R>& pa, const CellFields<R>& cf, gptr<int32_t> trank,
+                                         int base, int phi, int lane, int c, R f, R ux, R uy, R uz, int& n0,
+                                         int& n1, int& n2, double (&acc)[4]) {
+    const int i = base + lane;
+    const bool valid = i < phi;
+    int d = 0, off = 1;
+    R a0 = R(0), a1 = R(0);
+    if (valid) push_cell<R>(pa, cf, c, f, ux, uy, uz, d, off, a0, a1);
+    const unsigned long long bd0 = __ballot(valid && d == -1);
+    const unsigned long long bd1 = __ballot(valid && d == 0);
+    const unsigned long long bd2 = __ballot(valid && d == 1);
+    const unsigned long long mine = d < 0 ? bd0 : (d == 0 ? bd1 : bd2);
+    const int carry = d < 0 ? n0 : (d == 0 ? n1 : n2);
+    const int r = carry + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mine >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((unsigned)mine, 0u));
+    if (valid) sto(trank, (unsigned)i, (r << 2) | (d + 1));
+    const R s0 = off == 0 ? a0 : R(0);
+    const R s1 = off == 0 ? a1 : (off == 1 ? a0 : R(0));
+    const R s2 = off == 1 ? a1 : (off == 2 ? a0 : R(0));
+    const R s3 = off == 2 ? a1 : R(0);
+    acc[0] += (double)s0;
+    acc[1] += (double)s1;
+    acc[2] += (double)s2;
+    acc[3] += (double)s3;
+    n0 += __popcll(bd0);
+    n1 += __popcll(bd1);
+    n2 += __popcll(bd2);
 }
 
 // kPushWaves independent waves per workgroup (a CU holds a limited number of
@@ -283,80 +360,96 @@ __global__ __launch_bounds__(64 * kPushWaves) void pic_push_count(const SimDev<R
     const int lo_cell = max(cmin - 1, 0), hi_cell = min(cmax + 1, N);   // counts cover the range +-1
     const int ngroups = (hi_cell - lo_cell + G) / G;
     auto clampi = [&](int v) { return min(max(v, 0), np); };
+    if (G == 1) {
+        // One-cell groups, two adjacent cells per wave with both cells' loads
+        // in flight together.  Every particle of a cell's range is in that
+        // cell, so the per-class carries are wave-uniform scalars, a
+        // particle's rank is its class carry plus the earlier lanes of its
+        // class (mbcnt), and no LDS table is needed.  Each cell keeps its own
+        // carries, lane sums and reduction, in chunk order: the same ranks,
+        // counts and sums (same order) as one cell per wave.
+        const int npairs = (hi_cell - lo_cell + 2) >> 1;
+        for (int gi = it.y; gi < npairs; gi += nwaves) {
+            const int gA = lo_cell + 2 * gi, gB = gA + 1;
+            const bool hasB = gB <= hi_cell;
+            // particles of live cells only: the graveyard cells 1 and N-2 can
+            // hold most of a beam once it has drifted out, and their particles
+            // neither move nor deposit (their cells' counts stay 0 below).
+            // Three bounds in one load: start of A (lane 0), start of B
+            // (lane 1), end of B (lane 2; an empty B when the pair has one cell).
+            const int bi = lane == 0 ? gA : (lane == 1 || !hasB ? gB : gB + 1);
+            const int sb = ldo(st, (unsigned)min(max(bi, 2), N - 2));
+            // the cells' fields, loaded alongside the bounds (clamped cells
+            // when a cell holds no live particle)
+            const CellFields<R> cfA = cell_fields<R>(pa, min(max(gA, 2), N - 3));
+            const CellFields<R> cfB = cell_fields<R>(pa, min(max(gB, 2), N - 3));
+            const int pA = clampi(__builtin_amdgcn_readlane(sb, 0));
+            const int pB = max(pA, clampi(__builtin_amdgcn_readlane(sb, 1)));
+            const int pE = max(pB, clampi(__builtin_amdgcn_readlane(sb, 2)));
+            int nA0 = 0, nA1 = 0, nA2 = 0, nB0 = 0, nB1 = 0, nB2 = 0;   // moving down / staying / up so far
+            double accA[4] = {0.0, 0.0, 0.0, 0.0}, accB[4] = {0.0, 0.0, 0.0, 0.0};
+            const int len = max(pB - pA, pE - pB);
+            for (int o = 0; o < len; o += 64 * kBatch) {
+                int ca[kBatch], cb[kBatch];
+                R fa[kBatch], xa[kBatch], ya[kBatch], za[kBatch];
+                R fb[kBatch], xb[kBatch], yb[kBatch], zb[kBatch];
+#pragma unroll
+                for (int q = 0; q < kBatch; ++q) {
+                    // unconditional loads at in-range indices (np >= 1)
+                    const unsigned ia = (unsigned)min(pA + o + 64 * q + lane, np - 1);
+                    const unsigned ib = (unsigned)min(pB + o + 64 * q + lane, np - 1);
+                    ca[q] = ldo(cell, ia);
+                    fa[q] = ldo(pf, ia);
+                    xa[q] = ldo(px, ia);
+                    ya[q] = ldo(py, ia);
+                    za[q] = ldo(pz, ia);
+                    cb[q] = ldo(cell, ib);
+                    fb[q] = ldo(pf, ib);
+                    xb[q] = ldo(px, ib);
+                    yb[q] = ldo(py, ib);
+                    zb[q] = ldo(pz, ib);
+                }
+#pragma unroll
+                for (int q = 0; q < kBatch; ++q)
+                    if (pA + o + 64 * q < pB)
+                        g1_chunk<R>(pa, cfA, trank, pA + o + 64 * q, pB, lane, ca[q], fa[q], xa[q], ya[q], za[q],
+                                    nA0, nA1, nA2, accA);
+#pragma unroll
+                for (int q = 0; q < kBatch; ++q)
+                    if (pB + o + 64 * q < pE)
+                        g1_chunk<R>(pa, cfB, trank, pB + o + 64 * q, pE, lane, cb[q], fb[q], xb[q], yb[q], zb[q],
+                                    nB0, nB1, nB2, accB);
+            }
+            wave_scan4(accA);                   // lane 63 holds the cells' sums
+            wave_scan4(accB);
+            if (lane == 63) {
+#pragma unroll
+                for (int w = 0; w < 4; ++w) Sg[4 * (size_t)gA + w] = accA[w];
+                if (hasB) {
+#pragma unroll
+                    for (int w = 0; w < 4; ++w) Sg[4 * (size_t)gB + w] = accB[w];
+                }
+            }
+            if (lane == 0) {
+                // H^{n-1/2} of the pair's cells for the scatter's push
+                cup[gA] = nA2;
+                cdown[gA] = nA0;
+                hold[gA] = pa.Hp[gA];
+                if (hasB) {
+                    cup[gB] = nB2;
+                    cdown[gB] = nB0;
+                    hold[gB] = pa.Hp[gB];
+                }
+            }
+        }
+        return;
+    }
     for (int gi = it.y; gi < ngroups; gi += nwaves) {
         const int g0 = lo_cell + gi * G;
         const int g1 = min(g0 + G, hi_cell + 1);
-        // particles of live cells only: the graveyard cells 1 and N-2 can hold
-        // most of a beam once it has drifted out, and their particles neither
-        // move nor deposit (their cells' counts stay 0 below).  Both bounds
-        // come in with one load (lane 0 and lane 1), read back with readlane.
+        // live cells only, as above; both bounds come in with one load (lane 0
+        // and lane 1), read back with readlane
         const int sb = ldo(st, (unsigned)(lane == 0 ? max(g0, 2) : min(g1, N - 2)));
-        if (G == 1) {
-            // One cell: every particle of the range is in cell g0, so the
-            // per-class carries are wave-uniform scalars, a particle's rank is
-            // its class carry plus the earlier lanes of its class (mbcnt), and
-            // no LDS table is needed.  Same ranks, counts and sums (same order)
-            // as the general path below.
-            const int plo1 = clampi(__builtin_amdgcn_readlane(sb, 0));
-            const int phi1 = max(plo1, clampi(__builtin_amdgcn_readlane(sb, 1)));
-            int n0 = 0, n1 = 0, n2 = 0;        // particles moving down / staying / up so far
-            double acc1[4] = {0.0, 0.0, 0.0, 0.0};
-            for (int bb = plo1; bb < phi1; bb += 64 * kBatch) {
-                int cc[kBatch];
-                R ff[kBatch], xx[kBatch], yy[kBatch], zz[kBatch];
-#pragma unroll
-                for (int q = 0; q < kBatch; ++q) {
-                    const unsigned ic = (unsigned)min(bb + 64 * q + lane, np - 1);
-                    cc[q] = ldo(cell, ic);
-                    ff[q] = ldo(pf, ic);
-                    xx[q] = ldo(px, ic);
-                    yy[q] = ldo(py, ic);
-                    zz[q] = ldo(pz, ic);
-                }
-#pragma unroll
-                for (int q = 0; q < kBatch; ++q) {
-                    const int base = bb + 64 * q;
-                    if (base >= phi1) break;
-                    const int i = base + lane;
-                    const bool valid = i < phi1;
-                    int c = cc[q], d = 0, off = 1;
-                    R f = ff[q], ux = xx[q], uy = yy[q], uz = zz[q];
-                    R a0 = R(0), a1 = R(0);
-                    if (valid) push_one<R>(pa, c, f, ux, uy, uz, d, off, a0, a1);
-                    const unsigned long long bd0 = __ballot(valid && d == -1);
-                    const unsigned long long bd1 = __ballot(valid && d == 0);
-                    const unsigned long long bd2 = __ballot(valid && d == 1);
-                    const unsigned long long mine = d < 0 ? bd0 : (d == 0 ? bd1 : bd2);
-                    const int carry = d < 0 ? n0 : (d == 0 ? n1 : n2);
-                    const int r = carry + (int)__builtin_amdgcn_mbcnt_hi(
-                                              (unsigned)(mine >> 32),
-                                              __builtin_amdgcn_mbcnt_lo((unsigned)mine, 0u));
-                    if (valid) sto(trank, i, (r << 2) | (d + 1));
-                    const R s0 = off == 0 ? a0 : R(0);
-                    const R s1 = off == 0 ? a1 : (off == 1 ? a0 : R(0));
-                    const R s2 = off == 1 ? a1 : (off == 2 ? a0 : R(0));
-                    const R s3 = off == 2 ? a1 : R(0);
-                    acc1[0] += (double)s0;
-                    acc1[1] += (double)s1;
-                    acc1[2] += (double)s2;
-                    acc1[3] += (double)s3;
-                    n0 += __popcll(bd0);
-                    n1 += __popcll(bd1);
-                    n2 += __popcll(bd2);
-                }
-            }
-            wave_scan4(acc1);                   // lane 63 holds the cell's sums
-            if (lane == 63) {
-#pragma unroll
-                for (int w = 0; w < 4; ++w) Sg[4 * (size_t)g0 + w] = acc1[w];
-            }
-            if (lane == 0) {
-                cup[g0] = n2;
-                cdown[g0] = n0;
-                hold[g0] = pa.Hp[g0];
-            }
-            continue;
-        }
         for (int k = lane; k < G; k += 64) {
             cnt[k][0] = cnt[k][1] = cnt[k][2] = 0;
             Sl[k][0] = Sl[k][1] = Sl[k][2] = Sl[k][3] = 0.0;

@@ -1,4 +1,4 @@
-"""CPU check of the index arithmetic of the PIC kernels (pic.hip:
+"""CPU check of the index arithmetic of the PIC kernels (pic_kernels.hpp:
 pic_push_count and pic_scatter), emulated lane by lane over several steps of
 random moves: the new indices form the stable sort by new cell with parked
 particles keeping their index, start[] stays the exclusive count scan, J'
@@ -57,3 +57,51 @@ def test_pic_step_indices(N, n, lo, hi, G, steps):
         exp = (int(newcell[nl].min()), int(newcell[nl].max())) if nl.any() else (N, -1)
         assert nrange == exp, (step, nrange, exp)
         cell, ids, start, crange, urange = newcell, ids[order], nstart, nrange, nurange
+
+
+def _one_cell_pairs(start, cmin, cmax, N, np_):
+    """The particle ranges push/count's one-cell path reads, two adjacent
+    cells per wave (pic_kernels.hpp, G == 1): each pair loads three cell
+    starts (lanes 0-2) at clamped cells and reads them back with readlane."""
+    clampi = lambda v: min(max(v, 0), np_)
+    cl = lambda c: min(max(c, 2), N - 2)
+    lo_cell, hi_cell = max(cmin - 1, 0), min(cmax + 1, N)
+    out = []
+    for gi in range((hi_cell - lo_cell + 2) >> 1):
+        gA = lo_cell + 2 * gi
+        gB = gA + 1
+        hasB = gB <= hi_cell
+        sb = [start[cl(gA)], start[cl(gB)], start[cl(gB + 1 if hasB else gB)]]
+        pA = clampi(sb[0])
+        pB = max(pA, clampi(sb[1]))
+        pE = max(pB, clampi(sb[2]))
+        out.append((gA, pA, pB))
+        if hasB:
+            out.append((gB, pB, pE))
+        else:
+            assert pE == pB
+    return out
+
+
+@pytest.mark.parametrize("N,lo,hi,n", [(64, 2, 62, 900), (64, 2, 5, 300), (64, 58, 62, 300), (200, 90, 91, 50),
+                                       (200, 40, 160, 7)])
+def test_one_cell_pairs_cover_each_cell_once(N, lo, hi, n):
+    """Pairs of one-cell groups visit every cell of [cmin-1, cmax+1] once, in
+    order, each with the particle range the one-cell-per-wave path reads
+    (start[max(c, 2)] .. start[min(c+1, N-2)]); graveyard cells read nothing."""
+    rng = np.random.default_rng(N + lo + hi + n)
+    for park in (0, 1):
+        cell = np.sort(np.r_[rng.integers(lo, hi, n), np.full(5 * park, 1), np.full(7 * park, N - 2)])
+        start = np.searchsorted(cell, np.arange(N + 2), side="left").astype(np.int64)
+        live = (cell >= 2) & (cell <= N - 3)
+        cmin, cmax = int(cell[live].min()), int(cell[live].max())
+        got = _one_cell_pairs(start, cmin, cmax, N, len(cell))
+        lo_cell, hi_cell = max(cmin - 1, 0), min(cmax + 1, N)
+        assert [g for g, _, _ in got] == list(range(lo_cell, hi_cell + 1))
+        for g, p0, p1 in got:
+            plo = start[max(g, 2)]
+            phi = max(plo, start[min(g + 1, N - 2)])
+            assert (p0, p1) == (plo, phi), g
+            assert np.all(cell[p0:p1] == g)
+        # every live particle is read exactly once
+        assert sum(p1 - p0 for _, p0, p1 in got) == int(live.sum())
