@@ -304,6 +304,34 @@ __device__ __forceinline__ void g1_chunk(const PushArgs<R>& pa, const CellFields
     n2 += __popcll(bd2);
 }
 
+// The fields of a multi-cell group, staged in the wave's LDS once per group:
+// E and the time-averaged H (formed as push_one forms it) at the nodes of
+// cells c0 - 1 .. c0 + n (entry k holds node c0 - 1 + k, clamped to the grid).
+template <typename R>
+__device__ __forceinline__ void stage_fields(const PushArgs<R>& pa, R* Et, R* Ht, int c0, int n, int lane) {
+#pragma clang fp contract(off)
+    for (int k = lane; k < n + 2; k += 64) {
+        const unsigned cc = (unsigned)min(max(c0 - 1 + k, 0), pa.N);
+        Et[k] = ldo(pa.E, cc);
+        Ht[k] = R(0.5) * (ldo(pa.Hp, cc) + ldo(pa.Hc, cc));
+    }
+}
+
+// push_one for a particle of cell c in [c0, c0 + n) with the staged fields:
+// the same values and operations (bitwise the same result).
+template <typename R>
+__device__ __forceinline__ void push_tab(const PushArgs<R>& pa, const R* Et, const R* Ht, int c0, int& c, R& f,
+                                         R& ux, R& uy, R& uz, int& d, int& off, R& a0, R& a1) {
+#pragma clang fp contract(off)
+    const int k = c - c0 + 1;
+    const R Eg = (R(1) - f) * Et[k] + f * Et[k + 1];
+    int j;
+    R g;
+    if (f >= R(0.5)) { j = k; g = f - R(0.5); } else { j = k - 1; g = f + R(0.5); }
+    const R Hg = (R(1) - g) * Ht[j] + g * Ht[j + 1];
+    push_at<R>(pa, c, f, ux, uy, uz, Eg, Hg, d, off, a0, a1);
+}
+
 // kPushWaves independent waves per workgroup (a CU holds a limited number of
 // workgroups; single-wave workgroups would cap the resident waves), each with
 // its own LDS tables; item 4*block + wave, items with x < 0 are padding.
@@ -316,11 +344,14 @@ __global__ __launch_bounds__(64 * kPushWaves) void pic_push_count(const SimDev<R
                                                                   long long launch_idx) {
     __shared__ int cnt_all[kPushWaves][kPicGMax][3];    // down, stay, up so far per group cell (rank carry)
     __shared__ double S_all[kPushWaves][kPicGMax][4];   // J' sums onto nodes c-1 .. c+2
+    __shared__ R fld_all[kPushWaves][2][kPicGMax + 2];  // the group's E and averaged H (stage_fields)
     // wave-uniform by construction; readfirstlane lets the compiler see it, so
     // the item and the descriptors come in through scalar loads
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     int (&cnt)[kPicGMax][3] = cnt_all[wv];
     double (&Sl)[kPicGMax][4] = S_all[wv];
+    R* const Et = fld_all[wv][0];
+    R* const Ht = fld_all[wv][1];
     const int2 it = items[blockIdx.x * kPushWaves + wv];
     if (it.x < 0) return;
     const BeamDev<R>& b = beams[it.x];
@@ -454,6 +485,7 @@ __global__ __launch_bounds__(64 * kPushWaves) void pic_push_count(const SimDev<R
             cnt[k][0] = cnt[k][1] = cnt[k][2] = 0;
             Sl[k][0] = Sl[k][1] = Sl[k][2] = Sl[k][3] = 0.0;
         }
+        stage_fields<R>(pa, Et, Ht, g0, g1 - g0, lane);
         wave_sync();
         const int plo = clampi(__builtin_amdgcn_readlane(sb, 0));
         const int phi = max(plo, clampi(__builtin_amdgcn_readlane(sb, 1)));
@@ -485,7 +517,7 @@ __global__ __launch_bounds__(64 * kPushWaves) void pic_push_count(const SimDev<R
                 R f = ff[q], ux = xx[q], uy = yy[q], uz = zz[q];
                 R a0 = R(0), a1 = R(0);
                 const bool live = k.valid && c_old >= 2 && c_old <= N - 3;
-                if (live) push_one<R>(pa, c, f, ux, uy, uz, d, off, a0, a1);
+                if (live) push_tab<R>(pa, Et, Ht, g0, c, f, ux, uy, uz, d, off, a0, a1);
                 const unsigned long long seg = lanes_le(lane) & ~lanes_lt(k.fl);
                 const unsigned long long bd0 = __ballot(k.valid && d == -1);
                 const unsigned long long bd1 = __ballot(k.valid && d == 0);
