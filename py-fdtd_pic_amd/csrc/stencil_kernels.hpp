@@ -540,8 +540,12 @@ __device__ __forceinline__ void tile_body(const SimDev<R>& sd, int out0_, long l
         }
     }
 
-    // one lean loop per tile class
-    const bool gen = c.Jc != nullptr || c.snap_interval > 0 || c.npr > 2;
+    // one lean loop per tile class.  With a beam, J' is formed from the CIC
+    // sums of this step's live range [cmin, cmax] and is zero off the nodes
+    // cmin-2 .. cmax+3; a tile whose window misses them runs without J.
+    bool needJ = c.Jc != nullptr;
+    if (c.pS) needJ = c.pcmin <= c.pcmax && c.t0 + 64 * K - 1 >= c.pcmin - 2 && c.t0 <= c.pcmax + 3;
+    const bool gen = needJ || c.snap_interval > 0 || c.npr > 2;
     const bool extras = tile_strip || tile_src || tile_tf || c.npr > 0;
     // A slab-boundary tile of the E-form Kerr slab runs the pure-slab loop:
     // a vacuum cell is a slab cell with D = E, P = 0, eps = 1, chi3 = 0 when
