@@ -489,23 +489,36 @@ __global__ __launch_bounds__(64 * kPushWaves) void pic_push_count(const SimDev<R
         wave_sync();
         const int plo = clampi(__builtin_amdgcn_readlane(sb, 0));
         const int phi = max(plo, clampi(__builtin_amdgcn_readlane(sb, 1)));
+        // particle loads one batch ahead: the next batch is in flight while
+        // this one is processed (the wave fences below do not wait for it);
+        // unconditional loads at an in-range index (np >= 1), selects on use
+        int nc[kBatch];
+        R nf[kBatch], nx[kBatch], ny[kBatch], nz[kBatch];
+        auto fetch = [&](int bb) {
+#pragma unroll
+            for (int q = 0; q < kBatch; ++q) {
+                const unsigned ic = (unsigned)min(bb + 64 * q + lane, np - 1);
+                nc[q] = ldo(cell, ic);
+                nf[q] = ldo(pf, ic);
+                nx[q] = ldo(px, ic);
+                ny[q] = ldo(py, ic);
+                nz[q] = ldo(pz, ic);
+            }
+        };
+        if (plo < phi) fetch(plo);
         for (int bb = plo; bb < phi; bb += 64 * kBatch) {
             int cc[kBatch];
             R ff[kBatch], xx[kBatch], yy[kBatch], zz[kBatch];
 #pragma unroll
             for (int q = 0; q < kBatch; ++q) {
-                // unconditional loads at an in-range index (np >= 1), then selects
-                const int i = bb + 64 * q + lane;
-                const bool v = i < phi;
-                const unsigned ic = (unsigned)min(i, np - 1);
-                const int c_ = ldo(cell, ic);
-                const R f_ = ldo(pf, ic), x_ = ldo(px, ic), y_ = ldo(py, ic), z_ = ldo(pz, ic);
-                cc[q] = v ? c_ : 0x7fffffff;
-                ff[q] = v ? f_ : R(0);
-                xx[q] = v ? x_ : R(0);
-                yy[q] = v ? y_ : R(0);
-                zz[q] = v ? z_ : R(0);
+                const bool v = bb + 64 * q + lane < phi;
+                cc[q] = v ? nc[q] : 0x7fffffff;
+                ff[q] = v ? nf[q] : R(0);
+                xx[q] = v ? nx[q] : R(0);
+                yy[q] = v ? ny[q] : R(0);
+                zz[q] = v ? nz[q] : R(0);
             }
+            if (bb + 64 * kBatch < phi) fetch(bb + 64 * kBatch);
 #pragma unroll
             for (int q = 0; q < kBatch; ++q) {
                 const int base = bb + 64 * q;
