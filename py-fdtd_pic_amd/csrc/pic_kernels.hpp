@@ -420,26 +420,39 @@ __global__ __launch_bounds__(64 * kPushWaves) void pic_push_count(const SimDev<R
             int nA0 = 0, nA1 = 0, nA2 = 0, nB0 = 0, nB1 = 0, nB2 = 0;   // moving down / staying / up so far
             double accA[4] = {0.0, 0.0, 0.0, 0.0}, accB[4] = {0.0, 0.0, 0.0, 0.0};
             const int len = max(pB - pA, pE - pB);
+            // particle loads one batch ahead (cells of more than 64 * kBatch
+            // particles), unconditional at in-range indices (np >= 1)
+            int nca[kBatch], ncb[kBatch];
+            R nfa[kBatch], nxa[kBatch], nya[kBatch], nza[kBatch];
+            R nfb[kBatch], nxb[kBatch], nyb[kBatch], nzb[kBatch];
+            auto fetch = [&](int o) {
+#pragma unroll
+                for (int q = 0; q < kBatch; ++q) {
+                    const unsigned ia = (unsigned)min(pA + o + 64 * q + lane, np - 1);
+                    const unsigned ib = (unsigned)min(pB + o + 64 * q + lane, np - 1);
+                    nca[q] = ldo(cell, ia);
+                    nfa[q] = ldo(pf, ia);
+                    nxa[q] = ldo(px, ia);
+                    nya[q] = ldo(py, ia);
+                    nza[q] = ldo(pz, ia);
+                    ncb[q] = ldo(cell, ib);
+                    nfb[q] = ldo(pf, ib);
+                    nxb[q] = ldo(px, ib);
+                    nyb[q] = ldo(py, ib);
+                    nzb[q] = ldo(pz, ib);
+                }
+            };
+            if (len > 0) fetch(0);
             for (int o = 0; o < len; o += 64 * kBatch) {
                 int ca[kBatch], cb[kBatch];
                 R fa[kBatch], xa[kBatch], ya[kBatch], za[kBatch];
                 R fb[kBatch], xb[kBatch], yb[kBatch], zb[kBatch];
 #pragma unroll
                 for (int q = 0; q < kBatch; ++q) {
-                    // unconditional loads at in-range indices (np >= 1)
-                    const unsigned ia = (unsigned)min(pA + o + 64 * q + lane, np - 1);
-                    const unsigned ib = (unsigned)min(pB + o + 64 * q + lane, np - 1);
-                    ca[q] = ldo(cell, ia);
-                    fa[q] = ldo(pf, ia);
-                    xa[q] = ldo(px, ia);
-                    ya[q] = ldo(py, ia);
-                    za[q] = ldo(pz, ia);
-                    cb[q] = ldo(cell, ib);
-                    fb[q] = ldo(pf, ib);
-                    xb[q] = ldo(px, ib);
-                    yb[q] = ldo(py, ib);
-                    zb[q] = ldo(pz, ib);
+                    ca[q] = nca[q]; fa[q] = nfa[q]; xa[q] = nxa[q]; ya[q] = nya[q]; za[q] = nza[q];
+                    cb[q] = ncb[q]; fb[q] = nfb[q]; xb[q] = nxb[q]; yb[q] = nyb[q]; zb[q] = nzb[q];
                 }
+                if (o + 64 * kBatch < len) fetch(o + 64 * kBatch);
 #pragma unroll
                 for (int q = 0; q < kBatch; ++q)
                     if (pA + o + 64 * q < pB)
