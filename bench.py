@@ -24,9 +24,10 @@ sys.path.insert(0, os.path.join(ROOT, "py-fdtd_pic_amd"))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# Launches stamped with timing events: one in PROFILE_EVERY (an event pair
-# costs the host ~20 us, more than a launch; stamping every launch would make
-# the timed run host-bound).
+# Timing events: a field-only run is timed in runs of PROFILE_EVERY
+# consecutive launches between two stream markers; a PIC run stamps one step
+# in PROFILE_EVERY kernel by kernel (an event pair costs the host ~20 us, more
+# than a launch; stamping every launch would make the timed run host-bound).
 PROFILE_EVERY = 16
 METRIC = "Gcell-updates/s + Gparticle-pushes/s at \u22641e-5 rel L2 vs FP64; p50/p99 sweep latency"
 
@@ -303,7 +304,7 @@ def main():
     cells = p["N"] * p["steps"]
     value = cells / (wall / a.steps) / 1e9
     bpc = configs.bytes_per_cell_update(p, a.precision)
-    launch_ms = kern_ms / max(sampled, 1)                 # average over the sampled launches
+    launch_ms = kern_ms / max(sampled, 1)                 # average launch-to-launch duration
     achieved = bpc * p["N"] * S / (launch_ms / 1e3) / 1e9 if kern_ms > 0 else 0.0
     out = {
         "metric": METRIC,

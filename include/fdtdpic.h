@@ -224,19 +224,23 @@ int fdtd_step_op(fdtd_ctx *ctx, int op, int narr, double *const *arrays, const i
 
 /* ---- diagnostics ------------------------------------------------------ */
 const char *fdtd_last_error(void);
-/* Device time (ms) of the sampled stencil launches since the last reset, from
- * HIP events stamped by the kernels' own dispatch packets; also how many
- * launches were sampled. */
+/* Device time (ms) of the sampled field launches since the last reset (see
+ * fdtd_set_profile for how they are sampled); also how many launches the
+ * samples hold. */
 int fdtd_timing(fdtd_ctx *ctx, double *ms, int64_t *launches, int reset);
 /* Device time (ms) and launch count of one kernel kind (FDTD_KERNEL_*) since
  * the last reset, under profiling. */
 int fdtd_kernel_timing(fdtd_ctx *ctx, int kind, double *ms, int64_t *launches);
-/* Profiling: every `every`-th launch (0: off, 1: all) carries start/stop HIP
- * events stamped by the kernels' dispatch packets, so fdtd_timing and
- * fdtd_kernel_timing report the summed durations and the number of sampled
- * launches (their ratio is the average launch duration used for the
- * roofline).  An event pair costs more host time than a launch, so sampling
- * keeps a long run GPU-bound. */
+/* Profiling (every = 0: off).  Without a beam, runs of `every` consecutive
+ * field launches are bracketed by two stream markers (HIP events recorded on
+ * the context's stream), so fdtd_timing reports the summed duration of the
+ * runs and the number of launches in them: their ratio is the average launch
+ * duration used for the roofline, gaps between launches included, and no
+ * launch carries an event of its own.  With a beam, every `every`-th step's
+ * kernels (1: all) carry start/stop events stamped by their own dispatch
+ * packets, and fdtd_kernel_timing reports them per kernel kind.  An event
+ * pair costs more host time than a launch, so sampling keeps a long run
+ * GPU-bound. */
 int fdtd_set_profile(fdtd_ctx *ctx, int every);
 /* Bytes of device memory the context currently holds. */
 int fdtd_device_bytes(fdtd_ctx *ctx, int64_t *bytes);

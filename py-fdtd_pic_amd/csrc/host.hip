@@ -686,6 +686,7 @@ static EvPair* next_event(fdtd_ctx* c, int kind) {
     }
     EvPair* e = &c->events[c->ev_used++];
     e->kind = kind;
+    e->launches = 1;
     return e;
 }
 
@@ -729,6 +730,17 @@ static int launch_once(fdtd_ctx* c, int n, EvPair* ev, hipStream_t st, bool fuse
     return FDTD_OK;
 }
 
+// Field-only profiling brackets a run of consecutive launches with stream
+// markers; this closes the open run (before a rebuild, at the end of a run
+// call, before the timings are read).
+static int close_bracket(fdtd_ctx* c) {
+    if (c->bracket < 0) return FDTD_OK;
+    EvPair& b = c->events[(size_t)c->bracket];
+    c->bracket = -1;
+    if (b.launches > 0) HIPCHK(hipEventRecord(b.second, c->st));
+    return FDTD_OK;
+}
+
 extern "C" {
 
 int fdtd_run(fdtd_ctx* c, int64_t nsteps) {
@@ -737,7 +749,9 @@ int fdtd_run(fdtd_ctx* c, int64_t nsteps) {
     int64_t remaining = nsteps;
     while (remaining > 0) {
         if (c->dirty) {
-            int rc = c->prec == FDTD_F64 ? rebuild_t<double>(c) : rebuild_t<float>(c);
+            int rc = close_bracket(c);
+            if (rc) return rc;
+            rc = c->prec == FDTD_F64 ? rebuild_t<double>(c) : rebuild_t<float>(c);
             if (rc) return rc;
         }
         if (c->ntiles == 0) break;
@@ -778,12 +792,33 @@ int fdtd_run(fdtd_ctx* c, int64_t nsteps) {
                 return fdtd_fail(FDTD_E_HIP, std::string("push/count launch ") + std::to_string(c->launch_idx) +
                                                  " (" + std::to_string(c->npitems) + " waves): " + hipGetErrorString(e));
         }
-        // a stencil event only when some field launch runs by itself
-        EvPair* ev = (prof && !(fused && c->groups.size() == 1)) ? next_event(c, FDTD_KERNEL_STENCIL) : nullptr;
-        if (prof && !ev && !(fused && c->groups.size() == 1)) return FDTD_E_HIP;
-        EvPair fev{e2, e3, FDTD_KERNEL_FIELD_SORT};
-        int rc = launch_once(c, n, ev, c->st, fused, prof ? &fev : nullptr);
-        if (rc) return rc;
+        int rc;
+        if (!c->has_pic && c->profile > 0) {
+            // field only: runs of `profile` consecutive launches between two
+            // stream markers, so the average launch duration includes the
+            // gaps between launches and no launch carries an event of its
+            // own (a launch stamped by its dispatch packet runs ~1 us longer)
+            if (c->bracket < 0) {
+                EvPair* b = next_event(c, FDTD_KERNEL_STENCIL);
+                if (!b) return FDTD_E_HIP;
+                b->launches = 0;
+                HIPCHK(hipEventRecord(b->first, c->st));
+                c->bracket = (long long)(b - c->events.data());
+            }
+            rc = launch_once(c, n, nullptr, c->st);
+            if (rc) return rc;
+            if (++c->events[(size_t)c->bracket].launches == c->profile) {
+                rc = close_bracket(c);
+                if (rc) return rc;
+            }
+        } else {
+            // a stencil event only when some field launch runs by itself
+            EvPair* ev = (prof && !(fused && c->groups.size() == 1)) ? next_event(c, FDTD_KERNEL_STENCIL) : nullptr;
+            if (prof && !ev && !(fused && c->groups.size() == 1)) return FDTD_E_HIP;
+            EvPair fev{e2, e3, FDTD_KERNEL_FIELD_SORT, 1};
+            rc = launch_once(c, n, ev, c->st, fused, prof ? &fev : nullptr);
+            if (rc) return rc;
+        }
         if (c->has_pic && !fused) {
             hipError_t e = c->prec == FDTD_F64
                 ? launch_scatter<double>((const SimDev<double>*)c->d_sims, (const BeamDev<double>*)c->d_beams,
@@ -810,7 +845,7 @@ int fdtd_run(fdtd_ctx* c, int64_t nsteps) {
         remaining -= n;
         if (finished) c->dirty = true;
     }
-    return FDTD_OK;
+    return close_bracket(c);
 }
 
 int fdtd_run_all(fdtd_ctx* c) {
@@ -847,15 +882,17 @@ int fdtd_busy(fdtd_ctx* c, int* busy) {
 int fdtd_kernel_timing(fdtd_ctx* c, int kind, double* ms, int64_t* launches) {
     if (!c) return fdtd_fail(FDTD_E_ARG, "null ctx");
     if (kind < FDTD_KERNEL_STENCIL || kind > FDTD_KERNEL_FIELD_SORT) return fdtd_fail(FDTD_E_ARG, "bad kernel kind");
+    int rc = close_bracket(c);
+    if (rc) return rc;
     HIPCHK(hipStreamSynchronize(c->st));
     double tot = 0;
     int64_t n = 0;
     for (size_t i = 0; i < c->ev_used; ++i) {
-        if (c->events[i].kind != kind) continue;
+        if (c->events[i].kind != kind || c->events[i].launches <= 0) continue;
         float m = 0;
         HIPCHK(hipEventElapsedTime(&m, c->events[i].first, c->events[i].second));
         tot += m;
-        ++n;
+        n += c->events[i].launches;
     }
     if (ms) *ms = tot;
     if (launches) *launches = n;
@@ -865,7 +902,10 @@ int fdtd_kernel_timing(fdtd_ctx* c, int kind, double* ms, int64_t* launches) {
 int fdtd_timing(fdtd_ctx* c, double* ms, int64_t* launches, int reset) {
     int rc = fdtd_kernel_timing(c, FDTD_KERNEL_STENCIL, ms, launches);
     if (rc) return rc;
-    if (reset) c->ev_used = 0;
+    if (reset) {
+        c->ev_used = 0;
+        c->bracket = -1;
+    }
     return FDTD_OK;
 }
 
@@ -874,6 +914,7 @@ int fdtd_set_profile(fdtd_ctx* c, int every) {
     if (every < 0) return fdtd_fail(FDTD_E_ARG, "negative sampling interval");
     c->profile = every;
     c->ev_used = 0;
+    c->bracket = -1;
     return FDTD_OK;
 }
 

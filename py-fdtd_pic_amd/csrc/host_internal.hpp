@@ -41,6 +41,7 @@ struct BeamHost {
 struct EvPair {
     hipEvent_t first = nullptr, second = nullptr;
     int kind = 0;               // FDTD_KERNEL_*
+    int launches = 1;           // launches between first and second
 };
 
 struct TileGroup {
@@ -73,6 +74,7 @@ struct fdtd_ctx {
     int profile = 0;            // stamp every profile-th launch with events (0: off)
     std::vector<EvPair> events;   // profiling: start/stop per launch, tagged by kernel kind
     size_t ev_used = 0;
+    long long bracket = -1;       // field-only profiling: the open run of launches (index into events)
     long long launches = 0;
     long long dev_bytes = 0;
     // particle beams
