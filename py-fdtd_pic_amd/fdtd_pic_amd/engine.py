@@ -208,10 +208,14 @@ class Context:
         return bool(b.value)
 
     def set_profile(self, on=True, every=1):
-        """Stamp every `every`-th launch with timing events (off: on=False)."""
+        """Timing events (off: on=False): without a beam, runs of `every`
+        consecutive field launches between two stream markers; with a beam,
+        every `every`-th step's kernels stamped one by one."""
         check(self.lib.fdtd_set_profile(self.h, int(every) if on else 0))
 
     def timing(self, reset=False):
+        """(ms, launches) of the sampled field launches; ms / launches is the
+        average launch duration."""
         ms, n = C.c_double(), C.c_int64()
         check(self.lib.fdtd_timing(self.h, C.byref(ms), C.byref(n), 1 if reset else 0))
         return ms.value, n.value
