@@ -65,6 +65,7 @@ def parse():
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "stencil_tiles_config2_hbm_traffic.txt")
+PIC_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pic_config3_kernel_trace_and_traffic.txt")
 
 
 def calibrate_copy():
@@ -89,6 +90,18 @@ def measured_traffic(precision, K, S, Ks):
         if len(f) == 6 and f[0] == "traffic" and f[1] == precision and f[2] == f"K={K}" and f[3] == f"Kslab={Ks}" \
                 and f[4] == f"S={S}":
             return float(f[5])
+    return None
+
+
+def measured_pic_traffic(K, fused):
+    """HBM bytes per PIC step (every kernel of the step) from the committed
+    config 3 rocprofv3 --pmc summary, or None when none matches."""
+    if not os.path.exists(PIC_TRAFFIC_FILE):
+        return None
+    for line in open(PIC_TRAFFIC_FILE):
+        f = line.split()
+        if len(f) == 5 and f[:2] == ["traffic", "pic"] and f[2] == f"K={K}" and f[3] == f"fused={int(fused)}":
+            return float(f[4])
     return None
 
 
@@ -160,7 +173,9 @@ def pic_bench(a):
            "kernels_avg_us": {k: round(v[0] * 1e3 / max(v[1], 1), 2) for k, v in kt.items()},
            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_push": bpp,
-                        "bytes_per_cell_update": round(bpc, 2), "traffic": None}}
+                        "bytes_per_cell_update": round(bpc, 2),
+                        "traffic": measured_pic_traffic(a.pic_K, a.pic_fusion == "on"),
+                        "traffic_per": "step (push/count + field step + scatter)"}}
     if not a.no_cpu:
         out["cpu_baseline"] = pic_cpu_baseline(p, x, u, q, m, w)
     return out
