@@ -60,19 +60,13 @@ class SimSpec:
         plain = (ke == expect_ke) & (kh == expect_kh)
         for a in arrs[2:]:
             plain &= (a == 0.0)
-        # longest run of plain cells
+        # longest run of plain cells (the first one on ties)
+        edges = np.diff(np.concatenate(([0], plain.astype(np.int8), [0])))
+        starts, ends = np.flatnonzero(edges == 1), np.flatnonzero(edges == -1)
         best = (0, 0)
-        i = 0
-        while i < n:
-            if plain[i]:
-                j = i
-                while j < n and plain[j]:
-                    j += 1
-                if j - i > best[1] - best[0]:
-                    best = (i, j)
-                i = j
-            else:
-                i += 1
+        if len(starts):
+            k = int(np.argmax(ends - starts))
+            best = (int(starts[k]), int(ends[k]))
         Ls, Rs = best if best[1] > best[0] else (n, n)
         self.Ls, self.Rs = int(Ls), int(Rs)
         self.ke0, self.kh0 = float(ke0), float(kh0)
@@ -281,12 +275,21 @@ def pml_coefficients(N, W, Sc, order=3, R0=1e-8):
     """
     n = np.arange(N + 1, dtype=np.float64)
     s0 = Sc * (order + 1) * (-math.log(R0)) / (2.0 * W)
-    depth_e = np.where(n < W, (W - n) / W, np.where(n >= N - W, (n - (N - W)) / W, 0.0))
-    depth_h = np.where(n < W, (W - n - 0.5) / W,
-                       np.where(n >= N - W, np.minimum((n + 0.5 - (N - W)) / W, 1.0), 0.0))
-    bE = np.exp(-s0 * depth_e ** order)
-    bH = np.exp(-s0 * depth_h ** order)
-    return bE, bE - 1.0, bH, bH - 1.0, depth_e > 0, depth_h > 0
+    # depth 0 (b = exp(-0) = 1 exactly, c = 0) off the strips: evaluate the
+    # strip cells only
+    idx = np.flatnonzero((n < W) | (n >= N - W))
+    m = n[idx]
+    de = np.where(m < W, (W - m) / W, (m - (N - W)) / W)
+    dh = np.where(m < W, (W - m - 0.5) / W, np.minimum((m + 0.5 - (N - W)) / W, 1.0))
+    bE = np.ones(N + 1)
+    bH = np.ones(N + 1)
+    bE[idx] = np.exp(-s0 * de ** order)
+    bH[idx] = np.exp(-s0 * dh ** order)
+    in_e = np.zeros(N + 1, dtype=bool)
+    in_h = np.zeros(N + 1, dtype=bool)
+    in_e[idx] = de > 0
+    in_h[idx] = dh > 0
+    return bE, bE - 1.0, bH, bH - 1.0, in_e, in_h
 
 
 def lorentz_coefficients(dt, omega0, d_eps, gamma):
