@@ -210,6 +210,8 @@ def config4_bench(a):
     sch.run_batch(jobs, costly_first=a.c4_order == "costly")
     wall = time.perf_counter() - t0
     tiling = (sch.K, sch.S)
+    host = {"admission_s": round(sch.admit_s, 3), "admission_prepare_s": round(sch.admit_prep_s, 3),
+            "admission_add_s": round(sch.admit_add_s, 3), "device_wait_s": round(sch.wait_s, 3)}
     sch.close()
     cells = sum(serve.work(p)[0] for p in inst)
     pushes = sum(serve.work(p)[1] for p in inst)
@@ -251,7 +253,7 @@ def config4_bench(a):
             "per_sim_loop_sample": f"{len(sample)} jobs, half with beams, seeded; scaled per stratum by work",
             "per_sim_loop_sample_s": round(alone, 3),
             "per_sim_loop_est_s": round(est, 3), "speedup_vs_per_sim_loop": round(est / wall, 2),
-            "admission_order": a.c4_order,
+            "admission_order": a.c4_order, "batched_host": host,
             "spectra": "R(f), T(f) at 256 frequencies per job, FP64 DFT on the device"}
 
 
