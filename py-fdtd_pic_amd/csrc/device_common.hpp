@@ -44,6 +44,11 @@ enum SlabKind { SLAB_NONE = 0, SLAB_DIEL = 1, SLAB_LINEAR = 2, SLAB_KERR_REF = 3
 // keep their index from then on.
 constexpr int kPicGMax = 64;        // cells per push/count group (LDS tables)
 constexpr int kScatterPer = 2;      // particles per scatter thread (a block covers 256 * kScatterPer)
+// push/count waves per workgroup (a CU holds a limited number of workgroups;
+// single-wave workgroups would cap the resident waves; 8 measured slower:
+// config 3 push/count 13.0 against 11.6 us); the host pads the work list to
+// whole workgroups of this many waves
+constexpr int kPushWaves = 4;
 template <typename R>
 struct BeamDev {
     int64_t launch_begin;  // the simulation's, copied: the parity without a load of the sim descriptor

@@ -612,7 +612,7 @@ static int rebuild_t(fdtd_ctx* c) {
         for (int64_t p = 0; p < bh->np; p += 256 * kScatterPer) pblocks.push_back(make_int2(bi, (int)p));
         for (int k = 0; k < bh->nnode; ++k) pblocks.push_back(make_int2(bi, -(k + 1)));
     }
-    while (pitems.size() % 4) pitems.push_back(make_int2(-1, 0));   // whole 4-wave push/count blocks
+    while (pitems.size() % kPushWaves) pitems.push_back(make_int2(-1, 0));   // whole push/count workgroups
     c->has_pic = !bdev.empty();
     c->npblocks = (int)pblocks.size();
     c->npitems = (int)pitems.size();

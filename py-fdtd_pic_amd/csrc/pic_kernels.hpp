@@ -332,10 +332,9 @@ __device__ __forceinline__ void push_tab(const PushArgs<R>& pa, const R* Et, con
     push_at<R>(pa, c, f, ux, uy, uz, Eg, Hg, d, off, a0, a1);
 }
 
-// kPushWaves independent waves per workgroup (a CU holds a limited number of
-// workgroups; single-wave workgroups would cap the resident waves), each with
-// its own LDS tables; item 4*block + wave, items with x < 0 are padding.
-constexpr int kPushWaves = 4;
+// kPushWaves (device_common.hpp) independent waves per workgroup, each with
+// its own LDS tables; item kPushWaves * block + wave, items with x < 0 are
+// padding.
 
 template <typename R>
 __global__ __launch_bounds__(64 * kPushWaves) void pic_push_count(const SimDev<R>* __restrict__ sims,
