@@ -246,20 +246,47 @@ def results(V, P, C_V, C_P, time_Vec, RefCo=False, FFT=False, AnalRefCo=False, a
     return "results ran to end"
 
 
+def _plan_controller(V, P, C_V, C_P):
+    """Controller's dispatch (MasterController.py:451-469) as a plan: the host
+    set-up of its integrator runs now, the device passes later (Solver_Engine.execute)."""
+    probeReadFinishBe = int(P.timeSteps * 0.7)
+    probeReadStartAf = int(P.timeSteps * 0.05)
+    V.x1ColBe = np.zeros(P.timeSteps)
+    V.x1ColAf = np.zeros(P.timeSteps)
+    if P.LorentzMed:
+        return SE.plan_linlor(V, P, C_V, C_P, probeReadFinishBe, probeReadStartAf)
+    if P.FreeSpace:
+        return SE.plan_free_space(V, P, C_V, C_P, probeReadFinishBe, probeReadStartAf)
+    if P.nonLinMed:
+        return SE.plan_nl(V, P, C_V, C_P, probeReadFinishBe, probeReadStartAf)
+    return None
+
+
 def LoopedSim(Rep, V, P, C_V, C_P, MORmode, domainSize, lowLimTim, highLimTim,
               stringparamSweep="Input frequency sweep", loop=False, Low=3e9, Interval=1e8,
-              RefCoBool=True, points=20):
+              RefCoBool=True, points=20, batched=True):
     """MasterController.LoopedSim (MasterController.py:530-606) without plots/video.
 
     loop=True runs the "Input frequency sweep" and returns
-    (freqs, measured reflection, analytical reflection) in addition to the
-    last (V, P, C_V, C_P, Exs, Hys); see fdtd_pic_amd.sweep for the batched form.
+    (freqs, measured reflection, analytical reflection, per-point
+    (V, P, C_V, C_P)) in addition to the last (V, P, C_V, C_P, Exs, Hys).  The reference runs the points one after
+    another (:543-559).  With batched=True (the default) the host set-up of
+    every point runs first, in the reference's order (each point's envSetup
+    reads the previous point's V and P, as there), then all points' passes
+    run on the device together as jobs of one fdtd_pic_amd.sweep.Scheduler
+    (FP64, admitted under its HBM budget), and each point's results are
+    taken from its own V / P.  The analytical reflection is evaluated when
+    its point is planned, at that point's freq_in, as the reference does
+    before it steps freq_in on.  batched=False runs them one at a time.
     """
     Exs = Hys = None
     if loop:
         dataRange = np.arange(Low, points * Interval + Low, Interval)
         ref = np.zeros(points)
         anal = np.zeros(points)
+        if stringparamSweep != "Input frequency sweep":
+            return V, P, C_V, C_P, Exs, Hys, (dataRange, ref, anal, [])
+        plans, objs = [], []
         for k in range(points):
             setup = envDef.envSetup(P.freq_in, domainSize, lowLimTim, highLimTim, VExists=True, V=V, P=P)
             P = Params(*setup, P.MORmode, domainSize, P.freq_in, 20, LorentzMed=P.LorentzMed,
@@ -267,12 +294,33 @@ def LoopedSim(Rep, V, P, C_V, C_P, MORmode, domainSize, lowLimTim, highLimTim,
             V = Variables(P.Nz, P.timeSteps, P.vidInterval, 1)
             C_P = CPML_Params(P.dz)
             C_V = CPML_Variables(P.Nz, P.timeSteps)
-            V, P, C_V, C_P, Exs, Hys = Controller(V, P, C_V, C_P)
-            t = np.arange(0, len(V.x1ColBe)) * P.delT
-            ref[k] = results(V, P, C_V, C_P, t, RefCo=True)
-            anal[k] = results(V, P, C_V, C_P, t, AnalRefCo=True)
+            if batched:
+                plan = _plan_controller(V, P, C_V, C_P)
+                plans.append(plan)
+                objs.append((V, P, C_V, C_P))
+                anal[k] = results(V, P, C_V, C_P, None, AnalRefCo=True)
+            else:
+                V, P, C_V, C_P, Exs, Hys = Controller(V, P, C_V, C_P)
+                t = np.arange(0, len(V.x1ColBe)) * P.delT
+                ref[k] = results(V, P, C_V, C_P, t, RefCo=True)
+                anal[k] = results(V, P, C_V, C_P, t, AnalRefCo=True)
+                objs.append((V, P, C_V, C_P))
             P.freq_in = P.freq_in + Interval
-        return V, P, C_V, C_P, Exs, Hys, (dataRange, ref, anal)
+        if batched:
+            from fdtd_pic_amd import sweep
+            sch = sweep.Scheduler(SE.PRECISION)
+            try:
+                outs = SE.execute([pl for pl in plans if pl is not None], SE.PRECISION, scheduler=sch)
+            finally:
+                sch.close()
+            it = iter(outs)
+            for k, (pl, (Vk, Pk, Ck, CPk)) in enumerate(zip(plans, objs)):
+                if pl is not None:
+                    Exs, Hys = next(it)[2:4]
+                t = np.arange(0, len(Vk.x1ColBe)) * Pk.delT
+                ref[k] = results(Vk, Pk, Ck, CPk, t, RefCo=True)
+            V, P, C_V, C_P = objs[-1]
+        return V, P, C_V, C_P, Exs, Hys, (dataRange, ref, anal, objs)
     tic = tim.perf_counter()
     V, P, C_V, C_P, Exs, Hys = Controller(V, P, C_V, C_P)
     if Rep is not None:

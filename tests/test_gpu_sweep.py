@@ -159,3 +159,25 @@ def test_served_trace_alone_vs_mix():
     ex = np.abs(jobs[0].X[1] - X1r).max() / np.abs(X0r).max()
     print("served beam job vs oracle", ex, flush=True)
     assert ex <= 1e-4
+
+
+def test_jobs_of_different_dt_finishing_together():
+    """Two jobs with different cell sizes (so different dt) that finish in the
+    same slice each get the DFT at their own dt: bitwise their spectra alone."""
+    from fdtd_pic_amd import configs, sweep
+    ps = []
+    for k, dz in enumerate((configs.DZ, 1.25 * configs.DZ)):
+        p = configs.base(1200, 2400, slab_start=600, slab_stop=800, dz=dz, src_cell=80, probes=[40, 1150])
+        p["src_f0"] = 200e12 / (1.0 + 0.25 * k)
+        ps.append(p)
+    sch = sweep.Scheduler("f32", K=2, S=48, freqs=FREQS)
+    jobs = [sweep.Job(i, p) for i, p in enumerate(ps)]
+    try:
+        sch.run_batch(jobs)
+    finally:
+        sch.close()
+    assert jobs[0].finished == jobs[1].finished
+    for p, j in zip(ps, jobs):
+        a = sweep.run_alone(p, "f32", K=2, S=48, freqs=FREQS)
+        for k in range(4):
+            assert np.array_equal(a.X[k], j.X[k]), (j.id, k)
