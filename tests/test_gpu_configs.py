@@ -1,0 +1,165 @@
+"""The workloads the bench quotes, run on the GPU at their own shapes.
+
+* config 1 (N = 2,000, 10,000 steps, Lorentz slab, FP64): both passes of the
+  two-pass protocol against the FP64 oracle (oracle/physical.py) to 1e-10
+  relative L2 on the probe series and on the fields at step 3,000 (the pulse
+  in the grid); the final fields, a ~1e-11 remnant once the pulse has left,
+  to 1e-10 of the step-3,000 field; the device DFT at 256 frequencies
+  against the oracle's to 1e-10 of the spectrum's peak, R and T to 1e-8 in
+  band.
+* config 4 (all 4,096 sweep points, 1,024 with beams, FP32, autotuned
+  tiling): every point finishes with finite R and T where the incident
+  spectrum exceeds 10 % of its peak; a seeded sample equals the point run
+  alone to 1e-5 (the autotuner may pick another tiling than run_alone's
+  default, so FP32 rounding differs); the three smallest field-only points
+  match the compiled FP64 oracle (oracle/fdtd_cpu.c) to 1e-4 of the
+  spectrum's peak, and in R and T in band.
+* config 5 (the first 8 requests of the seeded Poisson trace, served):
+  every request completes after it arrives; a sampled request is bitwise
+  the same alone; the smallest beam request of the prefix, run alone for its
+  first 300 steps, matches the FP64 particle oracle to 1e-5.
+"""
+import numpy as np
+import pytest
+
+from refcases import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+FREQS = np.linspace(100e12, 300e12, 256)
+
+
+def test_config1_full_fp64():
+    from fdtd_pic_amd import configs, engine, sweep
+    from oracle import physical as ph
+    p = configs.config1()
+    print("config1: N", p["N"], "steps", p["steps"], flush=True)
+    f, R64, T64, s1, p0, p1 = ph.spectra(p, FREQS)
+    mid = 3000                     # the pulse is inside the slab region then
+    for with_slab, probes_ref in ((False, p0), (True, p1)):
+        q = sweep.pass_params(p, with_slab)
+        sm, _ = ph.run(ph.build(q), steps=mid, record=False)
+        with engine.Context("f64") as ctx:
+            sim = ctx.add(engine.physical_spec(q))
+            ctx.run(mid)
+            Em, Hm = sim.read("E"), sim.read("H")
+            ctx.run_all()
+            E, H, pr = sim.read("E"), sim.read("H"), sim.probes()
+        assert rel_l2(pr, probes_ref) <= 1e-10
+        assert rel_l2(Em, sm.E) <= 1e-10 and rel_l2(Hm, sm.H) <= 1e-10
+        if with_slab:
+            # by step 10,000 the pulse has left: E is a ~1e-11 remnant, compared
+            # at the scale of the field it came from
+            scale = np.linalg.norm(sm.E)
+            assert np.linalg.norm(E - s1.E) <= 1e-10 * scale and np.linalg.norm(H - s1.H) <= 1e-10 * scale
+    job = sweep.run_alone(p, "f64", freqs=FREQS)
+    X0r, X1r, X0t, X1t = job.X
+    m = ph.build(p)
+    Xo = [ph.dft(p0, m["dt"], FREQS), ph.dft(p1, m["dt"], FREQS)]
+    for got, want in ((X0r, Xo[0][0]), (X1r, Xo[1][0]), (X0t, Xo[0][1]), (X1t, Xo[1][1])):
+        assert np.abs(got - want).max() <= 1e-10 * np.abs(want).max()
+    # in band (incident spectrum above 10 % of its peak at that probe) the
+    # X bound gives |dR| <= (2 + R) 1e-10 / 0.1 ~ 3e-9
+    br, bt = _in_band(Xo[0][0]), _in_band(Xo[0][1])
+    assert br.sum() > 50 and bt.sum() > 50
+    assert np.abs(job.R - R64)[br].max() <= 1e-8 and np.abs(job.T - T64)[bt].max() <= 1e-8
+
+
+@pytest.fixture(scope="module")
+def config4_batch():
+    from fdtd_pic_amd import configs, sweep
+    inst = configs.config4_instances()
+    jobs = [sweep.Job(i, p) for i, p in enumerate(inst)]
+    sch = sweep.Scheduler("f32", budget_bytes=200 << 30, autotune=True, freqs=FREQS)
+    try:
+        sch.run_batch(jobs)
+        tiling = (sch.K, sch.S)
+    finally:
+        sch.close()
+    print("config4: batched", len(jobs), "jobs, tiling", tiling, flush=True)
+    return inst, jobs
+
+
+def _in_band(X0):
+    return np.abs(X0) > 0.1 * np.abs(X0).max()
+
+
+def test_config4_every_point_finite(config4_batch):
+    inst, jobs = config4_batch
+    assert len(jobs) == 4096 and sum(1 for p in inst if p.get("beam")) == 1024
+    for j in jobs:
+        assert j.finished is not None, j.id
+        br, bt = _in_band(j.X[0]), _in_band(j.X[2])
+        assert br.any() and bt.any(), j.id
+        assert np.isfinite(j.R[br]).all() and np.isfinite(j.T[bt]).all(), j.id
+
+
+def test_config4_sample_equals_alone(config4_batch):
+    from fdtd_pic_amd import sweep
+    inst, jobs = config4_batch
+    rng = np.random.default_rng(404)
+    field = [i for i, p in enumerate(inst) if not p.get("beam")]
+    beam = [i for i, p in enumerate(inst) if p.get("beam")]
+    pick = sorted(rng.choice(field, 3, replace=False).tolist() + rng.choice(beam, 2, replace=False).tolist())
+    for i in pick:
+        a = sweep.run_alone(inst[i], "f32", freqs=FREQS)
+        for k in range(4):
+            err = np.abs(a.X[k] - jobs[i].X[k]).max() / np.abs(a.X[k]).max()
+            print(f"config4 job {i} beam={bool(inst[i].get('beam'))} X{k} alone vs batch {err:.2e}", flush=True)
+            assert err <= 1e-5, (i, k)
+
+
+def test_config4_smallest_field_points_vs_compiled_oracle(config4_batch):
+    from fdtd_pic_amd import sweep
+    from oracle import cpu_ref, physical as ph
+    inst, jobs = config4_batch
+    small = sorted((p["N"], i) for i, p in enumerate(inst) if not p.get("beam"))[:3]
+    for _, i in small:
+        X = []
+        for with_slab in (False, True):
+            m = ph.build(sweep.pass_params(inst[i], with_slab))
+            sim, _ = cpu_ref.run_model(m, threads=4)
+            X.append(ph.dft(sim.probe_out, m["dt"], FREQS))
+        want = (X[0][0], X[1][0], X[0][1], X[1][1])
+        for k in range(4):
+            err = np.abs(jobs[i].X[k] - want[k]).max() / np.abs(want[k]).max()
+            print(f"config4 job {i} N={inst[i]['N']} chi3={inst[i]['chi3']:.2e} X{k} vs oracle {err:.2e}",
+                  flush=True)
+            assert err <= 1e-4, (i, k)
+        br, bt = _in_band(want[0]), _in_band(want[2])
+        R64 = np.abs(want[1] - want[0]) / np.abs(want[0])
+        T64 = np.abs(want[3]) / np.abs(want[2])
+        assert np.abs(jobs[i].R - R64)[br].max() <= 1e-4 and np.abs(jobs[i].T - T64)[bt].max() <= 1e-4
+
+
+def test_config5_trace_prefix():
+    from fdtd_pic_amd import configs, engine, serve, sweep
+    from oracle import physical as ph, pic as P
+    arr, ps = configs.config5_requests()
+    n = 8
+    srv = serve.Server("f32", budget_bytes=200 << 30, freqs=FREQS)
+    try:
+        jobs, wall = srv.replay(arr[:n], ps[:n])
+    finally:
+        srv.close()
+    print("config5 prefix:", serve.summarize(jobs, wall), flush=True)
+    assert all(j.finished is not None and j.finished >= j.arrival for j in jobs)
+    # the smallest request of the prefix, alone: bitwise the served result
+    k = min(range(n), key=lambda i: serve.work(ps[i])[0] + 4 * serve.work(ps[i])[1])
+    a = sweep.run_alone(ps[k], "f32", freqs=FREQS)
+    for q in range(4):
+        assert np.array_equal(a.X[q], jobs[k].X[q]), (k, q)
+    # that request's first 300 steps against the FP64 particle oracle
+    p = dict(ps[k])
+    p["steps"] = 300
+    m = ph.build(p)
+    x, u, qq, mm, w = configs.beam_inputs(p)
+    b = P.Beam(x, u, qq, mm, w, m["dt"], p["N"], p["Sc"])
+    s, probes = P.run(m, b)
+    with engine.Context("f32") as ctx:
+        sim = ctx.add(engine.physical_spec(p), beam=engine.Beam(x, u, qq, mm, w, m["dt"], p["dz"]))
+        ctx.run_all()
+        E, xo, uo = sim.read("E"), *sim.particles()[:2]
+    errs = (rel_l2(E, s.E), rel_l2(xo - x, b.x - x), rel_l2(uo, b.u))
+    print("config5 request", k, "Np", p["beam"]["n"], "E, dx, u rel L2", errs, flush=True)
+    assert max(errs) <= 1e-5
