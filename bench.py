@@ -42,6 +42,9 @@ def parse():
     ap.add_argument("--K", type=int, default=0)
     ap.add_argument("--S", type=int, default=0)
     ap.add_argument("--K-slab", type=int, default=0, help="cells per lane of slab tiles (K or K/2; 0: default)")
+    ap.add_argument("--persistent", default="off", choices=["on", "off"],
+                    help="config 2: persistent field launches (many blocks of S steps per launch; measured "
+                         "slower than one-block launches at K=2 S=48, faster at K=1 and K=4 shapes)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU baseline sample length")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-pic", action="store_true", help="skip the config 3 (PIC) measurement")
@@ -298,12 +301,15 @@ def main():
 
     p = configs.config2(steps=a.sim_steps)
     ctx = engine.Context(a.precision)
-    K0, S0, Ks0 = configs.TILING_CONFIG2[a.precision]
+    persistent = a.persistent == "on"
+    K0, S0, Ks0 = (configs.TILING_CONFIG2_PERSISTENT if persistent else configs.TILING_CONFIG2)[a.precision]
     K = a.K or K0
     Ks = a.K_slab or (Ks0 if K == K0 else K)
     S = a.S or (S0 if K == K0 else 8 * Ks)
     ctx.set_tiling(K, S, Ks)
+    ctx.set_persistent(persistent)
     sim = ctx.add(engine.physical_spec(p))
+    persistent = ctx.persistent_active()
     for _ in range(a.warmup):
         sim.reset()
         ctx.run_all()
@@ -337,7 +343,8 @@ def main():
         "dtype": "fp32+fp64-kerr" if a.precision == "f32" else "f64",
         "data": "synthetic (seeded config 2 parameters)",
         "config": {"workload": "config2: N=65536 cells, 50000 steps, Lorentz+Kerr slab [20000,40000)",
-                   "sim_steps": p["steps"], "K": K, "K_slab": Ks, "S": S, "precision": a.precision},
+                   "sim_steps": p["steps"], "K": K, "K_slab": Ks, "S": S, "precision": a.precision,
+                   "persistent": persistent},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(a.precision, K, S, Ks),

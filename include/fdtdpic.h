@@ -18,9 +18,10 @@
  *     Python layer.
  *   - Host arrays are plain pointers with explicit lengths; values are FP64 on
  *     the host side whatever the device precision.
- *   - A context owns a compute stream and an upload stream; fdtd_run is
- *     asynchronous, fdtd_sync waits.  fdtd_add_sim and pic_attach return
- *     once their uploads are done, without waiting for running launches.
+ *   - A context owns a compute stream; the contexts of one device share one
+ *     upload stream.  fdtd_run is asynchronous, fdtd_sync waits.
+ *     fdtd_add_sim and pic_attach return once their uploads are done,
+ *     without waiting for running launches.
  *     Contexts are not thread-safe; use one per host thread.
  */
 #ifndef FDTDPIC_H
@@ -131,6 +132,22 @@ int fdtd_set_tiling_mixed(fdtd_ctx *ctx, int K, int K_slab, int S);
  * counterpart in the reference (which has no particles); it selects how
  * this library schedules its own kernels. */
 int fdtd_set_pic_fusion(fdtd_ctx *ctx, int on);
+
+/* Persistent field launches (on = 1): when the context's tiles all fit on
+ * the device at once (one slab kind, uniform tiles, S <= 64K - 2S, no beam),
+ * one launch runs up to 4,096 blocks of S steps.  Each tile keeps its window
+ * in registers for the whole launch; between blocks it writes its interior,
+ * publishes a per-tile flag, waits for its two neighbour tiles to reach the
+ * same block, and reloads only its halo cells.  The waits are bounded (2 s);
+ * one that times out makes fdtd_sync and the reads fail with FDTD_E_STATE.
+ * Otherwise, and for the steps left over, launches run one block each.
+ * Results are bitwise those of one-block launches of the same (K, S).  No
+ * counterpart in the reference; it replaces the per-S-steps relaunch of this
+ * library's own loop (the `for counts in range(P.timeSteps)` bodies of
+ * Solver_Engine.py:167,236,294). */
+int fdtd_set_persistent(fdtd_ctx *ctx, int on);
+/* *active = 1 when the next fdtd_run uses persistent launches. */
+int fdtd_persistent_active(fdtd_ctx *ctx, int *active);
 
 /* ---- simulations ------------------------------------------------------ */
 /* Uploads coefficients, allocates zeroed state (FieldInit + CPML_FieldInit,

@@ -37,6 +37,25 @@ struct SimDev {
 
 enum SlabKind { SLAB_NONE = 0, SLAB_DIEL = 1, SLAB_LINEAR = 2, SLAB_KERR_REF = 3, SLAB_KERR_E = 4 };
 
+// A field launch that runs nblocks blocks of S steps instead of one.  Every
+// tile stays resident for the whole launch with its window in registers:
+// after a block it writes its interior (as a one-block launch does) and
+// publishes flags[tile] = the global index of the block it finished; before
+// the next block it waits until both neighbour tiles (nbr: left, right; -1
+// none) have finished the same block, then reloads only its halo cells, which
+// lie in their interiors (the host guarantees S <= U, uniform tiles, and no
+// more tiles than the device holds at once).  A wait that passes
+// spin_ticks of the 100 MHz real-time counter sets *err and ends the tile
+// (a bounded wait: the launch always drains).  nblocks = 1 and flags = null
+// is the plain one-block launch.
+struct PersistArgs {
+    int nblocks;
+    unsigned long long* flags;
+    const int2* nbr;
+    int32_t* err;
+    long long spin_ticks;
+};
+
 // Particle beam of one simulation.  Particles are kept sorted by cell
 // (stable), structure of arrays, ping-ponged per step like the fields;
 // start[c] is the first particle of cell c (start[N+1] = np).  Live

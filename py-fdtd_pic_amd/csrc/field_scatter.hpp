@@ -32,7 +32,8 @@ __global__ __launch_bounds__(256) void field_scatter(const SimDev<R>* __restrict
     // wave-uniform tile index, so the descriptors come in through scalar loads
     const int tid = kFusedTileWaves * b + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     if (tid >= ntiles) return;
-    field_tile<R, KW, KAHAN, KIND>(sims, tiles, tid, launch_idx, gstep0, nsteps, S);
+    const PersistArgs one{1, nullptr, nullptr, nullptr, 0};
+    field_tile<R, KW, KAHAN, KIND>(sims, tiles, tid, launch_idx, gstep0, nsteps, S, one);
 }
 
 template <typename R, int K, bool KAHAN, int KIND>

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -15,8 +16,10 @@
 namespace fdtd {
 template <typename R, bool KAHAN>
 hipError_t launch_stencil(const SimDev<R>* sims, const int2* tiles, int ntiles, long long launch_idx,
-                          long long gstep0, int nsteps, int K, int S, int kind, hipStream_t st,
-                          hipEvent_t ev0, hipEvent_t ev1);
+                          long long gstep0, int nsteps, int K, int S, int kind, const PersistArgs& pa,
+                          hipStream_t st, hipEvent_t ev0, hipEvent_t ev1);
+template <typename R, bool KAHAN>
+int field_capacity_per_cu(int K, int kind);
 struct OpArgs {
     double* a[8];
     double s[4];
@@ -200,6 +203,30 @@ static int validate(const fdtd_sim_desc_t* q, int prec) {
 }
 
 // ------------------------------------------------------------------------
+// One upload stream per device, shared by every context on it (created on
+// first use, kept until the process ends).  A batch of two contexts then runs
+// three streams (two compute, one upload), which HIP's default of four
+// hardware queues per process gives a queue each, so an upload never waits
+// behind another stream's running launches on a shared queue.
+// ------------------------------------------------------------------------
+constexpr int kMaxDevices = 64;
+static std::mutex g_dev_mu;
+static hipStream_t g_upload[kMaxDevices] = {nullptr};
+
+static int shared_upload_stream(int dev, hipStream_t* out) {
+    if (dev < 0 || dev >= kMaxDevices) return fdtd_fail(FDTD_E_ARG, "device index beyond the library's table");
+    std::lock_guard<std::mutex> lk(g_dev_mu);
+    if (!g_upload[dev]) {
+        hipStream_t st = nullptr;
+        const hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+        if (e != hipSuccess) return fdtd_fail(FDTD_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+        g_upload[dev] = st;
+    }
+    *out = g_upload[dev];
+    return FDTD_OK;
+}
+
+// ------------------------------------------------------------------------
 // C ABI
 // ------------------------------------------------------------------------
 extern "C" {
@@ -213,25 +240,32 @@ int fdtd_ctx_create(int precision, fdtd_ctx** out) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return fdtd_fail(FDTD_E_HIP, "no HIP device visible");
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return fdtd_fail(FDTD_E_HIP, "hipGetDevice failed");
     fdtd_ctx* c = new fdtd_ctx();
     c->prec = precision;
+    c->device = dev;
     c->K = c->K_slab = precision == FDTD_F64 ? 2 : 4;
     c->S = default_S(c->K);
     if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return fdtd_fail(FDTD_E_HIP, "hipStreamCreate failed");
     }
-    if (hipStreamCreateWithFlags(&c->up, hipStreamNonBlocking) != hipSuccess) {
+    if (int rc = shared_upload_stream(dev, &c->up)) {
         (void)hipStreamDestroy(c->st);
         delete c;
-        return fdtd_fail(FDTD_E_HIP, "hipStreamCreate failed");
+        return rc;
     }
-    if (hipMalloc(&c->d_err, sizeof(int32_t)) != hipSuccess || hipMemset(c->d_err, 0, sizeof(int32_t)) != hipSuccess) {
-        (void)hipStreamDestroy(c->up);
+    if (hipMalloc(&c->d_err, sizeof(int32_t)) != hipSuccess || hipMemset(c->d_err, 0, sizeof(int32_t)) != hipSuccess ||
+        hipMalloc(&c->d_perr, sizeof(int32_t)) != hipSuccess || hipMemset(c->d_perr, 0, sizeof(int32_t)) != hipSuccess) {
+        if (c->d_err) (void)hipFree(c->d_err);
+        if (c->d_perr) (void)hipFree(c->d_perr);
         (void)hipStreamDestroy(c->st);
         delete c;
-        return fdtd_fail(FDTD_E_HIP, "allocation of the error flag failed");
+        return fdtd_fail(FDTD_E_HIP, "allocation of the error flags failed");
     }
+    if (hipDeviceGetAttribute(&c->cu_count, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        c->cu_count = 0;
     *out = c;
     return FDTD_OK;
 }
@@ -245,7 +279,8 @@ int fdtd_ctx_destroy(fdtd_ctx* c) {
         if (s) { if (s->block) (void)hipFree(s->block); delete s; }
     for (BeamHost* b : c->beams)
         if (b) { if (b->block) (void)hipFree(b->block); delete b; }
-    for (void* p : {(void*)c->d_beams, (void*)c->d_pblocks, (void*)c->d_pitems, (void*)c->d_err, c->d_scratch})
+    for (void* p : {(void*)c->d_beams, (void*)c->d_pblocks, (void*)c->d_pitems, (void*)c->d_err, c->d_scratch,
+                    (void*)c->d_perr, (void*)c->d_flags, (void*)c->d_nbr})
         if (p) (void)hipFree(p);
     if (c->d_sims) (void)hipFree(c->d_sims);
     if (c->d_tiles) (void)hipFree(c->d_tiles);
@@ -253,8 +288,7 @@ int fdtd_ctx_destroy(fdtd_ctx* c) {
     for (hipEvent_t e : c->throttle) (void)hipEventDestroy(e);
     if (c->stage) (void)hipHostFree(c->stage);
     if (c->stage_ev) (void)hipEventDestroy(c->stage_ev);
-    (void)hipStreamDestroy(c->up);
-    (void)hipStreamDestroy(c->st);
+    (void)hipStreamDestroy(c->st);        // c->up is the device's shared upload stream
     delete c;
     return FDTD_OK;
 }
@@ -650,6 +684,45 @@ static int rebuild_t(fdtd_ctx* c) {
     }
     ups.push_back({&c->d_sims, &c->d_sims_cap, devs.data(), sizeof(SimDev<R>) * devs.size()});
     ups.push_back({(void**)&c->d_tiles, &c->d_tiles_cap, tiles.data(), sizeof(int2) * tiles.size()});
+    // persistent launches: one tile group of uniform tiles whose halos lie in
+    // the neighbouring interiors (S <= U), no beam, and every tile resident at
+    // once; the neighbour table maps each tile to the tiles on either side
+    c->persist_ok = false;
+    std::vector<int2> nbr;
+    if (c->persist && !pic && c->groups.size() == 1 && c->K_slab == c->K && c->S_run <= U && c->cu_count > 0) {
+        const int per_cu = field_capacity_per_cu<R, sizeof(R) == 4>(c->K, c->groups[0].kind);
+        const long long wgs = ((long long)tiles.size() + 3) / 4;
+        if (per_cu > 0 && wgs <= (long long)per_cu * c->cu_count) {
+            std::vector<std::pair<long long, int>> key(tiles.size());
+            for (size_t t = 0; t < tiles.size(); ++t)
+                key[t] = {((long long)(tiles[t].x & 0xffffff) << 32) | (unsigned)tiles[t].y, (int)t};
+            std::sort(key.begin(), key.end());
+            auto find = [&](int sim, long long o) {
+                const long long k = ((long long)sim << 32) | (unsigned)o;
+                auto it = std::lower_bound(key.begin(), key.end(), std::make_pair(k, INT32_MIN));
+                return (it != key.end() && it->first == k) ? it->second : -1;
+            };
+            nbr.resize(tiles.size());
+            for (size_t t = 0; t < tiles.size(); ++t) {
+                const int sim = tiles[t].x & 0xffffff, o = tiles[t].y;
+                nbr[t] = make_int2(o > 0 ? find(sim, (long long)o - U) : -1,
+                                   o + U <= c->sims[sim]->d.N ? find(sim, (long long)o + U) : -1);
+            }
+            ups.push_back({(void**)&c->d_nbr, &c->d_nbr_cap, nbr.data(), sizeof(int2) * nbr.size()});
+            if (sizeof(unsigned long long) * tiles.size() > c->d_flags_cap) {
+                // flags only grow: a fresh buffer starts at zero, below every block index
+                HIPCHK(hipStreamSynchronize(c->st));
+                if (c->d_flags) HIPCHK(hipFree(c->d_flags));
+                c->d_flags = nullptr;
+                c->d_flags_cap = 0;
+                const size_t bytes = sizeof(unsigned long long) * tiles.size() * 2;
+                HIPCHK(hipMalloc(&c->d_flags, bytes));
+                HIPCHK(hipMemsetAsync(c->d_flags, 0, bytes, c->st));
+                c->d_flags_cap = bytes;
+            }
+            c->persist_ok = true;
+        }
+    }
     int rc = staged_upload(c, ups);
     if (rc) return rc;
     c->ntiles = (int)tiles.size();
@@ -694,7 +767,9 @@ static EvPair* next_event(fdtd_ctx* c, int kind) {
 // kernel, ev->second at the end of the last one.  With fused, the last group
 // runs in one launch with the particle scatter (stamped by fev instead).
 static int launch_once(fdtd_ctx* c, int n, EvPair* ev, hipStream_t st, bool fused = false,
-                       EvPair* fev = nullptr) {
+                       EvPair* fev = nullptr, int nblocks = 1) {
+    PersistArgs pa{1, nullptr, nullptr, nullptr, 0};
+    if (nblocks > 1) pa = PersistArgs{nblocks, c->d_flags, c->d_nbr, c->d_perr, kPersistSpinTicks};
     const size_t ng = c->groups.size();
     const size_t nplain = fused ? ng - 1 : ng;
     for (size_t gi = 0; gi < ng; ++gi) {
@@ -705,10 +780,10 @@ static int launch_once(fdtd_ctx* c, int n, EvPair* ev, hipStream_t st, bool fuse
             hipEvent_t e1 = (ev && gi + 1 == nplain) ? ev->second : nullptr;
             if (c->prec == FDTD_F64)
                 e = launch_stencil<double, false>((const SimDev<double>*)c->d_sims, c->d_tiles + g.offset, g.count,
-                                                  c->launch_idx, c->gstep, n, c->K, c->S_run, g.kind, st, e0, e1);
+                                                  c->launch_idx, c->gstep, n, c->K, c->S_run, g.kind, pa, st, e0, e1);
             else
                 e = launch_stencil<float, true>((const SimDev<float>*)c->d_sims, c->d_tiles + g.offset, g.count,
-                                                c->launch_idx, c->gstep, n, c->K, c->S_run, g.kind, st, e0, e1);
+                                                c->launch_idx, c->gstep, n, c->K, c->S_run, g.kind, pa, st, e0, e1);
         } else {
             hipEvent_t e0 = fev ? fev->first : nullptr, e1 = fev ? fev->second : nullptr;
             if (c->prec == FDTD_F64)
@@ -755,6 +830,40 @@ int fdtd_run(fdtd_ctx* c, int64_t nsteps) {
             if (rc) return rc;
         }
         if (c->ntiles == 0) break;
+        if (c->persist_ok && remaining >= 2 * (int64_t)c->S_run) {
+            // one persistent launch of nb blocks of S steps (see PersistArgs)
+            const int nb = (int)std::min<int64_t>(remaining / c->S_run, kPersistMaxBlocks);
+            int rc = close_bracket(c);
+            if (rc) return rc;
+            hipEvent_t pe0 = nullptr, pe1 = nullptr;
+            if (c->profile > 0) {
+                EvPair* ev = next_event(c, FDTD_KERNEL_STENCIL);
+                if (!ev) return FDTD_E_HIP;
+                ev->launches = nb;      // the timing is per block
+                pe0 = ev->first;
+                pe1 = ev->second;
+                HIPCHK(hipEventRecord(pe0, c->st));
+            }
+            rc = launch_once(c, c->S_run, nullptr, c->st, false, nullptr, nb);
+            if (rc) return rc;
+            if (pe1) HIPCHK(hipEventRecord(pe1, c->st));
+            rc = throttle(c);
+            if (rc) return rc;
+            c->launches += nb;
+            bool finished = false;
+            for (int i : c->tile_sims) {
+                SimHost* s = c->sims[i];
+                s->launches += nb;
+                const int64_t done = std::min<int64_t>(s->d.steps, c->gstep + (int64_t)nb * c->S_run - s->t_begin);
+                s->steps_done = std::max<int64_t>(s->steps_done, done);
+                if (s->steps_done >= s->d.steps) finished = true;
+            }
+            c->launch_idx += nb;
+            c->gstep += (int64_t)nb * c->S_run;
+            remaining -= (int64_t)nb * c->S_run;
+            if (finished) c->dirty = true;
+            continue;
+        }
         // steps this launch: S, bounded by the remaining request; one step when
         // a beam is coupled (the deposit feeds every E update)
         int n = (int)std::min<int64_t>(remaining, c->S_run);
@@ -864,10 +973,44 @@ static int check_pic_flag(fdtd_ctx* c) {
     return FDTD_OK;
 }
 
+// A persistent field launch whose neighbour wait timed out (the flag is
+// cleared once reported).  Call after the stream has drained.
+static int check_persist_flag(fdtd_ctx* c) {
+    if (!c->d_perr || !c->persist) return FDTD_OK;
+    int32_t e = 0;
+    HIPCHK(hipMemcpy(&e, c->d_perr, sizeof(e), hipMemcpyDeviceToHost));
+    if (!e) return FDTD_OK;
+    HIPCHK(hipMemset(c->d_perr, 0, sizeof(int32_t)));
+    return fdtd_fail(FDTD_E_STATE, "persistent field launch: a neighbour wait timed out (tiles not all resident "
+                                   "at once?); the fields of that run are invalid");
+}
+
 int fdtd_sync(fdtd_ctx* c) {
     if (!c) return fdtd_fail(FDTD_E_ARG, "null ctx");
     HIPCHK(hipStreamSynchronize(c->st));
+    int rc = check_persist_flag(c);
+    if (rc) return rc;
     return c->has_pic ? check_pic_flag(c) : FDTD_OK;
+}
+
+int fdtd_set_persistent(fdtd_ctx* c, int on) {
+    if (!c) return fdtd_fail(FDTD_E_ARG, "null ctx");
+    if (on != 0 && on != 1) return fdtd_fail(FDTD_E_ARG, "persistent flag must be 0 or 1");
+    c->persist = on != 0;
+    c->dirty = true;
+    return FDTD_OK;
+}
+
+int fdtd_persistent_active(fdtd_ctx* c, int* active) {
+    if (!c || !active) return fdtd_fail(FDTD_E_ARG, "null argument");
+    if (c->dirty) {
+        int rc = close_bracket(c);
+        if (rc) return rc;
+        rc = c->prec == FDTD_F64 ? rebuild_t<double>(c) : rebuild_t<float>(c);
+        if (rc) return rc;
+    }
+    *active = c->persist_ok ? 1 : 0;
+    return FDTD_OK;
 }
 
 int fdtd_busy(fdtd_ctx* c, int* busy) {
@@ -973,6 +1116,7 @@ static int xfer_field(fdtd_ctx* c, SimHost& s, int field, double* out, const dou
     if (n == 0) return FDTD_OK;
     std::vector<R> h(n);
     HIPCHK(hipStreamSynchronize(c->st));
+    if (int rc = check_persist_flag(c)) return rc;
     if (out) {
         HIPCHK(hipMemcpy(h.data(), src, sizeof(R) * n, hipMemcpyDeviceToHost));
         if (!strip) {
@@ -1011,6 +1155,7 @@ template <typename R>
 static int read_plain(fdtd_ctx* c, const R* src, int64_t n, double* out) {
     std::vector<R> h(n);
     HIPCHK(hipStreamSynchronize(c->st));
+    if (int rc = check_persist_flag(c)) return rc;
     if (n) HIPCHK(hipMemcpy(h.data(), src, sizeof(R) * n, hipMemcpyDeviceToHost));
     for (int64_t i = 0; i < n; ++i) out[i] = (double)h[i];
     return FDTD_OK;

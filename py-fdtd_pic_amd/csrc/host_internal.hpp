@@ -50,8 +50,10 @@ struct TileGroup {
 
 struct fdtd_ctx {
     int prec = FDTD_F64;
+    int device = 0;                          // HIP device current at fdtd_ctx_create
     hipStream_t st = nullptr;
-    hipStream_t up = nullptr;                // uploads of new simulations and beams (never waits on st)
+    hipStream_t up = nullptr;                // uploads of new simulations and beams (never waits on st);
+                                             // one per device, shared by its contexts
     // pinned staging of the descriptor / tile / work-list uploads; reused
     // once the event recorded after the last upload has completed
     void* stage = nullptr;
@@ -94,6 +96,15 @@ struct fdtd_ctx {
     size_t throttle_head = 0;
     long long throttle_launches = 0;
     int32_t* d_err = nullptr;                // index-check flag of the sort kernel
+    // persistent field launches (fdtd_set_persistent)
+    bool persist = false;                    // requested
+    bool persist_ok = false;                 // the current tile list qualifies
+    int cu_count = 0;
+    unsigned long long* d_flags = nullptr;   // per tile: last block finished (global launch index + 1)
+    size_t d_flags_cap = 0;
+    int2* d_nbr = nullptr;                   // per tile: left, right neighbour tile
+    size_t d_nbr_cap = 0;
+    int32_t* d_perr = nullptr;               // set by a tile whose neighbour wait timed out
     void* d_scratch = nullptr;               // spectra: pointer table, frequencies, results
     size_t d_scratch_cap = 0;
 };
@@ -102,6 +113,12 @@ int fdtd_fail(int code, const std::string& msg);
 
 // Largest field launch (tiles) that takes the particle scatter into its grid.
 constexpr int kFuseMaxTiles = 2048;
+
+// Blocks of S steps in one persistent field launch (bounded so that a long
+// run still returns control to the host now and then), and the bound on one
+// neighbour wait inside it (2 s of the 100 MHz real-time counter).
+constexpr int kPersistMaxBlocks = 4096;
+constexpr long long kPersistSpinTicks = 200000000LL;
 
 constexpr int kThrottleEvery = 128;
 constexpr size_t kThrottleDepth = 4;

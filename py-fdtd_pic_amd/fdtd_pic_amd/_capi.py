@@ -68,6 +68,8 @@ SIGNATURES = {
     "fdtd_set_tiling": (C.c_int, [_vp, C.c_int, C.c_int]),
     "fdtd_set_tiling_mixed": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     "fdtd_set_pic_fusion": (C.c_int, [_vp, C.c_int]),
+    "fdtd_set_persistent": (C.c_int, [_vp, C.c_int]),
+    "fdtd_persistent_active": (C.c_int, [_vp, C.POINTER(C.c_int)]),
     "fdtd_add_sim": (C.c_int, [_vp, C.POINTER(SimDesc), C.POINTER(C.c_int)]),
     "fdtd_remove_sim": (C.c_int, [_vp, C.c_int]),
     "fdtd_sim_progress": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int64)]),
@@ -95,16 +97,6 @@ SIGNATURES = {
 
 _lib = None
 
-# Hardware queues the HIP runtime may use in this process.  A context has a
-# compute and an upload stream, and a batch runs two contexts; with fewer
-# queues than streams, two streams share a queue, and a beam's upload then
-# waits behind the running slice on the other one (config 4: 26.8 s of
-# admission in a 35.6 s batch with 4 queues; 6.5 s in 28.4 s with 8).  Read
-# when the runtime starts, so it only takes effect if nothing in the process
-# has used HIP before the library loads.
-MIN_HW_QUEUES = 8
-
-
 def load(path=None):
     """Load libfdtdpic.so and bind every declared symbol (raises if missing)."""
     global _lib
@@ -113,9 +105,6 @@ def load(path=None):
     p = path or LIB_PATH
     if not os.path.exists(p):
         raise FdtdError(FDTD_E_STATE, f"{p} is not built; run __graft_entry__.build()")
-    q = os.environ.get("GPU_MAX_HW_QUEUES", "")
-    if not q.isdigit() or int(q) < MIN_HW_QUEUES:
-        os.environ["GPU_MAX_HW_QUEUES"] = str(MIN_HW_QUEUES)
     lib = C.CDLL(p)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

@@ -37,6 +37,9 @@ M_ELECTRON = 9.1093837139e-31      # CODATA 2022
 # cells per lane of plain tiles, steps per launch, cells per lane of tiles
 # that meet the slab.
 TILING_CONFIG2 = {"f32": (2, 48, 2), "f64": (2, 48, 2)}
+# With persistent launches a tile's halo must lie in its neighbours'
+# interiors (S <= 64K - 2S): at K = 2, S <= 42.
+TILING_CONFIG2_PERSISTENT = {"f32": (2, 42, 2), "f64": (2, 42, 2)}
 
 
 def base(N, steps, **kw):

@@ -170,6 +170,17 @@ class Context:
         launch is small (default on; bitwise the same results either way)."""
         check(self.lib.fdtd_set_pic_fusion(self.h, 1 if on else 0))
 
+    def set_persistent(self, on=True):
+        """Persistent field launches: many blocks of S steps per launch when
+        every tile fits on the device at once (bitwise the same results)."""
+        check(self.lib.fdtd_set_persistent(self.h, 1 if on else 0))
+
+    def persistent_active(self):
+        """True when the next run() uses persistent launches."""
+        v = C.c_int()
+        check(self.lib.fdtd_persistent_active(self.h, C.byref(v)))
+        return bool(v.value)
+
     def add(self, spec, beam=None):
         """Add a simulation; beam (optional) is a Beam whose current couples to it."""
         sid = C.c_int()
