@@ -210,6 +210,22 @@ int fdtd_read_probes(fdtd_ctx *ctx, int sim_id, double *out, int64_t len);
 /* snapshots as snap_rows x (N+1) (Ex_History, MasterController.py:160) */
 int fdtd_read_snapshots(fdtd_ctx *ctx, int sim_id, double *out, int64_t len);
 
+/* ---- energy ----------------------------------------------------------- */
+/* FP64 discrete field energy of one simulation, taken on the device after
+ * every `every`-th launch in which it steps (every = 0: off), up to `rows`
+ * values:  W = sum_n E_n^2 + sum_n H'_n H'prev_n  (E and H' = Z0 Hy after the
+ * launch's last step, H'prev before that step's H update; the leapfrog
+ * invariant of a closed lossless cavity, conserved to rounding).  FP32
+ * fields enter through their Kahan compensations; every term and the sum
+ * are FP64, in a fixed order.  The reference has no accumulator: its
+ * per-step observables are the probes of Solver_Engine.py:360-370.  A
+ * simulation with energy on runs the general tile class and never a
+ * persistent launch.  Setting it again clears the rows. */
+int fdtd_set_energy(fdtd_ctx *ctx, int sim_id, int every, int rows);
+/* Copies up to cap recorded values and the step index after which each was
+ * taken; *count = rows recorded so far. */
+int fdtd_read_energy(fdtd_ctx *ctx, int sim_id, double *values, int64_t *steps, int64_t cap, int64_t *count);
+
 /* ---- spectra ---------------------------------------------------------- */
 /* FP64 DFT X(f) = sum_t x_t exp(-2 pi i f t dt) dt of probe `probe` of each
  * listed simulation, computed on the device from the resident probe series

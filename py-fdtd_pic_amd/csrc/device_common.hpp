@@ -28,11 +28,23 @@ struct SimDev {
     const R *srcE, *srcH;
     R *probe_out;                               // nprobe x T
     R *snap;                                    // snap_rows x (N+1)
+    double* Hprev;                              // N+1 or null: H' before the last H update of a launch
+                                                // (compensated, FP64), for the energy sum
     const R *J;                                 // N+1 (PIC current, read every step)
     // with a beam attached the field step forms J' itself from the beam's
     // per-cell CIC sums (j_from_sums), so it does not wait for the scatter
     const double* pS;                           // 4 per cell, or null
     const int32_t* prange[2];                   // the beam's live range per parity
+};
+
+// One simulation's energy sum (energy.hip): the launch's final E, H (and
+// FP32 compensations), H' before its last H update, and where W goes.
+template <typename R>
+struct EnergyJob {
+    const R *E, *H, *cE, *cH;         // cE, cH null in FP64
+    const double* Hprev;
+    double* out;
+    int N;
 };
 
 enum SlabKind { SLAB_NONE = 0, SLAB_DIEL = 1, SLAB_LINEAR = 2, SLAB_KERR_REF = 3, SLAB_KERR_E = 4 };

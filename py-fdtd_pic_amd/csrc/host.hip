@@ -30,6 +30,8 @@ template <typename R>
 hipError_t launch_dft(const R* const* series, const int* lens, int nseries, const double* freqs, int nf, double dt,
                       double* out, hipStream_t st);
 template <typename R>
+hipError_t launch_energy(const EnergyJob<R>* jobs, int n, hipStream_t st);
+template <typename R>
 hipError_t launch_push(const SimDev<R>* sims, const BeamDev<R>* beams, const int2* push_items, int npush,
                        long long launch_idx, hipStream_t st, hipEvent_t e0, hipEvent_t e1);
 template <typename R>
@@ -109,6 +111,7 @@ static void fill_dev(SimHost& s, SimDev<R>& d) {
     d.probe_out = P(k++);
     d.snap = P(k++);
     d.J = q.has_current ? P(k++) : nullptr;
+    d.Hprev = s.e_every > 0 ? (double*)s.eblock : nullptr;
     (void)N; (void)ns; (void)nstrip; (void)T;
 }
 
@@ -276,7 +279,11 @@ int fdtd_ctx_destroy(fdtd_ctx* c) {
     (void)hipStreamSynchronize(c->st);
     (void)hipStreamSynchronize(c->up);
     for (SimHost* s : c->sims)
-        if (s) { if (s->block) (void)hipFree(s->block); delete s; }
+        if (s) {
+            if (s->block) (void)hipFree(s->block);
+            if (s->eblock) (void)hipFree(s->eblock);
+            delete s;
+        }
     for (BeamHost* b : c->beams)
         if (b) { if (b->block) (void)hipFree(b->block); delete b; }
     for (void* p : {(void*)c->d_beams, (void*)c->d_pblocks, (void*)c->d_pitems, (void*)c->d_err, c->d_scratch,
@@ -331,7 +338,8 @@ int fdtd_remove_sim(fdtd_ctx* c, int id) {
     HIPCHK(hipStreamSynchronize(c->st));
     SimHost* s = c->sims[id];
     if (s->block) (void)hipFree(s->block);
-    c->dev_bytes -= s->bytes;
+    if (s->eblock) (void)hipFree(s->eblock);
+    c->dev_bytes -= s->bytes + s->ebytes;
     delete s;
     c->sims[id] = nullptr;
     if (id < (int)c->beams.size() && c->beams[id]) {
@@ -360,6 +368,9 @@ int fdtd_reset_sim(fdtd_ctx* c, int id) {
     s->launches = 0;
     s->launch_begin = c->launch_idx;
     s->t_begin = c->gstep;
+    s->e_count = 0;
+    s->e_launches = 0;
+    s->e_steps.clear();
     c->dirty = true;
     return FDTD_OK;
 }
@@ -689,7 +700,10 @@ static int rebuild_t(fdtd_ctx* c) {
     // once; the neighbour table maps each tile to the tiles on either side
     c->persist_ok = false;
     std::vector<int2> nbr;
-    if (c->persist && !pic && c->groups.size() == 1 && c->K_slab == c->K && c->S_run <= U && c->cu_count > 0) {
+    bool energy = false;
+    for (int i : c->tile_sims) energy = energy || c->sims[i]->e_every > 0;
+    if (c->persist && !pic && !energy && c->groups.size() == 1 && c->K_slab == c->K && c->S_run <= U &&
+        c->cu_count > 0) {
         const int per_cu = field_capacity_per_cu<R, sizeof(R) == 4>(c->K, c->groups[0].kind);
         const long long wgs = ((long long)tiles.size() + 3) / 4;
         if (per_cu > 0 && wgs <= (long long)per_cu * c->cu_count) {
@@ -802,6 +816,34 @@ static int launch_once(fdtd_ctx* c, int n, EvPair* ev, hipStream_t st, bool fuse
         }
         if (e != hipSuccess) return fdtd_fail(FDTD_E_HIP, std::string("stencil launch: ") + hipGetErrorString(e));
     }
+    return FDTD_OK;
+}
+
+// Energy sums of the simulations due one after this launch (the bookkeeping
+// in fdtd_run has counted the launch): jobs on the context stream.
+template <typename R>
+static int record_energy(fdtd_ctx* c, const std::vector<int>& due) {
+    std::vector<EnergyJob<R>> jobs;
+    for (int i : due) {
+        SimHost* s = c->sims[i];
+        SimDev<R> d;
+        fill_dev<R>(*s, d);
+        const int par = (int)(s->launches & 1);
+        EnergyJob<R> j;
+        j.E = d.E[par];
+        j.H = d.H[par];
+        j.cE = sizeof(R) == 4 ? d.cE[par] : nullptr;
+        j.cH = sizeof(R) == 4 ? d.cH[par] : nullptr;
+        j.Hprev = (const double*)s->eblock;
+        j.out = (double*)((char*)s->eblock + sizeof(double) * (s->d.N + 1)) + s->e_count;
+        j.N = s->d.N;
+        jobs.push_back(j);
+        s->e_steps.push_back(s->steps_done - 1);
+        s->e_count++;
+    }
+    if (jobs.empty()) return FDTD_OK;
+    const hipError_t e = launch_energy<R>(jobs.data(), (int)jobs.size(), c->st);
+    if (e != hipSuccess) return fdtd_fail(FDTD_E_HIP, std::string("energy launch: ") + hipGetErrorString(e));
     return FDTD_OK;
 }
 
@@ -942,12 +984,20 @@ int fdtd_run(fdtd_ctx* c, int64_t nsteps) {
         if (rc) return rc;
         c->launches++;
         bool finished = false;
+        std::vector<int> due;      // simulations whose energy is taken after this launch
         for (int i : c->tile_sims) {
             SimHost* s = c->sims[i];
             s->launches++;
             const int64_t done = std::min<int64_t>(s->d.steps, c->gstep + n - s->t_begin);
+            const bool stepped = done > s->steps_done;
             s->steps_done = std::max<int64_t>(s->steps_done, done);
             if (s->steps_done >= s->d.steps) finished = true;
+            if (s->e_every > 0 && stepped && ++s->e_launches % s->e_every == 0 && s->e_count < s->e_rows)
+                due.push_back(i);
+        }
+        if (!due.empty()) {
+            rc = c->prec == FDTD_F64 ? record_energy<double>(c, due) : record_energy<float>(c, due);
+            if (rc) return rc;
         }
         c->launch_idx++;
         c->gstep += n;
@@ -991,6 +1041,54 @@ int fdtd_sync(fdtd_ctx* c) {
     int rc = check_persist_flag(c);
     if (rc) return rc;
     return c->has_pic ? check_pic_flag(c) : FDTD_OK;
+}
+
+int fdtd_set_energy(fdtd_ctx* c, int id, int every, int rows) {
+    if (!c || id < 0 || id >= (int)c->sims.size() || !c->sims[id]) return fdtd_fail(FDTD_E_ARG, "bad sim id");
+    if (every < 0 || rows < 0 || (every > 0 && rows < 1))
+        return fdtd_fail(FDTD_E_ARG, "energy: every >= 0 and rows >= 1 when on");
+    SimHost* s = c->sims[id];
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (s->eblock) {
+        (void)hipFree(s->eblock);
+        c->dev_bytes -= s->ebytes;
+        s->eblock = nullptr;
+        s->ebytes = 0;
+    }
+    s->e_every = 0;
+    s->e_rows = s->e_count = 0;
+    s->e_launches = 0;
+    s->e_steps.clear();
+    if (every > 0) {
+        const size_t bytes = sizeof(double) * ((size_t)s->d.N + 1 + (size_t)rows);
+        if (hipMalloc(&s->eblock, bytes) != hipSuccess) {
+            s->eblock = nullptr;
+            return fdtd_fail(FDTD_E_NOMEM, "hipMalloc of the energy rows failed");
+        }
+        HIPCHK(hipMemsetAsync(s->eblock, 0, bytes, c->st));
+        s->ebytes = bytes;
+        c->dev_bytes += bytes;
+        s->e_every = every;
+        s->e_rows = rows;
+    }
+    c->dirty = true;
+    return FDTD_OK;
+}
+
+int fdtd_read_energy(fdtd_ctx* c, int id, double* values, int64_t* steps, int64_t cap, int64_t* count) {
+    if (!c || id < 0 || id >= (int)c->sims.size() || !c->sims[id] || !count || cap < 0 ||
+        (cap > 0 && (!values || !steps)))
+        return fdtd_fail(FDTD_E_ARG, "bad argument");
+    SimHost* s = c->sims[id];
+    HIPCHK(hipStreamSynchronize(c->st));
+    const int64_t n = std::min<int64_t>(cap, s->e_count);
+    *count = s->e_count;
+    if (n > 0) {
+        HIPCHK(hipMemcpy(values, (char*)s->eblock + sizeof(double) * (s->d.N + 1), sizeof(double) * n,
+                         hipMemcpyDeviceToHost));
+        for (int64_t k = 0; k < n; ++k) steps[k] = s->e_steps[k];
+    }
+    return FDTD_OK;
 }
 
 int fdtd_set_persistent(fdtd_ctx* c, int on) {

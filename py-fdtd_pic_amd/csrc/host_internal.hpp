@@ -19,6 +19,13 @@ struct SimHost {
     long long t_begin = 0;      // context global step at admission
     long long steps_done = 0;
     long long launches = 0;     // launches this simulation took part in
+    // energy accumulator (fdtd_set_energy): H' before a launch's last H update
+    // (N+1 FP64) then the recorded W values (rows FP64), one allocation
+    void* eblock = nullptr;
+    size_t ebytes = 0;
+    int e_every = 0, e_rows = 0, e_count = 0;
+    long long e_launches = 0;   // launches in which this simulation stepped, since energy was set
+    std::vector<int64_t> e_steps;   // step index after which each row was taken
 };
 
 // One particle beam attached to a simulation (at most one per simulation).

@@ -201,6 +201,7 @@ struct TileCtx {
     R kD, jD;
     double chi3;
     const R *srcE, *srcH, *Jc;
+    double* Hprev;          // energy: H' before the launch's last H update, or null
     const double* pS;       // beam CIC sums (J' formed here), or null: J' read from Jc
     int pcmin, pcmax;
     R *probe_out, *snap;
@@ -364,6 +365,15 @@ __device__ __forceinline__ void run_steps(TILE_PARAMS, const TileCtx<R>& c, cons
 #pragma unroll
             for (int j = 0; j < K; ++j) H[j] = H[j] - mt[j] * hv;
         }
+        if constexpr (GEN) {
+            // energy: the interior's H' before the launch's last H update
+            if (c.Hprev && s == c.s_hi - 1) {
+#pragma unroll
+                for (int j = 0; j < K; ++j)
+                    if ((interior_mask >> j) & 1u)
+                        c.Hprev[c.base + j] = (double)H[j] - (KAHAN ? (double)cH[j] : 0.0);
+            }
+        }
         const R Er = shl1(E[0]);
 #pragma unroll
         for (int j = 0; j < K; ++j) {
@@ -494,6 +504,7 @@ __device__ __forceinline__ void tile_body(const SimDev<R>& sd, int out0_, long l
     c.kD = sd.kD; c.jD = sd.jD; c.chi3 = sd.kerr_chi3;
     c.nprobe = sd.nprobe; c.snap_interval = sd.snap_interval; c.snap_rows = sd.snap_rows;
     c.srcE = sd.srcE; c.srcH = sd.srcH; c.Jc = sd.J; c.probe_out = sd.probe_out; c.snap = sd.snap;
+    c.Hprev = sd.Hprev;
     c.pS = sd.pS;
     c.pcmin = 1;
     c.pcmax = 0;
@@ -580,7 +591,7 @@ __device__ __forceinline__ void tile_body(const SimDev<R>& sd, int out0_, long l
     // cmin-2 .. cmax+3; a tile whose window misses them runs without J.
     bool needJ = c.Jc != nullptr;
     if (c.pS) needJ = c.pcmin <= c.pcmax && c.t0 + 64 * K - 1 >= c.pcmin - 2 && c.t0 <= c.pcmax + 3;
-    const bool gen = needJ || c.snap_interval > 0 || c.npr > 2;
+    const bool gen = needJ || c.snap_interval > 0 || c.npr > 2 || c.Hprev != nullptr;
     const bool extras = tile_strip || tile_src || tile_tf || c.npr > 0;
     // A slab-boundary tile of the E-form Kerr slab runs the pure-slab loop:
     // a vacuum cell is a slab cell with D = E, P = 0, eps = 1, chi3 = 0 when

@@ -88,6 +88,8 @@ SIGNATURES = {
     "fdtd_device_bytes": (C.c_int, [_vp, C.POINTER(C.c_int64)]),
     "fdtd_version": (C.c_char_p, []),
     "fdtd_kernel_timing": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    "fdtd_set_energy": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    "fdtd_read_energy": (C.c_int, [_vp, C.c_int, _dp, C.POINTER(C.c_int64), C.c_int64, C.POINTER(C.c_int64)]),
     "fdtd_probe_spectra": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.c_int, _dp, C.c_int, C.c_double, _dp]),
     "fdtd_step_op": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(_dp), C.POINTER(C.c_int64), C.c_int, _dp,
                               C.POINTER(C.c_int32)]),

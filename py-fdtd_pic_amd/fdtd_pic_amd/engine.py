@@ -126,6 +126,22 @@ class Sim:
         check(self.ctx.lib.fdtd_read_snapshots(self.ctx.h, self.id, dptr(out), out.size))
         return out
 
+    def set_energy(self, every=1, rows=1024):
+        """Record the FP64 field energy after every `every`-th launch (0: off)."""
+        check(self.ctx.lib.fdtd_set_energy(self.ctx.h, self.id, int(every), int(rows)))
+        self.energy_rows = int(rows)
+
+    def energy(self):
+        """(steps, W): the step index after which each energy value was taken, and W."""
+        cap = getattr(self, "energy_rows", 0)
+        vals = np.zeros(max(cap, 1))
+        steps = np.zeros(max(cap, 1), dtype=np.int64)
+        n = C.c_int64()
+        check(self.ctx.lib.fdtd_read_energy(self.ctx.h, self.id, dptr(vals),
+                                            steps.ctypes.data_as(C.POINTER(C.c_int64)), cap, C.byref(n)))
+        k = min(n.value, cap)
+        return steps[:k], vals[:k]
+
     def progress(self):
         v = C.c_int64()
         check(self.ctx.lib.fdtd_sim_progress(self.ctx.h, self.id, C.byref(v)))
