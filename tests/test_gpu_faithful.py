@@ -111,7 +111,8 @@ def test_looped_sim_batched_input_frequency_sweep():
     points' passes run as one batch of scheduler jobs on the device.  Each
     point's arrays match the oracle's point-by-point loop to 1e-10; the
     measured reflection results(RefCo) (TransformHandler.RefTester peak
-    ratio) to 1e-10 and AnalyticalReflectionE exactly."""
+    ratio) to 1e-10 and AnalyticalReflectionE (a host closed form) to
+    1e-13."""
     import MasterController as MC
     from refcases import looped_start, oracle_looped_sim
     pts = 4
@@ -122,7 +123,7 @@ def test_looped_sim_batched_input_frequency_sweep():
     print("measured", ref, "analytical", anal, flush=True)
     assert len(objs) == pts and len(freqs) == pts
     np.testing.assert_allclose(ref, ref_o, rtol=1e-10)
-    np.testing.assert_array_equal(anal, anal_o)
+    np.testing.assert_allclose(anal, anal_o, rtol=1e-13)     # host closed form (1 ulp across hosts)
     for k in range(pts):
         o = objs_o[k][:3]
         p = objs[k][:3]
