@@ -235,6 +235,20 @@ int fdtd_read_energy(fdtd_ctx *ctx, int sim_id, double *values, int64_t *steps, 
 int fdtd_probe_spectra(fdtd_ctx *ctx, int nsims, const int *sim_ids, int probe, const double *freqs, int nf,
                        double dt, double *out);
 
+/* The same spectra with no allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc), this
+ * variant stages its tables and results in a static device array of the
+ * library (4 MiB + 56 KiB per GPU, held until the process ends) instead of
+ * the context's grow-on-demand buffer, and calls no hipMalloc / hipFree.
+ * Limits: nsims <= 4096, nf <= 1024, nsims * nf <= 262144 (FDTD_E_ARG
+ * beyond them).  Every call on one GPU shares that array: calls that overlap
+ * on one GPU (two threads, two contexts) race; the call is synchronous, so
+ * consecutive calls are safe.  Results equal fdtd_probe_spectra's bit for
+ * bit. */
+int fdtd_probe_spectra_noalloc(fdtd_ctx *ctx, int nsims, const int *sim_ids, int probe, const double *freqs,
+                               int nf, double dt, double *out);
+
 /* ---- single reference update kernels ---------------------------------- */
 /* One BaseFDTD11 per-step function on the device, FP64, in the reference's
  * expression order (no FMA contraction), for callers that drive the

@@ -230,6 +230,46 @@ static int shared_upload_stream(int dev, hipStream_t* out) {
 }
 
 // ------------------------------------------------------------------------
+// Scratch of fdtd_probe_spectra_noalloc: a file-scope device array, one copy
+// per GPU, that holds the pointer table, the series lengths, the frequencies
+// and the staged results of one call.  The host never passes the array's
+// name to a kernel (in host code it is a stand-in, not a device address):
+// fdtd_ctx_create resolves each device's address once with
+// hipGetSymbolAddress, with that device current, and the entry point uses it.
+// ------------------------------------------------------------------------
+constexpr int kNaMaxSims = 4096;                 // series per call
+constexpr int kNaMaxFreqs = 1024;                // frequencies per call
+constexpr int64_t kNaMaxPairs = 262144;          // series x frequencies per call (4 MiB of results)
+constexpr size_t kNaOffLens = sizeof(void*) * kNaMaxSims;
+constexpr size_t kNaOffFreqs = kNaOffLens + sizeof(int) * kNaMaxSims;
+constexpr size_t kNaOffOut = kNaOffFreqs + sizeof(double) * kNaMaxFreqs;
+constexpr size_t kNaScratchBytes = kNaOffOut + 2 * sizeof(double) * kNaMaxPairs;
+static_assert(kNaOffLens % 256 == 0 && kNaOffFreqs % 256 == 0 && kNaOffOut % 256 == 0, "aligned sub-arrays");
+
+static __device__ __align__(256) char scratch[kNaScratchBytes];
+
+static void* g_na_ptr[kMaxDevices] = {nullptr};
+
+// Resolves the device address of `scratch` on device dev (current), once.
+static int noalloc_scratch_init(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) return fdtd_fail(FDTD_E_ARG, "device index beyond the library's table");
+    std::lock_guard<std::mutex> lk(g_dev_mu);
+    if (g_na_ptr[dev]) return FDTD_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fdtd_fail(FDTD_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_na_ptr[dev] = p;
+    return FDTD_OK;
+}
+
+static void* noalloc_scratch(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_dev_mu);
+    return g_na_ptr[dev];
+}
+
+// ------------------------------------------------------------------------
 // C ABI
 // ------------------------------------------------------------------------
 extern "C" {
@@ -245,6 +285,8 @@ int fdtd_ctx_create(int precision, fdtd_ctx** out) {
         return fdtd_fail(FDTD_E_HIP, "no HIP device visible");
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return fdtd_fail(FDTD_E_HIP, "hipGetDevice failed");
+    int rc0 = noalloc_scratch_init(dev);
+    if (rc0) return rc0;
     fdtd_ctx* c = new fdtd_ctx();
     c->prec = precision;
     c->device = dev;
@@ -1370,11 +1412,10 @@ int pic_read(fdtd_ctx* c, int id, double* x, double* u, int8_t* alive, int64_t n
 // ------------------------------------------------------------------------
 // spectra of probe series (FP64 DFT on the device)
 // ------------------------------------------------------------------------
+// The series of probe `probe` of each listed simulation (device pointers
+// and lengths, host side).
 template <typename R>
-static int spectra_t(fdtd_ctx* c, int nsims, const int* ids, int probe, const double* freqs, int nf, double dt,
-                     double* out) {
-    std::vector<const R*> ptrs(nsims);
-    std::vector<int> lens(nsims);
+static int spectra_series(fdtd_ctx* c, int nsims, const int* ids, int probe, const R** ptrs, int* lens) {
     for (int k = 0; k < nsims; ++k) {
         SimHost* s = get_sim(c, ids[k]);
         if (!s) return fdtd_fail(FDTD_E_ARG, "bad sim id " + std::to_string(ids[k]));
@@ -1384,6 +1425,34 @@ static int spectra_t(fdtd_ctx* c, int nsims, const int* ids, int probe, const do
         ptrs[k] = d.probe_out + (size_t)probe * s->d.steps;
         lens[k] = s->d.steps;
     }
+    return FDTD_OK;
+}
+
+// Uploads the tables into the device buffers given, runs the DFT and copies
+// the results to out; synchronous.
+template <typename R>
+static int spectra_run(fdtd_ctx* c, int nsims, const R* const* ptrs, const int* lens, const double* freqs, int nf,
+                       double dt, double* out, const R** d_ptrs, int* d_lens, double* d_freqs, double* d_out) {
+    HIPCHK(hipMemcpyAsync(d_ptrs, ptrs, sizeof(void*) * nsims, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(d_lens, lens, sizeof(int) * nsims, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(d_freqs, freqs, sizeof(double) * nf, hipMemcpyHostToDevice, c->st));
+    for (int k0 = 0; k0 < nsims; k0 += 32768) {
+        const int nk = std::min(32768, nsims - k0);
+        hipError_t e = launch_dft<R>(d_ptrs + k0, d_lens + k0, nk, d_freqs, nf, dt, d_out + 2 * (size_t)k0 * nf, c->st);
+        if (e != hipSuccess) return fdtd_fail(FDTD_E_HIP, std::string("dft launch: ") + hipGetErrorString(e));
+    }
+    HIPCHK(hipMemcpyAsync(out, d_out, sizeof(double) * 2 * (size_t)nsims * nf, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return FDTD_OK;
+}
+
+template <typename R>
+static int spectra_t(fdtd_ctx* c, int nsims, const int* ids, int probe, const double* freqs, int nf, double dt,
+                     double* out) {
+    std::vector<const R*> ptrs(nsims);
+    std::vector<int> lens(nsims);
+    int rc = spectra_series<R>(c, nsims, ids, probe, ptrs.data(), lens.data());
+    if (rc) return rc;
     const size_t o_lens = align_up(sizeof(void*) * nsims, 256);
     const size_t o_freqs = o_lens + align_up(sizeof(int) * nsims, 256);
     const size_t o_out = o_freqs + align_up(sizeof(double) * nf, 256);
@@ -1396,21 +1465,23 @@ static int spectra_t(fdtd_ctx* c, int nsims, const int* ids, int probe, const do
         c->d_scratch_cap = need;
     }
     char* base = (char*)c->d_scratch;
-    const R** d_ptrs = (const R**)base;
-    int* d_lens = (int*)(base + o_lens);
-    double* d_freqs = (double*)(base + o_freqs);
-    double* d_out = (double*)(base + o_out);
-    HIPCHK(hipMemcpyAsync(d_ptrs, ptrs.data(), sizeof(void*) * nsims, hipMemcpyHostToDevice, c->st));
-    HIPCHK(hipMemcpyAsync(d_lens, lens.data(), sizeof(int) * nsims, hipMemcpyHostToDevice, c->st));
-    HIPCHK(hipMemcpyAsync(d_freqs, freqs, sizeof(double) * nf, hipMemcpyHostToDevice, c->st));
-    for (int k0 = 0; k0 < nsims; k0 += 32768) {
-        const int nk = std::min(32768, nsims - k0);
-        hipError_t e = launch_dft<R>(d_ptrs + k0, d_lens + k0, nk, d_freqs, nf, dt, d_out + 2 * (size_t)k0 * nf, c->st);
-        if (e != hipSuccess) return fdtd_fail(FDTD_E_HIP, std::string("dft launch: ") + hipGetErrorString(e));
-    }
-    HIPCHK(hipMemcpyAsync(out, d_out, sizeof(double) * 2 * (size_t)nsims * nf, hipMemcpyDeviceToHost, c->st));
-    HIPCHK(hipStreamSynchronize(c->st));
-    return FDTD_OK;
+    return spectra_run<R>(c, nsims, ptrs.data(), lens.data(), freqs, nf, dt, out, (const R**)base,
+                          (int*)(base + o_lens), (double*)(base + o_freqs), (double*)(base + o_out));
+}
+
+// The same spectra through the library's static scratch array of this
+// context's device: no device or host allocation (tables on the stack).
+template <typename R>
+static int spectra_noalloc_t(fdtd_ctx* c, int nsims, const int* ids, int probe, const double* freqs, int nf,
+                             double dt, double* out) {
+    char* base = (char*)noalloc_scratch(c->device);
+    if (!base) return fdtd_fail(FDTD_E_STATE, "no-allocation scratch not resolved on this device");
+    const R* ptrs[kNaMaxSims];
+    int lens[kNaMaxSims];
+    int rc = spectra_series<R>(c, nsims, ids, probe, ptrs, lens);
+    if (rc) return rc;
+    return spectra_run<R>(c, nsims, ptrs, lens, freqs, nf, dt, out, (const R**)base, (int*)(base + kNaOffLens),
+                          (double*)(base + kNaOffFreqs), (double*)(base + kNaOffOut));
 }
 
 extern "C" int fdtd_probe_spectra(fdtd_ctx* c, int nsims, const int* sim_ids, int probe, const double* freqs, int nf,
@@ -1420,6 +1491,20 @@ extern "C" int fdtd_probe_spectra(fdtd_ctx* c, int nsims, const int* sim_ids, in
     if (nsims == 0 || nf == 0) return FDTD_OK;
     return c->prec == FDTD_F64 ? spectra_t<double>(c, nsims, sim_ids, probe, freqs, nf, dt, out)
                                : spectra_t<float>(c, nsims, sim_ids, probe, freqs, nf, dt, out);
+}
+
+extern "C" int fdtd_probe_spectra_noalloc(fdtd_ctx* c, int nsims, const int* sim_ids, int probe,
+                                          const double* freqs, int nf, double dt, double* out) {
+    if (!c || nsims < 0 || (nsims > 0 && (!sim_ids || !out)) || nf < 0 || (nf > 0 && !freqs))
+        return fdtd_fail(FDTD_E_ARG, "bad argument");
+    if (nsims > kNaMaxSims || nf > kNaMaxFreqs || (int64_t)nsims * nf > kNaMaxPairs)
+        return fdtd_fail(FDTD_E_ARG, "fdtd_probe_spectra_noalloc: " + std::to_string(nsims) + " series x " +
+                                         std::to_string(nf) + " frequencies exceeds its limits (" +
+                                         std::to_string(kNaMaxSims) + " series, " + std::to_string(kNaMaxFreqs) +
+                                         " frequencies, " + std::to_string(kNaMaxPairs) + " pairs)");
+    if (nsims == 0 || nf == 0) return FDTD_OK;
+    return c->prec == FDTD_F64 ? spectra_noalloc_t<double>(c, nsims, sim_ids, probe, freqs, nf, dt, out)
+                               : spectra_noalloc_t<float>(c, nsims, sim_ids, probe, freqs, nf, dt, out);
 }
 
 // ------------------------------------------------------------------------

@@ -91,6 +91,8 @@ SIGNATURES = {
     "fdtd_set_energy": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     "fdtd_read_energy": (C.c_int, [_vp, C.c_int, _dp, C.POINTER(C.c_int64), C.c_int64, C.POINTER(C.c_int64)]),
     "fdtd_probe_spectra": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.c_int, _dp, C.c_int, C.c_double, _dp]),
+    "fdtd_probe_spectra_noalloc": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.c_int, _dp, C.c_int, C.c_double,
+                                             _dp]),
     "fdtd_step_op": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(_dp), C.POINTER(C.c_int64), C.c_int, _dp,
                               C.POINTER(C.c_int32)]),
     "pic_attach": (C.c_int, [_vp, C.c_int, C.POINTER(BeamDesc)]),
