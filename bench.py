@@ -184,21 +184,29 @@ def pic_bench(a):
     return out
 
 
-def pic_cpu_baseline(p, x, u, q, m, w, steps=40):
-    """The FP64 NumPy particle oracle (one core) on the first steps of the same workload."""
-    from oracle import pic as P, physical as ph
+def pic_cpu_baseline(p, x, u, q, m, w, target_s=10.0):
+    """The compiled particle + field step (oracle/fdtd_cpu.c: OpenMP push,
+    per-thread deposit partials, OpenMP field step; FP64) on every host
+    thread, over the first steps of the same workload until about target_s
+    seconds have been timed."""
+    from oracle import cpu_ref, pic as P, physical as ph
+    try:
+        cpu_ref.use(cpu_ref.build_lib(path="/tmp/fdtd_cpu_native.so", native=True))
+    except Exception:
+        cpu_ref.use(cpu_ref.build_lib())
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or cpu_ref.lib().cpu_max_threads()
     mm = ph.build(p)
-    b = P.Beam(x, u, q, m, w, mm["dt"], p["N"], p["Sc"])
-    s = ph.State(p["N"], mm["R"] - mm["F"])
-    Hprev = s.H.copy()
-    t0 = time.perf_counter()
-    for t in range(steps):
-        J = P.push_deposit(b, s.E, Hprev, s.H)
-        Hprev = s.H.copy()
-        ph.step(mm, s, t, J)
-    secs = time.perf_counter() - t0
-    return dict(value=len(x) * steps / secs / 1e9, unit="Gparticle-pushes/s", cores=1, kind="port",
-                sample=f"config3, first {steps} steps of {len(x)} particles, NumPy FP64 oracle, {secs:.1f} s")
+    c = cpu_ref.CpuPic(mm, P.Beam(x, u, q, m, w, mm["dt"], p["N"], p["Sc"]), threads=threads)
+    probe = 5
+    secs = c.run(0, probe)
+    steps = probe
+    while secs < 0.95 * target_s and steps < p["steps"]:
+        n = int(min(p["steps"] - steps, max(probe, (target_s - secs) / (secs / steps))))
+        secs += c.run(steps, steps + n)
+        steps += n
+    return dict(value=len(x) * steps / secs / 1e9, unit="Gparticle-pushes/s", cores=threads, kind="port",
+                sample=f"config3, first {steps} steps of {len(x)} particles with the field step, compiled FP64 C "
+                       f"(oracle/fdtd_cpu.c), {threads} OpenMP threads, {secs:.1f} s")
 
 
 def config4_bench(a):

@@ -31,6 +31,12 @@ class StateT(C.Structure):
     _fields_ = [(n, _dp) for n in ("E", "H", "psiE", "psiH", "D", "P", "Pm")]
 
 
+class BeamT(C.Structure):
+    _fields_ = [("n", C.c_int64), ("x", _dp), ("u", _dp), ("alive", C.POINTER(C.c_int8)), ("a", C.c_double),
+                ("jscale", C.c_double), ("Hprev", _dp), ("J", _dp), ("B1", _dp), ("B2", _dp), ("m0", _ip),
+                ("w0", _dp), ("w1", _dp)]
+
+
 def build_lib(force=False, path=None, native=False):
     """Compile fdtd_cpu.c.  The in-tree copy is portable (x86-64-v3); a
     `native=True` copy tuned for the host that runs the baseline can be built
@@ -65,6 +71,9 @@ def lib(path=None):
         _lib.cpu_phys_run.restype = C.c_double
         _lib.cpu_phys_run.argtypes = [C.POINTER(PhysT), C.POINTER(StateT), C.c_int, C.c_int, _dp, C.c_int]
         _lib.cpu_phys_run_many.restype = C.c_double
+        _lib.cpu_pic_run.restype = C.c_double
+        _lib.cpu_pic_run.argtypes = [C.POINTER(PhysT), C.POINTER(StateT), C.POINTER(BeamT), C.c_int, C.c_int, _dp,
+                                     C.c_int]
         _lib.cpu_max_threads.restype = C.c_int
     return _lib
 
@@ -124,3 +133,31 @@ def run_model(m, threads=1, steps=None):
     sim = CpuSim(m)
     secs = sim.run(0, m["steps"] if steps is None else steps, threads)
     return sim, secs
+
+
+class CpuPic(CpuSim):
+    """A model with a particle beam (oracle.pic.Beam's state, copied): the C
+    restatement of oracle/pic.run.  One thread is bitwise oracle/pic.py;
+    more threads push in parallel and sum the deposit in per-thread partials."""
+
+    def __init__(self, m, beam, threads=1):
+        super().__init__(m)
+        N, n = m["N"], len(beam.x)
+        self.threads = max(1, int(threads))
+        self.x = np.ascontiguousarray(beam.x, dtype=np.float64).copy()
+        self.u = np.ascontiguousarray(beam.u, dtype=np.float64).reshape(-1, 3).copy()
+        self.alive = np.ascontiguousarray(beam.alive, dtype=np.int8).copy()
+        self.Hprev = np.zeros(N + 1)
+        self.Jb = np.zeros(N + 1)
+        self.B1 = np.zeros(self.threads * (N + 1))
+        self.B2 = np.zeros(self.threads * (N + 1))
+        self.m0 = np.zeros(n, dtype=np.int32)
+        self.w0, self.w1 = np.zeros(n), np.zeros(n)
+        self.b = BeamT(n, _p(self.x), _p(self.u), self.alive.ctypes.data_as(C.POINTER(C.c_int8)), beam.a,
+                       beam.jscale, _p(self.Hprev), _p(self.Jb), _p(self.B1), _p(self.B2), _p(self.m0, _ip),
+                       _p(self.w0), _p(self.w1))
+
+    def run(self, t0=0, t1=None, threads=None):
+        t1 = self.m["steps"] if t1 is None else t1
+        nth = self.threads if threads is None else min(int(threads), self.threads)
+        return lib().cpu_pic_run(C.byref(self.p), C.byref(self.s), C.byref(self.b), t0, t1, _p(self.scratch), nth)

@@ -57,13 +57,19 @@ static double now_s(void)
     return ts.tv_sec + 1e-9 * ts.tv_nsec;
 }
 
+static void one_step_j(const cpu_phys_t *m, cpu_state_t *s, int t, double *dH, int nth, const double *J);
+
 static void one_step(const cpu_phys_t *m, cpu_state_t *s, int t, double *dH, int nth)
+{
+    one_step_j(m, s, t, dH, nth, m->J ? m->J + (size_t)t * (m->N + 1) : NULL);
+}
+
+static void one_step_j(const cpu_phys_t *m, cpu_state_t *s, int t, double *dH, int nth, const double *J)
 {
     const int N = m->N;
     const double Sc = m->Sc;
     const int F = m->F, R = m->R;
     double *E = s->E, *H = s->H;
-    const double *J = m->J ? m->J + (size_t)t * (N + 1) : NULL;
     (void)nth;
     /* polarisation with E at the start of the step (BaseFDTD11.py:631) */
 #pragma omp parallel for schedule(static) num_threads(nth) if (nth > 1)
@@ -134,6 +140,129 @@ double cpu_phys_run_many(int count, const cpu_phys_t *const *ms, cpu_state_t *co
             for (int k = 0; k < m->nprobe; ++k)
                 m->probe_out[(size_t)k * m->steps + t] = ss[i]->E[m->probes[k]];
         }
+    }
+    return now_s() - start;
+}
+
+/*
+ * Particles (oracle/pic.py restated): relativistic Boris push with linear
+ * gather of E^n and the time-averaged H', CIC deposit of J' at x^{n+1/2},
+ * absorption into cells 1 / N-2.  The push runs in parallel over particles.
+ * With one thread the deposit sums each node in particle order, as numpy's
+ * bincount does (two sums, J = (0 + B1) + B2), so the run is bitwise
+ * oracle/pic.py's; with more threads each thread sums a contiguous range of
+ * particles and the partials are added in thread order (deterministic for a
+ * given thread count).
+ */
+typedef struct {
+    int64_t n;
+    double *x, *u;                           /* n, n x 3 */
+    int8_t *alive;
+    double a, jscale;
+    double *Hprev;                           /* N+1: H' before the last field step */
+    double *J;                               /* N+1 */
+    double *B1, *B2;                         /* nthreads x (N+1) partial sums */
+    int32_t *m0;                             /* n: node of the first CIC weight (-1: none) */
+    double *w0, *w1;                         /* n: the two weights */
+} cpu_beam_t;
+
+static void push_particles(const cpu_phys_t *m, const cpu_state_t *s, cpu_beam_t *b, int nth)
+{
+    const int N = m->N;
+    const double Sc = m->Sc, a = b->a;
+    const double *E = s->E, *H = s->H, *Hp = b->Hprev;
+#pragma omp parallel for schedule(static) num_threads(nth) if (nth > 1)
+    for (int64_t i = 0; i < b->n; ++i) {
+        if (!b->alive[i]) { b->m0[i] = -1; continue; }
+        const double x = b->x[i];
+        const double fi = floor(x);
+        const int64_t ci = (int64_t)fi;
+        const double f = x - fi;
+        const double Eg = (1.0 - f) * E[ci] + f * E[ci + 1];
+        const double xh = x - 0.5;
+        const double fj = floor(xh);
+        const int64_t cj = (int64_t)fj;
+        const double g = xh - fj;
+        const double h0 = 0.5 * (Hp[cj] + H[cj]), h1 = 0.5 * (Hp[cj + 1] + H[cj + 1]);
+        const double Hg = (1.0 - g) * h0 + g * h1;
+        double *u = b->u + 3 * i;
+        const double um0 = u[0] + a * Eg, um1 = u[1], um2 = u[2];
+        const double gm = sqrt(1.0 + ((um0 * um0 + um1 * um1) + um2 * um2));
+        const double ty = -a * Hg / gm;
+        const double sy = 2.0 * ty / (1.0 + ty * ty);
+        const double upx = um0 - um2 * ty;
+        const double upz = um2 + um0 * ty;
+        const double un0 = (um0 - upz * sy) + a * Eg;
+        const double un2 = um2 + upx * sy;
+        u[0] = un0;
+        u[2] = un2;
+        const double gam = sqrt(1.0 + ((un0 * un0 + um1 * um1) + un2 * un2));
+        const double bx = un0 / gam, bz = un2 / gam;
+        const double xd = x + 0.5 * Sc * bz;
+        double xn = x + Sc * bz;
+        const double fm = floor(xd);
+        const double fh = xd - fm;
+        const double q = b->jscale * bx;
+        b->m0[i] = (int32_t)fm;
+        b->w0[i] = q * (1.0 - fh);
+        b->w1[i] = q * fh;
+        const double cell = floor(xn);
+        if (cell < 2.0) { xn = 1.5; b->alive[i] = 0; }
+        else if (cell > N - 3) { xn = (double)(N - 2) + 0.5; b->alive[i] = 0; }
+        b->x[i] = xn;
+    }
+}
+
+static void deposit(const cpu_phys_t *m, cpu_beam_t *b, int nth)
+{
+    const int n1 = m->N + 1;
+    if (nth <= 1) {
+        memset(b->B1, 0, sizeof(double) * n1);
+        memset(b->B2, 0, sizeof(double) * n1);
+        for (int64_t i = 0; i < b->n; ++i) {
+            if (b->m0[i] < 0) continue;
+            b->B1[b->m0[i]] += b->w0[i];
+            b->B2[b->m0[i] + 1] += b->w1[i];
+        }
+        for (int k = 0; k < n1; ++k) b->J[k] = (0.0 + b->B1[k]) + b->B2[k];
+        return;
+    }
+#pragma omp parallel num_threads(nth)
+    {
+        const int t = omp_get_thread_num(), nt = omp_get_num_threads();
+        double *B1 = b->B1 + (size_t)t * n1, *B2 = b->B2 + (size_t)t * n1;
+        memset(B1, 0, sizeof(double) * n1);
+        memset(B2, 0, sizeof(double) * n1);
+        const int64_t lo = b->n * t / nt, hi = b->n * (t + 1) / nt;
+        for (int64_t i = lo; i < hi; ++i) {
+            if (b->m0[i] < 0) continue;
+            B1[b->m0[i]] += b->w0[i];
+            B2[b->m0[i] + 1] += b->w1[i];
+        }
+#pragma omp barrier
+#pragma omp for schedule(static)
+        for (int k = 0; k < n1; ++k) {
+            double s1 = 0.0, s2 = 0.0;
+            for (int q = 0; q < nt; ++q) { s1 += b->B1[(size_t)q * n1 + k]; s2 += b->B2[(size_t)q * n1 + k]; }
+            b->J[k] = (0.0 + s1) + s2;
+        }
+    }
+}
+
+/* Coupled run over steps [t0, t1): push/deposit, then the field step with J'.
+ * Returns wall seconds. */
+double cpu_pic_run(const cpu_phys_t *m, cpu_state_t *s, cpu_beam_t *b, int t0, int t1, double *scratch,
+                   int nthreads)
+{
+    const int n1 = m->N + 1;
+    double start = now_s();
+    for (int t = t0; t < t1; ++t) {
+        push_particles(m, s, b, nthreads);
+        deposit(m, b, nthreads);
+        memcpy(b->Hprev, s->H, sizeof(double) * n1);
+        one_step_j(m, s, t, scratch, nthreads, b->J);
+        for (int k = 0; k < m->nprobe; ++k)
+            m->probe_out[(size_t)k * m->steps + t] = s->E[m->probes[k]];
     }
     return now_s() - start;
 }

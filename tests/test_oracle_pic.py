@@ -117,3 +117,34 @@ def test_cold_beam_drifts_without_current():
     assert np.all(s.E == 0) and np.all(s.H == 0)
     beta = 0.1 / math.sqrt(1.01)
     np.testing.assert_allclose(b.x, x + 50 * 0.5 * beta, rtol=0, atol=1e-9)
+
+
+def test_c_particle_step_bitwise_equal_to_numpy_oracle():
+    """oracle/fdtd_cpu.c's particle step (the compiled PIC CPU baseline): one
+    thread is bitwise oracle/pic.py (fields, probes, x, u, alive) on a driven
+    beam with absorption at the left end; eight threads (per-thread deposit
+    partials) agree to 1e-12."""
+    from oracle import cpu_ref
+    spec = ph.default_spec(N=900, steps=300, slab_start=600, slab_stop=700, src_amp=1e9, probes=[50, 800])
+    m = ph.build(spec)
+    rng = np.random.default_rng(8)
+    n = 3000
+    x = rng.uniform(3, 200, n)
+    u = rng.normal(0, 0.02, (n, 3))
+    u[:, 2] -= 0.2                          # drifting left: some are absorbed
+    def beam():
+        return _beam(x, u, N=900, Sc=spec["Sc"], dz=spec["dz"])
+    b = beam()
+    s, pr = P.run(m, b)
+    assert (~b.alive).any() and b.alive.any()
+    for threads, exact in ((1, True), (8, False)):
+        c = cpu_ref.CpuPic(m, beam(), threads=threads)
+        c.run()
+        if exact:
+            assert np.array_equal(c.E, s.E) and np.array_equal(c.H, s.H)
+            assert np.array_equal(c.probe_out, pr)
+            assert np.array_equal(c.x, b.x) and np.array_equal(c.u, b.u)
+            assert np.array_equal(c.alive.astype(bool), b.alive)
+        else:
+            for got, want in ((c.E, s.E), (c.probe_out, pr), (c.u, b.u)):
+                assert np.linalg.norm(got - want) <= 1e-12 * np.linalg.norm(want)
