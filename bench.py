@@ -61,13 +61,20 @@ def parse():
     ap.add_argument("--c4-sample", type=int, default=24, help="instances timed in the per-simulation loop")
     ap.add_argument("--c4-order", default="arrival", choices=["costly", "arrival"],
                     help="batched admission order: costly jobs first, or as listed")
-    ap.add_argument("--c5-requests", type=int, default=32, help="requests of the config 5 trace replayed")
+    ap.add_argument("--c5-requests", type=int, default=256, help="requests of the config 5 trace replayed")
+    ap.add_argument("--no-f64", action="store_true", help="skip the FP64 config 2 object")
+    ap.add_argument("--no-c4", action="store_true", help="skip the config 4 (batched sweep) object")
+    ap.add_argument("--no-c5", action="store_true", help="skip the config 5 (served trace) object")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 256 MiB device copy (FETCH_SIZE/WRITE_SIZE calibration under rocprofv3 --pmc)")
     return ap.parse_args()
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "stencil_tiles_config2_hbm_traffic.txt")
+ISSUE_FILE = os.path.join(ROOT, "profiles", "stencil_tiles_config2_valu_issue.txt")
+# VALU issue peak without packed FP32: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64
+# instruction per 4 cycles per SIMD (G wave-instructions/s)
+VALU_ISSUE_PEAK = 256 * 4 * 2.4 / 4
 PIC_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pic_config3_kernel_trace_and_traffic.txt")
 
 
@@ -91,6 +98,19 @@ def measured_traffic(precision, K, S, Ks):
     for line in open(TRAFFIC_FILE):
         f = line.split()
         if len(f) == 6 and f[0] == "traffic" and f[1] == precision and f[2] == f"K={K}" and f[3] == f"Kslab={Ks}" \
+                and f[4] == f"S={S}":
+            return float(f[5])
+    return None
+
+
+def measured_issue(precision, K, S, Ks):
+    """VALU wave-instructions per field launch from the committed rocprofv3
+    --pmc SQ_INSTS_VALU summary, or None when none matches this tiling."""
+    if not os.path.exists(ISSUE_FILE):
+        return None
+    for line in open(ISSUE_FILE):
+        f = line.split()
+        if len(f) == 6 and f[0] == "issue" and f[1] == precision and f[2] == f"K={K}" and f[3] == f"Kslab={Ks}" \
                 and f[4] == f"S={S}":
             return float(f[5])
     return None
@@ -287,30 +307,14 @@ def cpu_run(cpu_ref, m, n, threads):
     return cpu_ref.CpuSim(m).run(0, n, threads)
 
 
-def main():
-    a = parse()
-    if a.gpus != 1:
-        print(f"bench.py runs on one GPU only (--gpus {a.gpus} requested): simulations shard across GPUs "
-              "as independent instances or 1-D domain slices, and that path is not built", file=sys.stderr)
-        sys.exit(2)
-    if a.calibrate_copy:
-        calibrate_copy()
-        return
-    if a.pic_only:
-        print(json.dumps(pic_bench(a)), flush=True)
-        return
-    if a.workload == "config4":
-        print(json.dumps(config4_bench(a)), flush=True)
-        return
-    if a.workload == "config5":
-        print(json.dumps(config5_bench(a)), flush=True)
-        return
+def config2_line(a, precision):
+    """The config 2 measurement in one precision: (bench line dict, parameters)."""
     from fdtd_pic_amd import configs, engine
 
     p = configs.config2(steps=a.sim_steps)
-    ctx = engine.Context(a.precision)
+    ctx = engine.Context(precision)
     persistent = a.persistent == "on"
-    K0, S0, Ks0 = (configs.TILING_CONFIG2_PERSISTENT if persistent else configs.TILING_CONFIG2)[a.precision]
+    K0, S0, Ks0 = (configs.TILING_CONFIG2_PERSISTENT if persistent else configs.TILING_CONFIG2)[precision]
     K = a.K or K0
     Ks = a.K_slab or (Ks0 if K == K0 else K)
     S = a.S or (S0 if K == K0 else 8 * Ks)
@@ -334,7 +338,7 @@ def main():
     ms_per_step = wall * 1e3 / a.steps
     cells = p["N"] * p["steps"]
     value = cells / (wall / a.steps) / 1e9
-    bpc = configs.bytes_per_cell_update(p, a.precision)
+    bpc = configs.bytes_per_cell_update(p, precision)
     launch_ms = kern_ms / max(sampled, 1)                 # average launch-to-launch duration
     achieved = bpc * p["N"] * S / (launch_ms / 1e3) / 1e9 if kern_ms > 0 else 0.0
     out = {
@@ -348,24 +352,81 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32+fp64-kerr" if a.precision == "f32" else "f64",
+        "dtype": "fp32+fp64-kerr" if precision == "f32" else "f64",
         "data": "synthetic (seeded config 2 parameters)",
         "config": {"workload": "config2: N=65536 cells, 50000 steps, Lorentz+Kerr slab [20000,40000)",
-                   "sim_steps": p["steps"], "K": K, "K_slab": Ks, "S": S, "precision": a.precision,
+                   "sim_steps": p["steps"], "K": K, "K_slab": Ks, "S": S, "precision": precision,
                    "persistent": persistent},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(a.precision, K, S, Ks),
+                     "traffic": measured_traffic(precision, K, S, Ks),
                      "algorithmic_bytes_per_launch": round(bpc * p["N"] * S),
                      "bytes_per_cell_update": round(bpc, 2),
                      "avg_launch_us": round(launch_ms * 1e3, 2), "sampled_launches": sampled,
-                     "launches": a.steps * ((p["steps"] + S - 1) // S)},
+                     "launches": a.steps * ((p["steps"] + S - 1) // S),
+                     "note": "achieved = algorithmic bytes of the one-step streaming count per launch over the "
+                             "launch time: an effective figure (temporal blocking keeps the state on chip); "
+                             "the DRAM bytes the kernel moves are `traffic`, dram_frac their rate over the peak"},
     }
+    tr = out["roofline"]["traffic"]
+    if tr and kern_ms > 0:
+        out["roofline"]["dram_frac"] = round(tr / (launch_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)
+    issued = measured_issue(precision, K, S, Ks)
+    if issued and kern_ms > 0 and not persistent:
+        rate = issued / (launch_ms / 1e3) / 1e9
+        U = 64 * K - 2 * S
+        out["issue"] = {"bound": "valu_issue", "achieved": round(rate, 1), "peak": VALU_ISSUE_PEAK,
+                        "unit": "G wave-instructions/s", "frac": round(rate / VALU_ISSUE_PEAK, 4),
+                        "wave_insts_per_launch": issued,
+                        "lane_insts_per_cell_update": round(issued * 64 / (p["N"] * S), 1),
+                        "useful_fraction": round(U / (64 * K), 3),
+                        "useful_lane_insts_per_cell_update": round(issued * 64 / (p["N"] * S) * U / (64 * K), 1),
+                        "source": "rocprofv3 --pmc SQ_INSTS_VALU (profiles/stencil_tiles_config2_valu_issue.txt) "
+                                  "over the live launch time"}
     ctx.close()
+    return out, p
+
+
+def progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (--gpus {a.gpus} requested): simulations shard across GPUs "
+              "as independent instances or 1-D domain slices, and that path is not built", file=sys.stderr)
+        sys.exit(2)
+    if a.calibrate_copy:
+        calibrate_copy()
+        return
+    if a.pic_only:
+        print(json.dumps(pic_bench(a)), flush=True)
+        return
+    if a.workload == "config4":
+        print(json.dumps(config4_bench(a)), flush=True)
+        return
+    if a.workload == "config5":
+        print(json.dumps(config5_bench(a)), flush=True)
+        return
+    out, p = config2_line(a, a.precision)
+    progress(f"config2 {a.precision}: {out['value']} Gcell-updates/s")
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, a.cpu_seconds)
+        progress(f"config2 cpu baseline: {out['cpu_baseline']['value']:.3f} Gcell-updates/s")
     if not a.no_pic:
         out["pic"] = pic_bench(a)
+        progress(f"config3 pic: {out['pic']['value']} Gparticle-pushes/s")
+    if not a.no_f64 and a.precision != "f64":
+        f64, _ = config2_line(a, "f64")
+        out["config2_f64"] = {k: f64[k] for k in ("value", "unit", "ms_per_step", "dtype", "config", "roofline")}
+        progress(f"config2 f64: {f64['value']} Gcell-updates/s")
+    if not a.no_c4:
+        out["config4"] = config4_bench(a)
+        progress(f"config4: {out['config4']['batched_s']} s batched, {out['config4']['speedup_vs_per_sim_loop']}x")
+    if not a.no_c5:
+        out["config5"] = config5_bench(a)
+        progress(f"config5: p50 {out['config5']['p50_latency_s']} s, p99 {out['config5']['p99_latency_s']} s")
     print(json.dumps(out), flush=True)
 
 

@@ -28,34 +28,69 @@ def work(p):
 
 
 class Server:
-    def __init__(self, precision="f32", budget_bytes=64 << 30, max_slots=512, freqs=None):
+    def __init__(self, precision="f32", budget_bytes=64 << 30, max_slots=512, freqs=None, prep_workers=8,
+                 prep_ahead=64):
         self.sch = sweep.Scheduler(precision, budget_bytes=budget_bytes, max_slots=max_slots, freqs=freqs)
+        self.prep_workers, self.prep_ahead = int(prep_workers), int(prep_ahead)
 
     def replay(self, arrivals, params, time_scale=1.0, log=None):
-        """Replay a trace in wall-clock time (arrival t is released at t * time_scale)."""
+        """Replay a trace in wall-clock time (arrival t is released at t * time_scale).
+
+        Each loop queues one slice on the contexts that hold jobs, then, while
+        the device runs it (fdtd_busy), admits the requests that have arrived
+        and fit, releasing new arrivals as the clock passes them; it then waits
+        for the slice and completes the jobs whose passes are done.  The next
+        prep_ahead waiting requests' inputs (particles, descriptors) are built
+        on worker threads meanwhile.  With nothing resident it admits what has
+        arrived, or sleeps until the next arrival."""
+        from concurrent.futures import ThreadPoolExecutor
         jobs = [sweep.Job(i, p, a * time_scale) for i, (a, p) in enumerate(zip(arrivals, params))]
-        waiting = []
-        nxt = 0
-        t0 = time.perf_counter()
         sch = self.sch
-        while nxt < len(jobs) or waiting or sch.active:
-            now = time.perf_counter() - t0
-            while nxt < len(jobs) and jobs[nxt].arrival <= now:
-                waiting.append(jobs[nxt])
-                nxt += 1
-            while waiting and (sch.fits(waiting[0]) or not sch.active):
-                sch.admit(waiting.pop(0), now=time.perf_counter() - t0)
-            if not sch.active:
-                if nxt < len(jobs):
-                    time.sleep(max(0.0, jobs[nxt].arrival - (time.perf_counter() - t0)))
-                continue
-            sch.step()
-            fin = sch.collect(now=time.perf_counter() - t0)
-            if log and fin:
-                log(f"t={time.perf_counter() - t0:.3f}s done {[j.id for j in fin]} active {len(sch.active)} "
-                    f"waiting {len(waiting)}")
-        wall = time.perf_counter() - t0
-        return jobs, wall
+        pool = ThreadPoolExecutor(self.prep_workers) if self.prep_workers > 1 else None
+        waiting = []
+        state = {"next": 0}
+        t0 = time.perf_counter()
+        clock = lambda: time.perf_counter() - t0
+
+        def release():
+            now = clock()
+            while state["next"] < len(jobs) and jobs[state["next"]].arrival <= now:
+                waiting.append(jobs[state["next"]])
+                state["next"] += 1
+            if pool is not None:
+                for job in waiting[:self.prep_ahead]:
+                    if job.prepared is None:
+                        job.prepared = pool.submit(sweep.prepare, job)
+
+        def admit_one():
+            sch.admit(waiting.pop(0), now=clock())
+
+        try:
+            while state["next"] < len(jobs) or waiting or sch.active:
+                release()
+                if not sch.active:
+                    if not waiting:
+                        time.sleep(max(0.0, jobs[state["next"]].arrival - clock()))
+                        continue
+                    admit_one()                     # an idle device takes work at once
+                    while waiting and sch.fits(waiting[0]):
+                        admit_one()
+                sch.launch()
+                # admit while the slice runs: arrivals are released as the clock passes them
+                while True:
+                    release()
+                    if not (waiting and sch.fits(waiting[0]) and sch.busy()):
+                        break
+                    admit_one()
+                sch.wait()
+                fin = sch.collect(now=clock())
+                if log and fin:
+                    log(f"t={clock():.3f}s done {[j.id for j in fin]} active {len(sch.active)} "
+                        f"waiting {len(waiting)}")
+        finally:
+            if pool is not None:
+                pool.shutdown(wait=True, cancel_futures=True)
+        return jobs, clock()
 
     def close(self):
         self.sch.close()
