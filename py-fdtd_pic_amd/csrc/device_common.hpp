@@ -84,7 +84,8 @@ template <typename R>
 struct BeamDev {
     int64_t launch_begin;  // the simulation's, copied: the parity without a load of the sim descriptor
     int32_t sim, N, np, G;
-    int32_t nwaves;      // push/count waves of this beam (grid-stride over cell groups)
+    int32_t nwaves;      // push/count waves of this beam (grid-stride over cell groups; one-cell
+                         // groups: whole workgroups of kPushWaves waves)
     int32_t nnode;       // scatter node blocks of this beam (grid-stride over nodes)
     R a;                 // q dt / (2 m c)
     R jscale;            // Z0 q w c: J' of one particle per unit u_x/gamma
@@ -95,13 +96,15 @@ struct BeamDev {
     int32_t* start[2];   // N+2
     int32_t* range[2];   // [cmin, cmax] of the live particles' cells (empty: cmin > cmax)
     int32_t* urange[2];  // union of the live ranges so far (where J and start[] can change)
-    // per-step record at the old (sorted) index, push/count -> scatter (the
-    // scatter re-runs the push for the new state, bitwise the same)
+    // multi-cell groups: per-step record at the old (sorted) index,
+    // push/count -> scatter (the scatter re-runs the push for the new state,
+    // bitwise the same)
     int32_t* trank;      // rank << 2 | (d + 1), d = new cell - old cell, rank among (cell, d)
     int32_t* park;       // [2]: the previous step's start[2] and start[N-2] (push/count -> scatter):
                          // particles below / at or above them are parked and in both buffers already
     R* hold;             // H^{n-1/2} on the live range +-1, copied by push/count for the scatter's
-                         // push (the field step of the same launch may overwrite that buffer)
+                         // push of multi-cell groups (the field step of the same launch may
+                         // overwrite that buffer)
     // per-cell results of the push/count pass, valid on the live range +-1
     int32_t *cup, *cdown;  // particles moving up / down out of the cell
     double* S;             // 4 per cell: J' sums onto nodes c-1, c, c+1, c+2

@@ -32,8 +32,9 @@ hipError_t launch_dft(const R* const* series, const int* lens, int nseries, cons
 template <typename R>
 hipError_t launch_energy(const EnergyJob<R>* jobs, int n, hipStream_t st);
 template <typename R>
-hipError_t launch_push(const SimDev<R>* sims, const BeamDev<R>* beams, const int2* push_items, int npush,
-                       long long launch_idx, hipStream_t st, hipEvent_t e0, hipEvent_t e1);
+hipError_t launch_push(const SimDev<R>* sims, const BeamDev<R>* beams, const int2* cell_items, int ncell,
+                       const int2* group_items, int ngroup, long long launch_idx, hipStream_t st, hipEvent_t e0,
+                       hipEvent_t e1);
 template <typename R>
 hipError_t launch_scatter(const SimDev<R>* sims, const BeamDev<R>* beams, const int2* scatter_blocks,
                           int nscatter, long long launch_idx, hipStream_t st, hipEvent_t e0, hipEvent_t e1);
@@ -617,9 +618,15 @@ static int attach_t(fdtd_ctx* c, int id, SimHost& s, const fdtd_beam_desc_t* bd)
     }
     b->G = std::min(std::max(G, 1), kPicGMax);
     const int span = b->range0[1] - b->range0[0] + 3;
-    // one-cell groups run two cells per wave
-    const int64_t cells_per_wave = b->G == 1 ? 2 : b->G;
-    b->nwaves = (int)std::min<int64_t>(32768, std::max<int64_t>(1, (span + cells_per_wave - 1) / cells_per_wave));
+    // one-cell groups run two cells per wave, in workgroups of kPushWaves
+    // waves that take blocks of 2 * kPushWaves cells together
+    if (b->G == 1) {
+        const int64_t cells_per_wg = 2 * kPushWaves;
+        b->nwaves = kPushWaves * (int)std::min<int64_t>(32768 / kPushWaves,
+                                                        std::max<int64_t>(1, (span + cells_per_wg - 1) / cells_per_wg));
+    } else {
+        b->nwaves = (int)std::min<int64_t>(32768, std::max<int64_t>(1, (span + b->G - 1) / b->G));
+    }
     b->nnode = std::min(64, std::max(1, (span + 8 + 255) / 256 + 1));
     // device block
     Layout L;
@@ -630,10 +637,11 @@ static int attach_t(fdtd_ctx* c, int id, SimHost& s, const fdtd_beam_desc_t* bd)
     for (int k = 0; k < 2; ++k) L.add(in);                       // id
     for (int k = 0; k < 2; ++k) L.add(cn);                       // start
     for (int k = 0; k < 4; ++k) L.add(2 * sizeof(int32_t));      // range x2, urange x2
-    L.add(in);                                                   // trank
+    const bool one = b->G == 1;
+    L.add(one ? 0 : in);                                         // trank (multi-cell groups)
     L.add(cn); L.add(cn);                                        // cup cdown
     L.add(4 * sizeof(double) * (N + 2));                         // S
-    L.add(sizeof(R) * (N + 2));                                  // hold
+    L.add(one ? 0 : sizeof(R) * (N + 2));                        // hold (multi-cell groups)
     L.add(2 * sizeof(int32_t));                                  // park
     for (int k = 0; k < L.n; ++k) b->off[k] = L.off[k];
     b->bytes = L.total;
@@ -689,20 +697,30 @@ static int rebuild_t(fdtd_ctx* c) {
     }
     // particle beams of the active simulations
     std::vector<BeamDev<R>> bdev;
-    std::vector<int2> pblocks, pitems;
+    std::vector<int2> pblocks, pitems, citems;
     for (int i : c->tile_sims) {
         BeamHost* bh = i < (int)c->beams.size() ? c->beams[i] : nullptr;
         if (!bh) continue;
         const int bi = (int)bdev.size();
         bdev.push_back(fill_beam_dev<R>(*bh, c->d_err, c->sims[i]->launch_begin));
-        for (int w = 0; w < bh->nwaves; ++w) pitems.push_back(make_int2(bi, w));
-        for (int64_t p = 0; p < bh->np; p += 256 * kScatterPer) pblocks.push_back(make_int2(bi, (int)p));
+        if (bh->G == 1) {
+            // one item (beam, workgroup index) per workgroup of pic_push_cell
+            for (int g = 0; g < bh->nwaves / kPushWaves; ++g) citems.push_back(make_int2(bi, g));
+        } else {
+            for (int w = 0; w < bh->nwaves; ++w) pitems.push_back(make_int2(bi, w));
+        }
+        // particle blocks for multi-cell groups only: push/count moves the
+        // particles of one-cell groups, and the node blocks the rest
+        if (bh->G != 1)
+            for (int64_t p = 0; p < bh->np; p += 256 * kScatterPer) pblocks.push_back(make_int2(bi, (int)p));
         for (int k = 0; k < bh->nnode; ++k) pblocks.push_back(make_int2(bi, -(k + 1)));
     }
     while (pitems.size() % kPushWaves) pitems.push_back(make_int2(-1, 0));   // whole push/count workgroups
     c->has_pic = !bdev.empty();
     c->npblocks = (int)pblocks.size();
     c->npitems = (int)pitems.size();
+    c->npcell = (int)citems.size();
+    pitems.insert(pitems.end(), citems.begin(), citems.end());             // the one-cell items follow
     std::vector<Upload> ups;
     if (c->has_pic) {
         ups.push_back({&c->d_beams, &c->d_beams_cap, bdev.data(), sizeof(BeamDev<R>) * bdev.size()});
@@ -978,12 +996,15 @@ int fdtd_run(fdtd_ctx* c, int64_t nsteps) {
                                                  std::to_string(c->launch_idx) + ": " + hipGetErrorString(stale));
             hipError_t e = c->prec == FDTD_F64
                 ? launch_push<double>((const SimDev<double>*)c->d_sims, (const BeamDev<double>*)c->d_beams,
-                                      c->d_pitems, c->npitems, c->launch_idx, c->st, e0, e1)
+                                      c->d_pitems + c->npitems, c->npcell, c->d_pitems, c->npitems,
+                                      c->launch_idx, c->st, e0, e1)
                 : launch_push<float>((const SimDev<float>*)c->d_sims, (const BeamDev<float>*)c->d_beams,
-                                     c->d_pitems, c->npitems, c->launch_idx, c->st, e0, e1);
+                                     c->d_pitems + c->npitems, c->npcell, c->d_pitems, c->npitems,
+                                     c->launch_idx, c->st, e0, e1);
             if (e != hipSuccess)
                 return fdtd_fail(FDTD_E_HIP, std::string("push/count launch ") + std::to_string(c->launch_idx) +
-                                                 " (" + std::to_string(c->npitems) + " waves): " + hipGetErrorString(e));
+                                                 " (" + std::to_string(c->npcell) + " one-cell workgroups, " +
+                                                 std::to_string(c->npitems) + " waves): " + hipGetErrorString(e));
         }
         int rc;
         if (!c->has_pic && c->profile > 0) {
@@ -1358,7 +1379,18 @@ static int read_beam(fdtd_ctx* c, SimHost& s, BeamHost& b, double* x, double* u,
     if (rc) return rc;
     const char* blk = (const char*)b.block;
     const size_t rn = sizeof(R) * n, in = sizeof(int32_t) * n;
-    HIPCHK(hipMemcpy(cl.data(), blk + b.off[BO_CELL + par], in, hipMemcpyDeviceToHost));
+    if (b.G == 1) {
+        // one-cell beams keep no cell array: cell c holds [start[c], start[c+1])
+        std::vector<int32_t> st(N + 2);
+        HIPCHK(hipMemcpy(st.data(), blk + b.off[BO_START + par], sizeof(int32_t) * (N + 2), hipMemcpyDeviceToHost));
+        for (int cc = 0; cc <= N + 1; ++cc) {
+            const int64_t e = cc <= N ? st[cc + 1] : n;
+            if (st[cc] < 0 || e > n || e < st[cc]) return fdtd_fail(FDTD_E_STATE, "corrupt start[] table");
+            for (int64_t k = st[cc]; k < e; ++k) cl[k] = cc;
+        }
+    } else {
+        HIPCHK(hipMemcpy(cl.data(), blk + b.off[BO_CELL + par], in, hipMemcpyDeviceToHost));
+    }
     HIPCHK(hipMemcpy(id.data(), blk + b.off[BO_ID + par], in, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(fr.data(), blk + b.off[BO_FRAC + par], rn, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(ux.data(), blk + b.off[BO_UX + par], rn, hipMemcpyDeviceToHost));

@@ -92,9 +92,9 @@ struct fdtd_ctx {
     size_t d_beams_cap = 0;
     int2* d_pblocks = nullptr;               // scatter blocks (beam, first particle | -(node block + 1))
     size_t d_pblocks_cap = 0;
-    int2* d_pitems = nullptr;                // push/count waves (beam, wave index)
-    size_t d_pitems_cap = 0;
-    int npblocks = 0, npitems = 0;
+    int2* d_pitems = nullptr;                // push/count items: npitems multi-cell waves (beam, wave
+    size_t d_pitems_cap = 0;                 // index), then npcell one-cell workgroups (beam, workgroup)
+    int npblocks = 0, npitems = 0, npcell = 0;
     bool has_pic = false;                    // some active simulation carries a beam
     bool fuse_pic = true;                    // scatter in the same launch as the last tile group's field step
     // bound on queued work: an event every kThrottleEvery launches; the host

@@ -8,18 +8,39 @@ namespace fdtd {
 
 // With profiling events the kernels go through hipExtLaunchKernelGGL, which
 // stamps the events from their own dispatch packets.
+//
+// Push/count of one step: the one-cell beams' kernel (one item per
+// workgroup) and the multi-cell beams' kernel (one item per wave, padded to
+// whole workgroups), e0 at the start of the first launch, e1 at the end of
+// the last.
 template <typename R>
-hipError_t launch_push(const SimDev<R>* sims, const BeamDev<R>* beams, const int2* push_items, int npush,
-                       long long launch_idx, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
-    if (npush <= 0) return hipSuccess;
-    const int nb = (npush + kPushWaves - 1) / kPushWaves;    // items are padded to whole blocks
-    if (e0 || e1)
-        hipExtLaunchKernelGGL((pic_push_count<R>), dim3(nb), dim3(64 * kPushWaves), 0, st, e0, e1, 0, sims, beams,
-                              push_items, launch_idx);
-    else
-        hipLaunchKernelGGL((pic_push_count<R>), dim3(nb), dim3(64 * kPushWaves), 0, st, sims, beams, push_items,
-                           launch_idx);
-    return hipGetLastError();
+hipError_t launch_push(const SimDev<R>* sims, const BeamDev<R>* beams, const int2* cell_items, int ncell,
+                       const int2* group_items, int ngroup, long long launch_idx, hipStream_t st, hipEvent_t e0,
+                       hipEvent_t e1) {
+    const bool ev = e0 || e1;
+    if (ncell > 0) {
+        hipEvent_t a0 = e0, a1 = ngroup > 0 ? nullptr : e1;
+        if (ev)
+            hipExtLaunchKernelGGL((pic_push_cell<R>), dim3(ncell), dim3(64 * kPushWaves), 0, st, a0, a1, 0, sims,
+                                  beams, cell_items, launch_idx);
+        else
+            hipLaunchKernelGGL((pic_push_cell<R>), dim3(ncell), dim3(64 * kPushWaves), 0, st, sims, beams,
+                               cell_items, launch_idx);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (ngroup > 0) {
+        const int nb = (ngroup + kPushWaves - 1) / kPushWaves;    // items are padded to whole blocks
+        hipEvent_t a0 = ncell > 0 ? nullptr : e0, a1 = e1;
+        if (ev)
+            hipExtLaunchKernelGGL((pic_push_count<R>), dim3(nb), dim3(64 * kPushWaves), 0, st, a0, a1, 0, sims,
+                                  beams, group_items, launch_idx);
+        else
+            hipLaunchKernelGGL((pic_push_count<R>), dim3(nb), dim3(64 * kPushWaves), 0, st, sims, beams,
+                               group_items, launch_idx);
+        return hipGetLastError();
+    }
+    return hipSuccess;
 }
 
 // The scatter reads nothing the field step of the same launch writes
@@ -38,10 +59,10 @@ hipError_t launch_scatter(const SimDev<R>* sims, const BeamDev<R>* beams, const 
     return hipGetLastError();
 }
 
-template hipError_t launch_push<double>(const SimDev<double>*, const BeamDev<double>*, const int2*, int, long long,
-                                        hipStream_t, hipEvent_t, hipEvent_t);
-template hipError_t launch_push<float>(const SimDev<float>*, const BeamDev<float>*, const int2*, int, long long,
-                                       hipStream_t, hipEvent_t, hipEvent_t);
+template hipError_t launch_push<double>(const SimDev<double>*, const BeamDev<double>*, const int2*, int,
+                                        const int2*, int, long long, hipStream_t, hipEvent_t, hipEvent_t);
+template hipError_t launch_push<float>(const SimDev<float>*, const BeamDev<float>*, const int2*, int, const int2*,
+                                       int, long long, hipStream_t, hipEvent_t, hipEvent_t);
 template hipError_t launch_scatter<double>(const SimDev<double>*, const BeamDev<double>*, const int2*, int,
                                            long long, hipStream_t, hipEvent_t, hipEvent_t);
 template hipError_t launch_scatter<float>(const SimDev<float>*, const BeamDev<float>*, const int2*, int, long long,

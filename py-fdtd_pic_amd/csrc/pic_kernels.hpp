@@ -18,16 +18,24 @@
 //                   sums of the CIC weights of J' at x^{n+1/2} onto nodes
 //                   c-1 .. c+2 (one-cell groups: per-lane sums and a wave
 //                   scan; larger groups: segmented DPP scans; a fixed order
-//                   either way).
-//   pic_scatter     one thread per particle: the push again (bitwise the
-//                   same: no contraction in push_one), then the new index
+//                   either way).  The new index of a particle is
 //                      d=-1: start[c] - up(c-1) + r
 //                      d= 0: start[c] + down(c) + r
 //                      d=+1: start[c+1] - up(c) + down(c+1) + r
-//                   (parked particles keep theirs); node threads form
-//                   J'[m] = S+2(m-2) + S+1(m-1) + S0(m) + S-1(m+1), the new
-//                   start[] and the next live range (a single writer each:
-//                   the first and the last non-empty cell).
+//                   One-cell groups (dense beams; two cells per wave) write
+//                   the new state of every particle whose index needs only
+//                   the wave's own counts, i.e. all but the movers into a
+//                   neighbouring wave's cells, which are staged; multi-cell
+//                   groups write each particle's (rank, d) record instead.
+//   pic_scatter     node threads: J'[m] = S+2(m-2) + S+1(m-1) + S0(m) +
+//                   S-1(m+1), the new start[] and the next live range (a
+//                   single writer each: the first and the last non-empty
+//                   cell); one-cell groups: the staged movers to their new
+//                   index, and the particles parked at the previous step
+//                   copied to the other buffer.  Multi-cell groups add one
+//                   thread per particle: the push again (bitwise the same:
+//                   no contraction in push_one) and the new state at the new
+//                   index (parked particles keep theirs).
 // Absorbed particles stay in the array (cells 1 and N-2), so the particle
 // count is constant and no global scan is needed.
 #pragma once
@@ -142,6 +150,11 @@ constexpr int kBatch = 2;           // chunks of 64 particles fetched per memory
 // global.
 template <typename T>
 using gptr = __attribute__((address_space(1))) T*;
+// read-only descriptors in the constant address space
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T*;
+template <typename T>
+using cref = const __attribute__((address_space(4))) T&;
 template <typename T>
 __device__ __forceinline__ gptr<T> gp(T* p) { return (gptr<T>)p; }
 template <typename T>
@@ -268,21 +281,69 @@ __device__ __forceinline__ void push_cell(const PushArgs<R>& pa, const CellField
     push_at<R>(pa, c, f, ux, uy, uz, Eg, Hg, d, off, a0, a1);
 }
 
-// One 64-particle chunk [base, base + 64) of a one-cell group whose
-// particles are [.., phi) (lane: the particle's offset; c, f, u: its state):
-// the push, its move class d, its rank among the cell's class d so far
-// (scalar carries n0, n1, n2 plus the earlier lanes of the class), the rank
-// record, and its CIC weights added to the lane's FP64 sums onto nodes
-// c-1 .. c+2.
+// A particle of a one-cell group through the push: its state (old, then
+// new) and dr = rank << 2 | (d + 1), d = new cell - old cell, rank among its
+// cell's class d.  The cell itself is not kept: in a one-cell group every
+// particle of a cell's index range is in that cell, and the new cell is the
+// old one plus d (the graveyard clamps give c - 1 = 1 and c + 1 = N - 2 too).
 template <typename R>
-__device__ __forceinline__ void g1_chunk(const PushArgs<R>& pa, const CellFields<R>& cf, gptr<int32_t> trank,
-                                         int base, int phi, int lane, int c, R f, R ux, R uy, R uz, int& n0,
-                                         int& n1, int& n2, double (&acc)[4]) {
-    const int i = base + lane;
-    const bool valid = i < phi;
+struct Pushed {
+    int id, dr;
+    R f, ux, uy, uz;
+};
+
+// Particle state arrays of one buffer (one-cell beams keep no cell array:
+// a particle's cell follows from its index and start[]).
+template <typename R>
+struct StateOut {
+    gptr<int32_t> id;
+    gptr<R> f, ux, uy, uz;
+};
+
+template <typename R>
+struct StateIn {
+    gptr<const int32_t> id;
+    gptr<const R> f, ux, uy, uz;
+};
+
+// One particle's state from index s of one buffer to index d of another.
+template <typename R>
+__device__ __forceinline__ void copy_state(const StateIn<R>& a, unsigned s, const StateOut<R>& b, unsigned d) {
+    sto(b.id, d, ldo(a.id, s));
+    sto(b.f, d, ldo(a.f, s));
+    sto(b.ux, d, ldo(a.ux, s));
+    sto(b.uy, d, ldo(a.uy, s));
+    sto(b.uz, d, ldo(a.uz, s));
+}
+
+// Loads of kBatch 64-particle chunks from index base: unconditional at an
+// in-range index (np >= 1); the callers select on use.
+template <typename R>
+__device__ __forceinline__ void g1_load(gptr<const int32_t> pid, gptr<const R> pf, gptr<const R> px,
+                                        gptr<const R> py, gptr<const R> pz, int base, int np, int lane,
+                                        Pushed<R> (&s)[kBatch]) {
+#pragma unroll
+    for (int q = 0; q < kBatch; ++q) {
+        const unsigned i = (unsigned)min(base + 64 * q + lane, np - 1);
+        s[q].id = ldo(pid, i);
+        s[q].f = ldo(pf, i);
+        s[q].ux = ldo(px, i);
+        s[q].uy = ldo(py, i);
+        s[q].uz = ldo(pz, i);
+    }
+}
+
+// One 64-particle chunk of a one-cell group (valid: the lane holds one of the
+// cell's particles): the push, its move class d, its rank among the cell's
+// class d so far (scalar carries n0, n1, n2 plus the earlier lanes of the
+// class) and, when SUMS, its CIC weights added to the lane's FP64 sums onto
+// nodes c-1 .. c+2.
+template <typename R, bool SUMS>
+__device__ __forceinline__ void g1_push(const PushArgs<R>& pa, const CellFields<R>& cf, int c, bool valid,
+                                        Pushed<R>& s, int& n0, int& n1, int& n2, double (&acc)[4]) {
     int d = 0, off = 1;
     R a0 = R(0), a1 = R(0);
-    if (valid) push_cell<R>(pa, cf, c, f, ux, uy, uz, d, off, a0, a1);
+    if (valid) push_cell<R>(pa, cf, c, s.f, s.ux, s.uy, s.uz, d, off, a0, a1);
     const unsigned long long bd0 = __ballot(valid && d == -1);
     const unsigned long long bd1 = __ballot(valid && d == 0);
     const unsigned long long bd2 = __ballot(valid && d == 1);
@@ -290,18 +351,53 @@ __device__ __forceinline__ void g1_chunk(const PushArgs<R>& pa, const CellFields
     const int carry = d < 0 ? n0 : (d == 0 ? n1 : n2);
     const int r = carry + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mine >> 32),
                                                          __builtin_amdgcn_mbcnt_lo((unsigned)mine, 0u));
-    if (valid) sto(trank, (unsigned)i, (r << 2) | (d + 1));
-    const R s0 = off == 0 ? a0 : R(0);
-    const R s1 = off == 0 ? a1 : (off == 1 ? a0 : R(0));
-    const R s2 = off == 1 ? a1 : (off == 2 ? a0 : R(0));
-    const R s3 = off == 2 ? a1 : R(0);
-    acc[0] += (double)s0;
-    acc[1] += (double)s1;
-    acc[2] += (double)s2;
-    acc[3] += (double)s3;
+    s.dr = (r << 2) | (d + 1);
+    if (SUMS) {
+        const R s0 = off == 0 ? a0 : R(0);
+        const R s1 = off == 0 ? a1 : (off == 1 ? a0 : R(0));
+        const R s2 = off == 1 ? a1 : (off == 2 ? a0 : R(0));
+        const R s3 = off == 2 ? a1 : R(0);
+        acc[0] += (double)s0;
+        acc[1] += (double)s1;
+        acc[2] += (double)s2;
+        acc[3] += (double)s3;
+    }
     n0 += __popcll(bd0);
     n1 += __popcll(bd1);
     n2 += __popcll(bd2);
+}
+
+// The new state of one particle at index i of o (an index outside [0, np)
+// sets *err instead; never expected).
+template <typename R>
+__device__ __forceinline__ void g1_put(const StateOut<R>& o, int i, int np, const Pushed<R>& s,
+                                       gptr<int32_t> err) {
+    if ((unsigned)i >= (unsigned)np) {
+        err[0] = 1;
+        return;
+    }
+    sto(o.id, (unsigned)i, s.id);
+    sto(o.f, (unsigned)i, s.f);
+    sto(o.ux, (unsigned)i, s.ux);
+    sto(o.uy, (unsigned)i, s.uy);
+    sto(o.uz, (unsigned)i, s.uz);
+}
+
+// The new state of kBatch chunks of one cell (its particles [.., phi), the
+// chunks from index base): the particle of class k = d + 1 and rank r goes
+// to index x_k + r of nxt, or of stg when k == staged (-1: none).
+template <typename R>
+__device__ __forceinline__ void g1_put_cell(const StateOut<R>& nxt, const StateOut<R>& stg,
+                                            const Pushed<R> (&s)[kBatch], int base, int phi, int lane, int x0,
+                                            int x1, int x2, int staged, int np, gptr<int32_t> err) {
+#pragma unroll
+    for (int q = 0; q < kBatch; ++q) {
+        if (base + 64 * q + lane < phi) {
+            const int k = s[q].dr & 3, i = (k == 0 ? x0 : (k == 1 ? x1 : x2)) + (s[q].dr >> 2);
+            if (k == staged) g1_put<R>(stg, i, np, s[q], err);
+            else g1_put<R>(nxt, i, np, s[q], err);
+        }
+    }
 }
 
 // The fields of a multi-cell group, staged in the wave's LDS once per group:
@@ -332,9 +428,25 @@ __device__ __forceinline__ void push_tab(const PushArgs<R>& pa, const R* Et, con
     push_at<R>(pa, c, f, ux, uy, uz, Eg, Hg, d, off, a0, a1);
 }
 
-// kPushWaves (device_common.hpp) independent waves per workgroup, each with
-// its own LDS tables; item kPushWaves * block + wave, items with x < 0 are
-// padding.
+// The first wave of a beam's push/count resets the next step's live range
+// (the scatter's node threads fill it), carries the union range over and
+// records the previous step's graveyard bounds (start[po] still holds that
+// step's start[]; this step's scatter overwrites it): a particle parked then
+// was copied to the other buffer then and keeps its index.
+template <typename R>
+__device__ __forceinline__ void begin_step(const BeamDev<R>& b, int par, int N) {
+    const int po = par ^ 1;
+    gp(b.range[po])[0] = N;
+    gp(b.range[po])[1] = -1;
+    gp(b.urange[po])[0] = gp((const int32_t*)b.urange[par])[0];
+    gp(b.urange[po])[1] = gp((const int32_t*)b.urange[par])[1];
+    gp(b.park)[0] = gp((const int32_t*)b.start[po])[2];
+    gp(b.park)[1] = gp((const int32_t*)b.start[po])[N - 2];
+}
+
+// Multi-cell groups (G > 1): kPushWaves independent waves per workgroup,
+// each with its own LDS tables; item kPushWaves * block + wave, items with
+// x < 0 are padding.
 
 template <typename R>
 __global__ __launch_bounds__(64 * kPushWaves) void pic_push_count(const SimDev<R>* __restrict__ sims,
@@ -356,137 +468,26 @@ __global__ __launch_bounds__(64 * kPushWaves) void pic_push_count(const SimDev<R
     const BeamDev<R>& b = beams[it.x];
     const SimDev<R>& sd = sims[b.sim];
     const int par = (int)((launch_idx - b.launch_begin) & 1);
-    const int po = par ^ 1;
     const int lane = threadIdx.x & 63;
-    // every descriptor field is read here, ahead of the range store below
+    // every descriptor field is read here, ahead of the kernel's first store
     const int N = b.N, np = b.np, nwaves = b.nwaves;
     const int G = max(1, min(b.G, kPicGMax));
     const PushArgs<R> pa{gp((const R*)sd.E[par]), gp((const R*)sd.H[par]), gp((const R*)sd.H[par ^ 1]),
                          b.a, b.jscale, b.Sc, N};
-    const gptr<int32_t> rng_p = gp(b.range[par]), rng_o = gp(b.range[po]);
-    const gptr<int32_t> urng_p = gp(b.urange[par]), urng_o = gp(b.urange[po]);
+    const gptr<const int32_t> rng_p = gp((const int32_t*)b.range[par]);
     const gptr<const int32_t> st = gp((const int32_t*)b.start[par]);
     const gptr<const int32_t> cell = gp((const int32_t*)b.cell[par]);
     const gptr<const R> pf = gp((const R*)b.frac[par]), px = gp((const R*)b.ux[par]);
     const gptr<const R> py = gp((const R*)b.uy[par]), pz = gp((const R*)b.uz[par]);
     const gptr<int32_t> trank = gp(b.trank), cup = gp(b.cup), cdown = gp(b.cdown);
     const gptr<R> hold = gp(b.hold);
-    const gptr<int32_t> park = gp(b.park);
-    const gptr<const int32_t> st_prev = gp((const int32_t*)b.start[po]);
     const gptr<double> Sg = gp(b.S);
     const int cmin = rng_p[0], cmax = rng_p[1];
-    if (it.y == 0 && lane == 0) {            // next step's ranges start from here
-        rng_o[0] = N;
-        rng_o[1] = -1;
-        urng_o[0] = urng_p[0];
-        urng_o[1] = urng_p[1];
-        // the previous step's graveyard bounds (start[po] still holds that
-        // step's start[]; this step's scatter overwrites it): a particle
-        // parked then was copied to the other buffer then and keeps its index
-        park[0] = st_prev[2];
-        park[1] = st_prev[N - 2];
-    }
+    if (it.y == 0 && lane == 0) begin_step<R>(b, par, N);
     if (cmin > cmax) return;
     const int lo_cell = max(cmin - 1, 0), hi_cell = min(cmax + 1, N);   // counts cover the range +-1
     const int ngroups = (hi_cell - lo_cell + G) / G;
     auto clampi = [&](int v) { return min(max(v, 0), np); };
-    if (G == 1) {
-        // One-cell groups, two adjacent cells per wave with both cells' loads
-        // in flight together.  Every particle of a cell's range is in that
-        // cell, so the per-class carries are wave-uniform scalars, a
-        // particle's rank is its class carry plus the earlier lanes of its
-        // class (mbcnt), and no LDS table is needed.  Each cell keeps its own
-        // carries, lane sums and reduction, in chunk order: the same ranks,
-        // counts and sums (same order) as one cell per wave.
-        const int npairs = (hi_cell - lo_cell + 2) >> 1;
-        for (int gi = it.y; gi < npairs; gi += nwaves) {
-            const int gA = lo_cell + 2 * gi, gB = gA + 1;
-            const bool hasB = gB <= hi_cell;
-            // particles of live cells only: the graveyard cells 1 and N-2 can
-            // hold most of a beam once it has drifted out, and their particles
-            // neither move nor deposit (their cells' counts stay 0 below).
-            // Three bounds in one load: start of A (lane 0), start of B
-            // (lane 1), end of B (lane 2; an empty B when the pair has one cell).
-            const int bi = lane == 0 ? gA : (lane == 1 || !hasB ? gB : gB + 1);
-            const int sb = ldo(st, (unsigned)min(max(bi, 2), N - 2));
-            // the cells' fields, loaded alongside the bounds (clamped cells
-            // when a cell holds no live particle)
-            const CellFields<R> cfA = cell_fields<R>(pa, min(max(gA, 2), N - 3));
-            const CellFields<R> cfB = cell_fields<R>(pa, min(max(gB, 2), N - 3));
-            const int pA = clampi(__builtin_amdgcn_readlane(sb, 0));
-            const int pB = max(pA, clampi(__builtin_amdgcn_readlane(sb, 1)));
-            const int pE = max(pB, clampi(__builtin_amdgcn_readlane(sb, 2)));
-            int nA0 = 0, nA1 = 0, nA2 = 0, nB0 = 0, nB1 = 0, nB2 = 0;   // moving down / staying / up so far
-            double accA[4] = {0.0, 0.0, 0.0, 0.0}, accB[4] = {0.0, 0.0, 0.0, 0.0};
-            const int len = max(pB - pA, pE - pB);
-            // particle loads one batch ahead (cells of more than 64 * kBatch
-            // particles), unconditional at in-range indices (np >= 1)
-            int nca[kBatch], ncb[kBatch];
-            R nfa[kBatch], nxa[kBatch], nya[kBatch], nza[kBatch];
-            R nfb[kBatch], nxb[kBatch], nyb[kBatch], nzb[kBatch];
-            auto fetch = [&](int o) {
-#pragma unroll
-                for (int q = 0; q < kBatch; ++q) {
-                    const unsigned ia = (unsigned)min(pA + o + 64 * q + lane, np - 1);
-                    const unsigned ib = (unsigned)min(pB + o + 64 * q + lane, np - 1);
-                    nca[q] = ldo(cell, ia);
-                    nfa[q] = ldo(pf, ia);
-                    nxa[q] = ldo(px, ia);
-                    nya[q] = ldo(py, ia);
-                    nza[q] = ldo(pz, ia);
-                    ncb[q] = ldo(cell, ib);
-                    nfb[q] = ldo(pf, ib);
-                    nxb[q] = ldo(px, ib);
-                    nyb[q] = ldo(py, ib);
-                    nzb[q] = ldo(pz, ib);
-                }
-            };
-            if (len > 0) fetch(0);
-            for (int o = 0; o < len; o += 64 * kBatch) {
-                int ca[kBatch], cb[kBatch];
-                R fa[kBatch], xa[kBatch], ya[kBatch], za[kBatch];
-                R fb[kBatch], xb[kBatch], yb[kBatch], zb[kBatch];
-#pragma unroll
-                for (int q = 0; q < kBatch; ++q) {
-                    ca[q] = nca[q]; fa[q] = nfa[q]; xa[q] = nxa[q]; ya[q] = nya[q]; za[q] = nza[q];
-                    cb[q] = ncb[q]; fb[q] = nfb[q]; xb[q] = nxb[q]; yb[q] = nyb[q]; zb[q] = nzb[q];
-                }
-                if (o + 64 * kBatch < len) fetch(o + 64 * kBatch);
-#pragma unroll
-                for (int q = 0; q < kBatch; ++q)
-                    if (pA + o + 64 * q < pB)
-                        g1_chunk<R>(pa, cfA, trank, pA + o + 64 * q, pB, lane, ca[q], fa[q], xa[q], ya[q], za[q],
-                                    nA0, nA1, nA2, accA);
-#pragma unroll
-                for (int q = 0; q < kBatch; ++q)
-                    if (pB + o + 64 * q < pE)
-                        g1_chunk<R>(pa, cfB, trank, pB + o + 64 * q, pE, lane, cb[q], fb[q], xb[q], yb[q], zb[q],
-                                    nB0, nB1, nB2, accB);
-            }
-            wave_scan4(accA);                   // lane 63 holds the cells' sums
-            wave_scan4(accB);
-            if (lane == 63) {
-#pragma unroll
-                for (int w = 0; w < 4; ++w) Sg[4 * (size_t)gA + w] = accA[w];
-                if (hasB) {
-#pragma unroll
-                    for (int w = 0; w < 4; ++w) Sg[4 * (size_t)gB + w] = accB[w];
-                }
-            }
-            if (lane == 0) {
-                // H^{n-1/2} of the pair's cells for the scatter's push
-                cup[gA] = nA2;
-                cdown[gA] = nA0;
-                hold[gA] = pa.Hp[gA];
-                if (hasB) {
-                    cup[gB] = nB2;
-                    cdown[gB] = nB0;
-                    hold[gB] = pa.Hp[gB];
-                }
-            }
-        }
-        return;
-    }
     for (int gi = it.y; gi < ngroups; gi += nwaves) {
         const int g0 = lo_cell + gi * G;
         const int g1 = min(g0 + G, hi_cell + 1);
@@ -587,6 +588,203 @@ __global__ __launch_bounds__(64 * kPushWaves) void pic_push_count(const SimDev<R
     }
 }
 
+// The pair's eight FP64 CIC sums (a: cell A onto nodes c-1 .. c+2, b: cell
+// B) over the wave's 64 lanes, in a fixed order, through the wave's own LDS
+// rows red[0..8]: lane l adds the eight values of sum l/8 held by lanes
+// 8(l%8) .. 8(l%8)+7, then lane k < 8 adds the eight partials of sum k
+// (about 20 VALU operations against ~150 for two DPP wave scans).  The wave
+// orders its own LDS accesses, so wavefront fences suffice.
+__device__ __forceinline__ double pair_sums(double (*red)[64], const double (&a)[4], const double (&b)[4],
+                                            int lane) {
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        red[w][lane] = a[w];
+        red[4 + w][lane] = b[w];
+    }
+    wave_sync();
+    const int k = lane >> 3, p = lane & 7;
+    double t = red[k][8 * p];
+#pragma unroll
+    for (int j = 1; j < 8; ++j) t += red[k][8 * p + j];
+    red[8][lane] = t;
+    wave_sync();
+    double u = 0.0;
+    if (lane < 8) {
+        u = red[8][8 * lane];
+#pragma unroll
+        for (int j = 1; j < 8; ++j) u += red[8][8 * lane + j];
+    }
+    wave_sync();
+    return u;
+}
+
+// One-cell groups (G = 1): one item (beam, workgroup index) per workgroup.
+template <typename R>
+__global__ __launch_bounds__(64 * kPushWaves) void pic_push_cell(const SimDev<R>* __restrict__ sims,
+                                                                 const BeamDev<R>* __restrict__ beams,
+                                                                 const int2* __restrict__ items,
+                                                                 long long launch_idx) {
+    __shared__ int xcnt[2][kPushWaves][2];      // per pair, down(A) and up(B) (double buffered)
+    __shared__ double red_all[kPushWaves][9][64];   // per wave: the CIC sum reduction (pair_sums)
+    // wave-uniform by construction; readfirstlane lets the compiler see it, so
+    // the item and the descriptors come in through scalar loads
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int2 it = items[blockIdx.x];
+    // the descriptors through the constant address space (the kernels never
+    // write them), so their fields can be loaded where they are used, after
+    // stores too, instead of all being held from the start
+    const cref<BeamDev<R>> b = ((cptr<BeamDev<R>>)beams)[it.x];
+    const cref<SimDev<R>> sd = ((cptr<SimDev<R>>)sims)[b.sim];
+    const int par = (int)((launch_idx - b.launch_begin) & 1);
+    const int po = par ^ 1;
+    const int lane = threadIdx.x & 63;
+    const int N = b.N, np = b.np, nwaves = b.nwaves;
+    const PushArgs<R> pa{gp((const R*)sd.E[par]), gp((const R*)sd.H[par]), gp((const R*)sd.H[par ^ 1]),
+                         b.a, b.jscale, b.Sc, N};
+    const gptr<const int32_t> rng_p = gp((const int32_t*)b.range[par]);
+    const gptr<const int32_t> st = gp((const int32_t*)b.start[par]);
+    const gptr<const int32_t> pid = gp((const int32_t*)b.id[par]);
+    const gptr<const R> pf = gp((const R*)b.frac[par]), px = gp((const R*)b.ux[par]);
+    const gptr<const R> py = gp((const R*)b.uy[par]), pz = gp((const R*)b.uz[par]);
+    const int cmin = rng_p[0], cmax = rng_p[1];
+    if (it.y == 0 && wv == 0 && lane == 0) begin_step<R>(beams[it.x], par, N);
+    if (cmin > cmax) return;
+    const int lo_cell = max(cmin - 1, 0), hi_cell = min(cmax + 1, N);   // counts cover the range +-1
+    // One-cell groups.  A workgroup takes blocks of kPushWaves pairs of
+    // adjacent cells (2 * kPushWaves cells), one pair (A, B) per wave with
+    // both cells' loads in flight together.  Every particle of a cell's
+    // range is in that cell, so the per-class carries are wave-uniform
+    // scalars and a particle's rank is its class carry plus the earlier
+    // lanes of its class (mbcnt); no LDS table is needed.  Each cell keeps
+    // its own carries, lane sums and reduction, in chunk order.
+    //
+    // The new state is written here.  A particle's new index needs its
+    // own cell's counts and one neighbour's: up(c-1) for d = -1, down(c+1)
+    // for d = +1.  The waves of the workgroup exchange their pairs'
+    // totals through LDS, so every particle whose new cell lies in the
+    // block goes straight to the other buffer; the down-movers of the
+    // block's first cell and the up-movers of its last cell are staged at
+    // start[c] + rank (inside the cell's own old range) and moved by the
+    // scatter's node threads.  The kPushWaves waves of a workgroup share
+    // its item (beam, workgroup index), so all of them run the same block
+    // loop and reach every barrier.
+    const int npairs = (hi_cell - lo_cell + 2) >> 1;
+    const int nblk = (npairs + kPushWaves - 1) / kPushWaves;
+    const int nwg = max(nwaves / kPushWaves, 1);
+    int dbuf = 0;
+    for (int bk = it.y; bk < nblk; bk += nwg, dbuf ^= 1) {
+        const int gi = bk * kPushWaves + wv;
+        const bool pv = gi < npairs;
+        const int gA = lo_cell + 2 * gi, gB = gA + 1;
+        const bool hasB = pv && gB <= hi_cell;
+        // particles of live cells only: the graveyard cells 1 and N-2 can
+        // hold most of a beam once it has drifted out, and their particles
+        // neither move nor deposit (their cells' counts stay 0 below).
+        // Three bounds in one load: start of A (lane 0), start of B
+        // (lane 1), end of B (lane 2; an empty B when the pair has one cell).
+        const int bi = lane == 0 ? gA : (lane == 1 || !hasB ? gB : gB + 1);
+        const int sb = ldo(st, (unsigned)min(max(bi, 2), N - 2));
+        // the cells' fields, loaded alongside the bounds (clamped cells
+        // when a cell holds no live particle)
+        const CellFields<R> cfA = cell_fields<R>(pa, min(max(gA, 2), N - 3));
+        const CellFields<R> cfB = cell_fields<R>(pa, min(max(gB, 2), N - 3));
+        const int pA = min(max(__builtin_amdgcn_readlane(sb, 0), 0), np);
+        const int pB = pv ? max(pA, min(max(__builtin_amdgcn_readlane(sb, 1), 0), np)) : pA;
+        const int pE = pv ? max(pB, min(max(__builtin_amdgcn_readlane(sb, 2), 0), np)) : pA;
+        double accA[4] = {0.0, 0.0, 0.0, 0.0}, accB[4] = {0.0, 0.0, 0.0, 0.0};
+        const int len = max(pB - pA, pE - pB);
+        // cells of up to 64 * kBatch live particles (the common case): one
+        // pass, the batch held in registers across the exchange; larger
+        // ones: a counting pass, then a pass that pushes again (bitwise the
+        // same) and writes
+        const bool one = len <= 64 * kBatch;
+        int tA0 = 0, tA2 = 0, tB0 = 0, tB2 = 0;        // the pair's totals: A down, A up, B down, B up
+        Pushed<R> qa[kBatch], qb[kBatch];
+        {
+            int nA0 = 0, nA1 = 0, nA2 = 0, nB0 = 0, nB1 = 0, nB2 = 0;
+            for (int o = 0; o < len; o += 64 * kBatch) {
+                g1_load<R>(pid, pf, px, py, pz, pA + o, np, lane, qa);
+                g1_load<R>(pid, pf, px, py, pz, pB + o, np, lane, qb);
+#pragma unroll
+                for (int q = 0; q < kBatch; ++q)
+                    g1_push<R, true>(pa, cfA, gA, pA + o + 64 * q + lane < pB, qa[q], nA0, nA1, nA2, accA);
+#pragma unroll
+                for (int q = 0; q < kBatch; ++q)
+                    g1_push<R, true>(pa, cfB, gB, pB + o + 64 * q + lane < pE, qb[q], nB0, nB1, nB2, accB);
+            }
+            tA0 = nA0; tA2 = nA2; tB0 = nB0; tB2 = nB2;
+        }
+        // the pair's totals to the block, the neighbours' back (double
+        // buffered: a wave writes this buffer again two barriers later,
+        // after every wave has read it)
+        if (lane == 0) {
+            xcnt[dbuf][wv][0] = tA0;
+            xcnt[dbuf][wv][1] = tB2;
+        }
+        __syncthreads();
+        // the outputs: the other buffer, and this buffer for the staged
+        // movers (slot start[c] + r of a cell's r-th staged mover lies in the
+        // cell's own range, at or below the mover's own index: only this wave
+        // reads that range, and the slot's particle is already in registers)
+        const StateOut<R> nxt{gp(b.id[po]), gp(b.frac[po]), gp(b.ux[po]), gp(b.uy[po]), gp(b.uz[po])};
+        const StateOut<R> stg{gp(b.id[par]), gp(b.frac[par]), gp(b.ux[par]), gp(b.uy[par]), gp(b.uz[par])};
+        const gptr<int32_t> err = gp(b.err);
+        const bool first = wv == 0, last = wv == kPushWaves - 1;
+        const int upPrev = first ? 0 : xcnt[dbuf][wv - 1][1];      // up(A-1): B of the pair below
+        const int downNext = last ? 0 : xcnt[dbuf][wv + 1][0];     // down(B+1): A of the pair above
+        // new indices (start[A] = pA, start[B] = pB, start[B+1] = pE for
+        // a pair with live particles):
+        //   A: down -> pA - up(A-1) + r, staged at pA + r in a block's
+        //      first pair; stay -> pA + down(A) + r; up -> pB - up(A) + down(B) + r
+        //   B: down -> pB - up(A) + r; stay -> pB + down(B) + r;
+        //      up -> pE - up(B) + down(B+1) + r, staged at pB + r in a
+        //      block's last pair
+        // per class k = d + 1, the new index of the class's rank 0
+        const int xa0 = first ? pA : pA - upPrev, xa1 = pA + tA0, xa2 = pB - tA2 + tB0;
+        const int xb0 = pB - tA2, xb1 = pB + tB0, xb2 = last ? pB : pE - tB2 + downNext;
+        if (one) {
+            if (len > 0) {
+                g1_put_cell<R>(nxt, stg, qa, pA, pB, lane, xa0, xa1, xa2, first ? 0 : -1, np, err);
+                g1_put_cell<R>(nxt, stg, qb, pB, pE, lane, xb0, xb1, xb2, last ? 2 : -1, np, err);
+            }
+        } else {
+            int nA0 = 0, nA1 = 0, nA2 = 0, nB0 = 0, nB1 = 0, nB2 = 0;   // rank carries again
+            for (int o = 0; o < len; o += 64 * kBatch) {
+                g1_load<R>(pid, pf, px, py, pz, pA + o, np, lane, qa);
+                g1_load<R>(pid, pf, px, py, pz, pB + o, np, lane, qb);
+#pragma unroll
+                for (int q = 0; q < kBatch; ++q)
+                    g1_push<R, false>(pa, cfA, gA, pA + o + 64 * q + lane < pB, qa[q], nA0, nA1, nA2, accA);
+#pragma unroll
+                for (int q = 0; q < kBatch; ++q)
+                    g1_push<R, false>(pa, cfB, gB, pB + o + 64 * q + lane < pE, qb[q], nB0, nB1, nB2, accB);
+                g1_put_cell<R>(nxt, stg, qa, pA + o, pB, lane, xa0, xa1, xa2, first ? 0 : -1, np, err);
+                g1_put_cell<R>(nxt, stg, qb, pB + o, pE, lane, xb0, xb1, xb2, last ? 2 : -1, np, err);
+            }
+        }
+        if (!pv) continue;
+        const gptr<int32_t> cup = gp(b.cup), cdown = gp(b.cdown);
+        const gptr<double> Sg = gp(b.S);
+        // lane k < 8 holds S[4 gA + k] (k >= 4: cell B's sums)
+        const double sk = pair_sums(red_all[wv], accA, accB, lane);
+        if (lane < (hasB ? 8 : 4)) Sg[4 * (size_t)gA + lane] = sk;
+        if (lane == 0) {
+            cup[gA] = tA2;
+            cdown[gA] = tA0;
+            if (hasB) {
+                cup[gB] = tB2;
+                cdown[gB] = tB0;
+            }
+        }
+    }
+}
+
+// A per-cell count of push/count (cup or cdown), which exists on the live
+// range [cmin, cmax] +-1 this step; zero elsewhere.
+__device__ __forceinline__ int count_in(gptr<const int32_t> a, int c, int cmin, int cmax) {
+    return (cmin <= cmax && c >= cmin - 1 && c <= cmax + 1) ? a[c] : 0;
+}
+
 // One 256-thread scatter block: entry bl = (beam, first particle) for a
 // particle block of 256 * PER particles, (beam, -(k + 1)) for node block k;
 // t is the thread's index in the block.  Shared by pic_scatter and the fused
@@ -601,7 +799,7 @@ __device__ __forceinline__ void scatter_block(const SimDev<R>* __restrict__ sims
     const int po = par ^ 1;
     // every descriptor field is read here, ahead of the kernel's first store
     // (scalar loads), as in pic_push_count
-    const int N = b.N, np = b.np, nnode = b.nnode;
+    const int N = b.N, np = b.np, nnode = b.nnode, G = b.G;
     // H^{n-1/2} from push/count's copy: the field step of this launch may run
     // alongside and overwrite H[par ^ 1]
     const PushArgs<R> pa{gp((const R*)sd.E[par]), gp((const R*)sd.H[par]), gp((const R*)b.hold),
@@ -622,9 +820,6 @@ __device__ __forceinline__ void scatter_block(const SimDev<R>* __restrict__ sims
     const gptr<R> J = gp(const_cast<R*>(sd.J));
     const int cmin = rng_p[0], cmax = rng_p[1];
     // counts and sums exist on [cmin-1, cmax+1] this step; zero elsewhere
-    auto inr = [&](int c) { return c >= cmin - 1 && c <= cmax + 1 && cmin <= cmax; };
-    auto up = [&](int c) { return inr(c) ? cup[c] : 0; };
-    auto down = [&](int c) { return inr(c) ? cdown[c] : 0; };
     if (bl.y >= 0) {
         // PER particles per thread, 256 apart: every load of all of
         // them is issued before the first store (the stores go to the other
@@ -658,9 +853,10 @@ __device__ __forceinline__ void scatter_block(const SimDev<R>* __restrict__ sims
                 int dn, off;
                 R a0, a1;
                 push_one<R>(pa, c, f[k], ux[k], uy[k], uz[k], dn, off, a0, a1);   // the push of pic_push_count
-                if (d < 0) ni = st[cc] - up(cc - 1) + r;
-                else if (d == 0) ni = st[cc] + down(cc) + r;
-                else ni = st[cc + 1] - up(cc) + down(cc + 1) + r;
+                if (d < 0) ni = st[cc] - count_in(cup, cc - 1, cmin, cmax) + r;
+                else if (d == 0) ni = st[cc] + count_in(cdown, cc, cmin, cmax) + r;
+                else
+                    ni = st[cc + 1] - count_in(cup, cc, cmin, cmax) + count_in(cdown, cc + 1, cmin, cmax) + r;
                 if (dn != d || ni < 0 || ni >= np) {
                     err[0] = 1;
                     continue;
@@ -679,14 +875,14 @@ __device__ __forceinline__ void scatter_block(const SimDev<R>* __restrict__ sims
     const int k = -bl.y - 1;
     const int u0 = urng_p[0], u1 = urng_p[1];
     const int e0 = max(u0 - 3, 0), e1 = min(u1 + 4, N);
-    const int nlo = st[2] + down(2);                 // new start of cell 2 (cell 1 sends nothing up)
-    const int nhi = st[N - 2] - up(N - 3);           // new start of cell N-2
+    const int nlo = st[2] + count_in(cdown, 2, cmin, cmax);      // new start of cell 2 (cell 1 sends nothing up)
+    const int nhi = st[N - 2] - count_in(cup, N - 3, cmin, cmax);    // new start of cell N-2
     for (int m = e0 + k * 256 + t; m <= e1; m += nnode * 256) {
         J[m] = (R)j_from_sums(Sg, cmin, cmax, m);     // the value the field step forms
-        const int ns = st[m] - up(m - 1) + down(m);
+        const int ns = st[m] - count_in(cup, m - 1, cmin, cmax) + count_in(cdown, m, cmin, cmax);
         st_o[m] = ns;
         if (m >= 2 && m <= N - 3) {
-            const int ns1 = st[m + 1] - up(m) + down(m + 1);
+            const int ns1 = st[m + 1] - count_in(cup, m, cmin, cmax) + count_in(cdown, m + 1, cmin, cmax);
             if (ns1 > ns) {
                 if (ns == nlo) {
                     rng_o[0] = m;
@@ -698,6 +894,53 @@ __device__ __forceinline__ void scatter_block(const SimDev<R>* __restrict__ sims
                 }
             }
         }
+    }
+    if (G != 1) return;
+    // the staged movers sit in this buffer (push/count wrote them over the
+    // start of their old cell's range)
+    const StateOut<R> dstb{ido, pfo, pxo, pyo, pzo};
+    const StateIn<R> srcb{idp, pf, px, py, pz};
+    // One-cell groups (push/count wrote every other particle's new state):
+    // the movers staged at start[c] + rank of this buffer, i.e. the down-movers of the first
+    // cell of each push/count block of 2 * kPushWaves cells and the up-movers
+    // of its last cell.  kXMove threads per block boundary, each taking every
+    // kXMove-th mover of both cells.
+    constexpr int kXMove = 8;
+    constexpr int kBlkCells = 2 * kPushWaves;
+    if (cmin <= cmax) {
+        const int lo_cell = max(cmin - 1, 0), hi_cell = min(cmax + 1, N);
+        const int nblk = (hi_cell - lo_cell + kBlkCells) / kBlkCells;
+        for (int j = k * 256 + t; j < nblk * kXMove; j += nnode * 256) {
+            const int md = lo_cell + (j / kXMove) * kBlkCells, mu = md + kBlkCells - 1;
+            const int r0 = j % kXMove;
+            if (md >= 2 && md <= N - 3) {
+                const int n = cdown[md];
+                const int src0 = st[md], dst0 = st[md] - count_in(cup, md - 1, cmin, cmax);
+                if (src0 < 0 || src0 + n > np || dst0 < 0 || dst0 + n > np) err[0] = 1;
+                else
+                    for (int r = r0; r < n; r += kXMove)
+                        copy_state<R>(srcb, (unsigned)(src0 + r), dstb, (unsigned)(dst0 + r));
+            }
+            if (mu >= 2 && mu <= min(hi_cell, N - 3)) {
+                const int n = cup[mu];
+                const int src0 = st[mu];
+                const int dst0 = st[mu + 1] - count_in(cup, mu, cmin, cmax) + count_in(cdown, mu + 1, cmin, cmax);
+                if (src0 < 0 || src0 + n > np || dst0 < 0 || dst0 + n > np) err[0] = 1;
+                else
+                    for (int r = r0; r < n; r += kXMove)
+                        copy_state<R>(srcb, (unsigned)(src0 + r), dstb, (unsigned)(dst0 + r));
+            }
+        }
+    }
+    // the particles parked at the previous step, [park[0], start[2]) and
+    // [start[N-2], park[1]): copied to the other buffer at the same index
+    // (from then on they are in both buffers and never copied again)
+    const int a0 = park[0], a1 = max(a0, st[2]);
+    const int b1 = park[1], b0 = min(st[N - 2], b1);
+    const int na = a1 - a0, nb = b1 - b0;
+    for (int j = k * 256 + t; j < na + nb; j += nnode * 256) {
+        const unsigned i = (unsigned)(j < na ? a0 + j : b0 + (j - na));
+        copy_state<R>(srcb, i, dstb, i);
     }
 }
 

@@ -197,3 +197,101 @@ def emulate_pic_step(cell, d_live, off_live, a0, a1, start, crange, urange, G, N
                     nrange[1] = m
                     nurange[1] = max(urange[1], m)
     return new_idx, nstart, J, tuple(nrange), tuple(nurange), cup, cdown
+
+
+def emulate_one_cell_writes(cell, d_live, start, prev_start, crange, N, waves=4):
+    """The writes of one step on the one-cell-group path (pic_kernels.hpp,
+    G == 1): push/count's pairs of cells in blocks of `waves` pairs (one
+    workgroup), the direct writes and the staging, then the scatter's node
+    threads (staged movers, particles parked at the previous step).  Returns
+    (dst, cup, cdown): dst[j] = the old index of the particle written to
+    index j of the other buffer, -1 where nothing is written.  Asserts that
+    every index is in range and written at most once, and that each staged
+    slot lies in its cell's old range."""
+    n = len(cell)
+    live = (cell >= 2) & (cell <= N - 3)
+    d = np.where(live, d_live, 0)
+    cup = np.zeros(N + 2, dtype=np.int64)
+    cdown = np.zeros(N + 2, dtype=np.int64)
+    dst = np.full(n, -1, dtype=np.int64)
+    staged = np.full(n, -1, dtype=np.int64)
+
+    def put(arr, j, src):
+        assert 0 <= j < n, (j, n)
+        assert arr[j] == -1, ("written twice", j)
+        arr[j] = src
+
+    cmin, cmax = crange
+    clampi = lambda v: min(max(int(v), 0), n)
+    cl = lambda c: min(max(c, 2), N - 2)
+    lo_cell, hi_cell = max(cmin - 1, 0), min(cmax + 1, N)
+    if cmin <= cmax:
+        npairs = (hi_cell - lo_cell + 2) >> 1
+        for bk in range((npairs + waves - 1) // waves):
+            pairs = []
+            for wv in range(waves):                 # counting, then the exchange
+                gi = bk * waves + wv
+                pv = gi < npairs
+                gA, gB = lo_cell + 2 * gi, lo_cell + 2 * gi + 1
+                hasB = pv and gB <= hi_cell
+                pA = clampi(start[cl(gA)])
+                pB = max(pA, clampi(start[cl(gB)])) if pv else pA
+                pE = max(pB, clampi(start[cl(gB + 1 if hasB else gB)])) if pv else pA
+                ranks, tot = {}, []
+                for p0, p1 in ((pA, pB), (pB, pE)):
+                    cnt = [0, 0, 0]
+                    for i in range(p0, p1):
+                        ranks[i] = cnt[d[i] + 1]
+                        cnt[d[i] + 1] += 1
+                    tot.append(cnt)
+                pairs.append((pv, gA, gB, hasB, pA, pB, pE, ranks, tot))
+            for wv, (pv, gA, gB, hasB, pA, pB, pE, ranks, tot) in enumerate(pairs):
+                (tA0, _, tA2), (tB0, _, tB2) = tot
+                first, last = wv == 0, wv == waves - 1
+                up_prev = 0 if first else pairs[wv - 1][8][1][2]
+                down_next = 0 if last else pairs[wv + 1][8][0][0]
+                for i in range(pA, pB):
+                    r = ranks[i]
+                    if d[i] < 0 and first:
+                        assert pA <= pA + r < pB
+                        put(staged, pA + r, i)
+                    else:
+                        put(dst, pA - up_prev + r if d[i] < 0 else
+                            (pA + tA0 + r if d[i] == 0 else pB - tA2 + tB0 + r), i)
+                for i in range(pB, pE):
+                    r = ranks[i]
+                    if d[i] > 0 and last:
+                        assert pB <= pB + r < pE
+                        put(staged, pB + r, i)
+                    else:
+                        put(dst, pE - tB2 + down_next + r if d[i] > 0 else
+                            (pB + tB0 + r if d[i] == 0 else pB - tA2 + r), i)
+                if pv:
+                    cup[gA], cdown[gA] = tA2, tA0
+                    if hasB:
+                        cup[gB], cdown[gB] = tB2, tB0
+    inr = lambda c: cmin <= cmax and cmin - 1 <= c <= cmax + 1
+    up = lambda c: cup[c] if inr(c) else 0
+    down = lambda c: cdown[c] if inr(c) else 0
+    moved = set()
+    if cmin <= cmax:
+        blk = 2 * waves
+        for j in range((hi_cell - lo_cell + blk) // blk):
+            md = lo_cell + j * blk
+            mu = md + blk - 1
+            if 2 <= md <= N - 3:
+                for r in range(cdown[md]):
+                    assert staged[start[md] + r] >= 0
+                    put(dst, start[md] - up(md - 1) + r, staged[start[md] + r])
+                    moved.add(int(staged[start[md] + r]))
+            if 2 <= mu <= min(hi_cell, N - 3):
+                for r in range(cup[mu]):
+                    assert staged[start[mu] + r] >= 0
+                    put(dst, start[mu + 1] - up(mu) + down(mu + 1) + r, staged[start[mu] + r])
+                    moved.add(int(staged[start[mu] + r]))
+    assert moved == set(staged[staged >= 0].tolist())      # every staged particle moved once
+    a0, b1 = prev_start[2], prev_start[N - 2]
+    a1, b0 = max(a0, start[2]), min(start[N - 2], b1)
+    for i in list(range(a0, a1)) + list(range(b0, b1)):
+        put(dst, i, i)
+    return dst, cup, cdown

@@ -6,7 +6,8 @@ Cases (config-1 slab, thermal electron beam drifting at 0.1c):
   absorbed   a beam drifting into the left boundary (particles parked)
   absorbed_right  the same at the right boundary
 FP64: fields, probes, particle displacement and u within 1e-10 relative L2;
-FP32: within 1e-5.  Also: bitwise invariance of the run under a permutation
+FP32: within 1e-5; the same for one-cell groups at mixed cell densities
+(both of push/count's write paths).  Also: bitwise invariance of the run under a permutation
 of the input particles, agreement between sort/deposit wave widths, the
 fused field + scatter launch bitwise equal to separate launches, and a
 reduced config 3 (N = 65,536, 1e5 particles, 300 steps) in FP32.
@@ -89,6 +90,28 @@ def test_pic_parity(oracles, kind, prec, tol):
         assert 100 < (~o["alive"]).sum() < len(o["alive"]) - 100
     for k, v in errs.items():
         assert v <= tol, (kind, prec, k, v)
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-10), ("f32", 1e-5)])
+def test_one_cell_groups_mixed_density(prec, tol):
+    """One-cell groups (push/count writes the new state) on a beam of about
+    125 particles per cell: cells of up to 128 particles take the one-pass
+    path, fuller ones the counting pass plus the writing pass, in one run;
+    against the oracle at the parity tolerances."""
+    p = _params("driven", steps=300, n=25000)
+    p["beam"].update(cell_lo=1200, cell_hi=1400)
+    o = _oracle(p)
+    x0 = o["x0"]
+    cells = np.bincount(np.floor(x0).astype(int))
+    assert cells.max() > 128 and (cells[1200:1400] <= 128).any()
+    print(f"one-cell groups {prec}", flush=True)
+    d = _device(p, prec, group=1)
+    errs = dict(E=rel_l2(d["E"], o["E"]), H=rel_l2(d["H"], o["H"]), probes=rel_l2(d["probes"], o["probes"]),
+                dx=rel_l2(d["x"] - x0, o["x"] - x0), u=rel_l2(d["u"], o["u"]))
+    print("one-cell groups", prec, {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    assert np.array_equal(d["alive"], o["alive"])
+    for k, v in errs.items():
+        assert v <= tol, (prec, k, v)
 
 
 def test_permutation_invariance_bitwise():

@@ -3,11 +3,14 @@ pic_push_count and pic_scatter), emulated lane by lane over several steps of
 random moves: the new indices form the stable sort by new cell with parked
 particles keeping their index, start[] stays the exclusive count scan, J'
 equals the CIC sum of every live particle's two weights (np.bincount), and
-the single-writer live range equals the min/max of the live cells."""
+the single-writer live range equals the min/max of the live cells.  For the
+one-cell-group path, which writes the new state from push/count and stages
+the movers into another wave's cells, two ping-pong buffers are followed
+through the steps."""
 import numpy as np
 import pytest
 
-from emulator import emulate_pic_step
+from emulator import emulate_one_cell_writes, emulate_pic_step
 
 
 def _moves(rng, cell, N):
@@ -105,3 +108,52 @@ def test_one_cell_pairs_cover_each_cell_once(N, lo, hi, n):
             assert np.all(cell[p0:p1] == g)
         # every live particle is read exactly once
         assert sum(p1 - p0 for _, p0, p1 in got) == int(live.sum())
+
+
+@pytest.mark.parametrize("N,n,lo,hi,steps", [
+    (200, 3000, 2, 198, 5),          # dense: up to ~40 per cell
+    (80, 9000, 20, 40, 4),           # cells of several hundred (the two-pass case)
+    (300, 500, 100, 140, 5),         # clustered, empty regions
+    (120, 900, 2, 6, 8),             # drifting into the left graveyard
+    (60, 3000, 54, 58, 8),           # most of the beam parked in the right graveyard
+    (64, 1, 30, 31, 3),              # one particle
+])
+def test_one_cell_path_buffers(N, n, lo, hi, steps):
+    """Two ping-pong buffers through several steps of random moves on the
+    one-cell-group path: after every step the buffer written holds every
+    particle at its sorted index (the indices of emulate_pic_step), parked
+    particles included, although the long-parked ones are never rewritten."""
+    rng = np.random.default_rng(N * 11 + n)
+    cell = np.sort(rng.integers(lo, hi, n))
+    start = np.searchsorted(cell, np.arange(N + 2), side="left").astype(np.int64)
+    prev_start = start.copy()
+    live = (cell >= 2) & (cell <= N - 3)
+    crange = (int(cell[live].min()), int(cell[live].max()))
+    urange = crange
+    buf = [np.arange(n), np.full(n, -1)]          # particle ids; the other buffer holds garbage
+    ids = np.arange(n)
+    for step in range(steps):
+        par = step & 1
+        d, newc = _moves(rng, cell, N)
+        off = np.ones(n, dtype=np.int64)
+        zero = np.zeros(n)
+        new_idx, nstart, _, nrange, nurange, cup_ref, cdown_ref = emulate_pic_step(
+            cell, d, off, zero, zero, start, crange, urange, 1, N)
+        dst, cup, cdown = emulate_one_cell_writes(cell, d, start, prev_start, crange, N)
+        lo_cell, hi_cell = max(crange[0] - 1, 0), min(crange[1] + 1, N)
+        if crange[0] <= crange[1]:
+            sl = slice(lo_cell, hi_cell + 1)
+            assert np.array_equal(cup[sl], cup_ref[sl]) and np.array_equal(cdown[sl], cdown_ref[sl])
+        w = dst >= 0
+        out = buf[par ^ 1].copy()
+        out[w] = buf[par][dst[w]]
+        # every live particle and every particle parked this or the previous
+        # step is written, at its new index
+        order = np.argsort(new_idx)
+        assert np.array_equal(out, buf[par][order]), step
+        assert np.array_equal(new_idx[dst[w]], np.nonzero(w)[0])
+        buf[par ^ 1] = out
+        prev_start, start = start, nstart
+        cell, crange, urange = newc[order], nrange, nurange
+        ids = ids[order]
+        assert np.array_equal(buf[par ^ 1], ids)
