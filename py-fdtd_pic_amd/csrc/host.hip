@@ -41,7 +41,8 @@ hipError_t launch_scatter(const SimDev<R>* sims, const BeamDev<R>* beams, const 
 template <typename R, bool KAHAN>
 hipError_t launch_field_scatter(const SimDev<R>* sims, const int2* tiles, int ntiles, long long launch_idx,
                                 long long gstep0, int nsteps, int K, int S, int kind, const BeamDev<R>* beams,
-                                const int2* sblocks, int nsblocks, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1);
+                                const int2* sblocks, int nsblocks, int nfirst, hipStream_t st, hipEvent_t ev0,
+                                hipEvent_t ev1);
 }
 
 using namespace fdtd;
@@ -697,7 +698,7 @@ static int rebuild_t(fdtd_ctx* c) {
     }
     // particle beams of the active simulations
     std::vector<BeamDev<R>> bdev;
-    std::vector<int2> pblocks, pitems, citems;
+    std::vector<int2> pblocks, xblocks, pitems, citems;
     for (int i : c->tile_sims) {
         BeamHost* bh = i < (int)c->beams.size() ? c->beams[i] : nullptr;
         if (!bh) continue;
@@ -709,14 +710,18 @@ static int rebuild_t(fdtd_ctx* c) {
         } else {
             for (int w = 0; w < bh->nwaves; ++w) pitems.push_back(make_int2(bi, w));
         }
-        // particle blocks for multi-cell groups only: push/count moves the
-        // particles of one-cell groups, and the node blocks the rest
-        if (bh->G != 1)
-            for (int64_t p = 0; p < bh->np; p += 256 * kScatterPer) pblocks.push_back(make_int2(bi, (int)p));
+        // node blocks first (they go ahead of the field workgroups in the
+        // fused launch); particle blocks for multi-cell groups only:
+        // push/count moves the particles of one-cell groups, and the node
+        // blocks the rest
         for (int k = 0; k < bh->nnode; ++k) pblocks.push_back(make_int2(bi, -(k + 1)));
+        if (bh->G != 1)
+            for (int64_t p = 0; p < bh->np; p += 256 * kScatterPer) xblocks.push_back(make_int2(bi, (int)p));
     }
     while (pitems.size() % kPushWaves) pitems.push_back(make_int2(-1, 0));   // whole push/count workgroups
     c->has_pic = !bdev.empty();
+    c->npnode = (int)pblocks.size();
+    pblocks.insert(pblocks.end(), xblocks.begin(), xblocks.end());
     c->npblocks = (int)pblocks.size();
     c->npitems = (int)pitems.size();
     c->npcell = (int)citems.size();
@@ -863,11 +868,13 @@ static int launch_once(fdtd_ctx* c, int n, EvPair* ev, hipStream_t st, bool fuse
             if (c->prec == FDTD_F64)
                 e = launch_field_scatter<double, false>(
                     (const SimDev<double>*)c->d_sims, c->d_tiles + g.offset, g.count, c->launch_idx, c->gstep, n,
-                    c->K, c->S_run, g.kind, (const BeamDev<double>*)c->d_beams, c->d_pblocks, c->npblocks, st, e0, e1);
+                    c->K, c->S_run, g.kind, (const BeamDev<double>*)c->d_beams, c->d_pblocks, c->npblocks,
+                    c->npnode, st, e0, e1);
             else
                 e = launch_field_scatter<float, true>(
                     (const SimDev<float>*)c->d_sims, c->d_tiles + g.offset, g.count, c->launch_idx, c->gstep, n,
-                    c->K, c->S_run, g.kind, (const BeamDev<float>*)c->d_beams, c->d_pblocks, c->npblocks, st, e0, e1);
+                    c->K, c->S_run, g.kind, (const BeamDev<float>*)c->d_beams, c->d_pblocks, c->npblocks,
+                    c->npnode, st, e0, e1);
             if (e != hipSuccess)
                 return fdtd_fail(FDTD_E_HIP, std::string("field + scatter launch ") + std::to_string(c->launch_idx) +
                                                  " (" + std::to_string(g.count) + " tiles, " +

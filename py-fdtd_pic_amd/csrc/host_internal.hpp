@@ -95,6 +95,7 @@ struct fdtd_ctx {
     int2* d_pitems = nullptr;                // push/count items: npitems multi-cell waves (beam, wave
     size_t d_pitems_cap = 0;                 // index), then npcell one-cell workgroups (beam, workgroup)
     int npblocks = 0, npitems = 0, npcell = 0;
+    int npnode = 0;                          // scatter blocks: npnode node blocks, then particle blocks
     bool has_pic = false;                    // some active simulation carries a beam
     bool fuse_pic = true;                    // scatter in the same launch as the last tile group's field step
     // bound on queued work: an event every kThrottleEvery launches; the host

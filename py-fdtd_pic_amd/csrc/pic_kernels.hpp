@@ -316,22 +316,44 @@ __device__ __forceinline__ void copy_state(const StateIn<R>& a, unsigned s, cons
     sto(b.uz, d, ldo(a.uz, s));
 }
 
-// Loads of kBatch 64-particle chunks from index base: unconditional at an
+// Loads of one 64-particle chunk from index base: unconditional at an
 // in-range index (np >= 1); the callers select on use.
+template <typename R>
+__device__ __forceinline__ void g1_load_one(gptr<const int32_t> pid, gptr<const R> pf, gptr<const R> px,
+                                            gptr<const R> py, gptr<const R> pz, int base, int np, int lane,
+                                            Pushed<R>& s) {
+    const unsigned i = (unsigned)min(base + lane, np - 1);
+    s.id = ldo(pid, i);
+    s.f = ldo(pf, i);
+    s.ux = ldo(px, i);
+    s.uy = ldo(py, i);
+    s.uz = ldo(pz, i);
+}
+
+// The same for kBatch chunks from index base, 64 apart.
 template <typename R>
 __device__ __forceinline__ void g1_load(gptr<const int32_t> pid, gptr<const R> pf, gptr<const R> px,
                                         gptr<const R> py, gptr<const R> pz, int base, int np, int lane,
                                         Pushed<R> (&s)[kBatch]) {
 #pragma unroll
-    for (int q = 0; q < kBatch; ++q) {
-        const unsigned i = (unsigned)min(base + 64 * q + lane, np - 1);
-        s[q].id = ldo(pid, i);
-        s[q].f = ldo(pf, i);
-        s[q].ux = ldo(px, i);
-        s[q].uy = ldo(py, i);
-        s[q].uz = ldo(pz, i);
-    }
+    for (int q = 0; q < kBatch; ++q) g1_load_one<R>(pid, pf, px, py, pz, base + 64 * q, np, lane, s[q]);
 }
+
+// One wave's LDS row for a pushed 64-particle chunk (structure of arrays:
+// lane l's fields at index l, conflict-free).
+template <typename R>
+struct OvfRow {
+    int id[64], dr[64];
+    R f[64], ux[64], uy[64], uz[64];
+    __device__ __forceinline__ void put(int l, const Pushed<R>& s) {
+        id[l] = s.id; dr[l] = s.dr; f[l] = s.f; ux[l] = s.ux; uy[l] = s.uy; uz[l] = s.uz;
+    }
+    __device__ __forceinline__ Pushed<R> get(int l) const {
+        Pushed<R> s;
+        s.id = id[l]; s.dr = dr[l]; s.f = f[l]; s.ux = ux[l]; s.uy = uy[l]; s.uz = uz[l];
+        return s;
+    }
+};
 
 // One 64-particle chunk of a one-cell group (valid: the lane holds one of the
 // cell's particles): the push, its move class d, its rank among the cell's
@@ -347,20 +369,21 @@ __device__ __forceinline__ void g1_push(const PushArgs<R>& pa, const CellFields<
     const unsigned long long bd0 = __ballot(valid && d == -1);
     const unsigned long long bd1 = __ballot(valid && d == 0);
     const unsigned long long bd2 = __ballot(valid && d == 1);
-    const unsigned long long mine = d < 0 ? bd0 : (d == 0 ? bd1 : bd2);
-    const int carry = d < 0 ? n0 : (d == 0 ? n1 : n2);
-    const int r = carry + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mine >> 32),
-                                                         __builtin_amdgcn_mbcnt_lo((unsigned)mine, 0u));
+    // the earlier lanes of each class (the masks are wave-uniform), then the
+    // lane's own class with its carry
+    const int m0 = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bd0 >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bd0, 0u));
+    const int m1 = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bd1 >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bd1, 0u));
+    const int m2 = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bd2 >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bd2, 0u));
+    const int r = d < 0 ? n0 + m0 : (d == 0 ? n1 + m1 : n2 + m2);
     s.dr = (r << 2) | (d + 1);
     if (SUMS) {
-        const R s0 = off == 0 ? a0 : R(0);
-        const R s1 = off == 0 ? a1 : (off == 1 ? a0 : R(0));
-        const R s2 = off == 1 ? a1 : (off == 2 ? a0 : R(0));
-        const R s3 = off == 2 ? a1 : R(0);
-        acc[0] += (double)s0;
-        acc[1] += (double)s1;
-        acc[2] += (double)s2;
-        acc[3] += (double)s3;
+        // the weights onto nodes c-1+off and c+off (exact conversions, so the
+        // same sums as converting each node's share)
+        const double w0 = (double)a0, w1 = (double)a1;
+        acc[0] += off == 0 ? w0 : 0.0;
+        acc[1] += off == 0 ? w1 : (off == 1 ? w0 : 0.0);
+        acc[2] += off == 1 ? w1 : (off == 2 ? w0 : 0.0);
+        acc[3] += off == 2 ? w1 : 0.0;
     }
     n0 += __popcll(bd0);
     n1 += __popcll(bd1);
@@ -383,21 +406,28 @@ __device__ __forceinline__ void g1_put(const StateOut<R>& o, int i, int np, cons
     sto(o.uz, (unsigned)i, s.uz);
 }
 
-// The new state of kBatch chunks of one cell (its particles [.., phi), the
-// chunks from index base): the particle of class k = d + 1 and rank r goes
-// to index x_k + r of nxt, or of stg when k == staged (-1: none).
+// The new state of one pushed chunk of a cell (its particles [.., phi), the
+// chunk from index base): the particle of class k = d + 1 and rank r goes to
+// index x_k + r of nxt, or of stg when k == staged (-1: none).
+template <typename R>
+__device__ __forceinline__ void g1_put_one(const StateOut<R>& nxt, const StateOut<R>& stg, const Pushed<R>& s,
+                                           int base, int phi, int lane, int x0, int x1, int x2, int staged,
+                                           int np, gptr<int32_t> err) {
+    if (base + lane < phi) {
+        const int k = s.dr & 3, i = (k == 0 ? x0 : (k == 1 ? x1 : x2)) + (s.dr >> 2);
+        if (k == staged) g1_put<R>(stg, i, np, s, err);
+        else g1_put<R>(nxt, i, np, s, err);
+    }
+}
+
+// The same for kBatch chunks from index base, 64 apart.
 template <typename R>
 __device__ __forceinline__ void g1_put_cell(const StateOut<R>& nxt, const StateOut<R>& stg,
                                             const Pushed<R> (&s)[kBatch], int base, int phi, int lane, int x0,
                                             int x1, int x2, int staged, int np, gptr<int32_t> err) {
 #pragma unroll
-    for (int q = 0; q < kBatch; ++q) {
-        if (base + 64 * q + lane < phi) {
-            const int k = s[q].dr & 3, i = (k == 0 ? x0 : (k == 1 ? x1 : x2)) + (s[q].dr >> 2);
-            if (k == staged) g1_put<R>(stg, i, np, s[q], err);
-            else g1_put<R>(nxt, i, np, s[q], err);
-        }
-    }
+    for (int q = 0; q < kBatch; ++q)
+        g1_put_one<R>(nxt, stg, s[q], base + 64 * q, phi, lane, x0, x1, x2, staged, np, err);
 }
 
 // The fields of a multi-cell group, staged in the wave's LDS once per group:
@@ -618,14 +648,21 @@ __device__ __forceinline__ double pair_sums(double (*red)[64], const double (&a)
     return u;
 }
 
+// Waves per SIMD that pic_push_cell asks the register allocator for: FP32
+// fits five (96 VGPRs, so a config-3 beam's ~5,000 waves run in one round;
+// left alone the allocator takes 99 and four), FP64 needs ~160 (three).
+template <typename R>
+constexpr int kPushCellOcc = sizeof(R) == 4 ? 5 : 3;
+
 // One-cell groups (G = 1): one item (beam, workgroup index) per workgroup.
 template <typename R>
-__global__ __launch_bounds__(64 * kPushWaves) void pic_push_cell(const SimDev<R>* __restrict__ sims,
+__global__ __launch_bounds__(64 * kPushWaves, kPushCellOcc<R>) void pic_push_cell(const SimDev<R>* __restrict__ sims,
                                                                  const BeamDev<R>* __restrict__ beams,
                                                                  const int2* __restrict__ items,
                                                                  long long launch_idx) {
     __shared__ int xcnt[2][kPushWaves][2];      // per pair, down(A) and up(B) (double buffered)
     __shared__ double red_all[kPushWaves][9][64];   // per wave: the CIC sum reduction (pair_sums)
+    __shared__ OvfRow<R> ovf_all[kPushWaves][2];     // per wave: the extra chunk of cells A and B
     // wave-uniform by construction; readfirstlane lets the compiler see it, so
     // the item and the descriptors come in through scalar loads
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -694,15 +731,20 @@ __global__ __launch_bounds__(64 * kPushWaves) void pic_push_cell(const SimDev<R>
         double accA[4] = {0.0, 0.0, 0.0, 0.0}, accB[4] = {0.0, 0.0, 0.0, 0.0};
         const int len = max(pB - pA, pE - pB);
         // cells of up to 64 * kBatch live particles (the common case): one
-        // pass, the batch held in registers across the exchange; larger
-        // ones: a counting pass, then a pass that pushes again (bitwise the
-        // same) and writes
-        const bool one = len <= 64 * kBatch;
+        // pass, the batch held in registers across the exchange; up to
+        // 64 * (kBatch + 1): the same, each cell's extra chunk pushed after
+        // the batch and its new state parked in the wave's LDS row (a config-3
+        // beam of ~100 particles per cell has a few such cells, and a second
+        // pass there would set the launch's tail); larger ones: a counting
+        // pass, then a pass that pushes again (bitwise the same) and writes
+        const bool one = len <= 64 * (kBatch + 1);
+        const bool extra = one && len > 64 * kBatch;
         int tA0 = 0, tA2 = 0, tB0 = 0, tB2 = 0;        // the pair's totals: A down, A up, B down, B up
         Pushed<R> qa[kBatch], qb[kBatch];
         {
             int nA0 = 0, nA1 = 0, nA2 = 0, nB0 = 0, nB1 = 0, nB2 = 0;
-            for (int o = 0; o < len; o += 64 * kBatch) {
+            const int oend = one ? min(len, 64 * kBatch) : len;
+            for (int o = 0; o < oend; o += 64 * kBatch) {
                 g1_load<R>(pid, pf, px, py, pz, pA + o, np, lane, qa);
                 g1_load<R>(pid, pf, px, py, pz, pB + o, np, lane, qb);
 #pragma unroll
@@ -711,6 +753,16 @@ __global__ __launch_bounds__(64 * kPushWaves) void pic_push_cell(const SimDev<R>
 #pragma unroll
                 for (int q = 0; q < kBatch; ++q)
                     g1_push<R, true>(pa, cfB, gB, pB + o + 64 * q + lane < pE, qb[q], nB0, nB1, nB2, accB);
+            }
+            if (extra) {
+                constexpr int ox = 64 * kBatch;
+                Pushed<R> x;
+                g1_load_one<R>(pid, pf, px, py, pz, pA + ox, np, lane, x);
+                g1_push<R, true>(pa, cfA, gA, pA + ox + lane < pB, x, nA0, nA1, nA2, accA);
+                ovf_all[wv][0].put(lane, x);
+                g1_load_one<R>(pid, pf, px, py, pz, pB + ox, np, lane, x);
+                g1_push<R, true>(pa, cfB, gB, pB + ox + lane < pE, x, nB0, nB1, nB2, accB);
+                ovf_all[wv][1].put(lane, x);
             }
             tA0 = nA0; tA2 = nA2; tB0 = nB0; tB2 = nB2;
         }
@@ -746,6 +798,13 @@ __global__ __launch_bounds__(64 * kPushWaves) void pic_push_cell(const SimDev<R>
             if (len > 0) {
                 g1_put_cell<R>(nxt, stg, qa, pA, pB, lane, xa0, xa1, xa2, first ? 0 : -1, np, err);
                 g1_put_cell<R>(nxt, stg, qb, pB, pE, lane, xb0, xb1, xb2, last ? 2 : -1, np, err);
+            }
+            if (extra) {
+                constexpr int ox = 64 * kBatch;
+                g1_put_one<R>(nxt, stg, ovf_all[wv][0].get(lane), pA + ox, pB, lane, xa0, xa1, xa2,
+                              first ? 0 : -1, np, err);
+                g1_put_one<R>(nxt, stg, ovf_all[wv][1].get(lane), pB + ox, pE, lane, xb0, xb1, xb2,
+                              last ? 2 : -1, np, err);
             }
         } else {
             int nA0 = 0, nA1 = 0, nA2 = 0, nB0 = 0, nB1 = 0, nB2 = 0;   // rank carries again

@@ -20,13 +20,13 @@ extern template int field_blocks_per_cu<double, false, SLAB_LINEAR>(int);
 extern template int field_blocks_per_cu<double, false, SLAB_KERR_E>(int);
 extern template int field_blocks_per_cu<double, false, SLAB_KERR_REF>(int);
 
-extern template hipError_t launch_fused_kind<float, true, SLAB_DIEL>(const SimDev<float>*, const int2*, int, long long, long long, int, int, int, const BeamDev<float>*, const int2*, int, hipStream_t, hipEvent_t, hipEvent_t);
-extern template hipError_t launch_fused_kind<float, true, SLAB_LINEAR>(const SimDev<float>*, const int2*, int, long long, long long, int, int, int, const BeamDev<float>*, const int2*, int, hipStream_t, hipEvent_t, hipEvent_t);
-extern template hipError_t launch_fused_kind<float, true, SLAB_KERR_E>(const SimDev<float>*, const int2*, int, long long, long long, int, int, int, const BeamDev<float>*, const int2*, int, hipStream_t, hipEvent_t, hipEvent_t);
-extern template hipError_t launch_fused_kind<double, false, SLAB_DIEL>(const SimDev<double>*, const int2*, int, long long, long long, int, int, int, const BeamDev<double>*, const int2*, int, hipStream_t, hipEvent_t, hipEvent_t);
-extern template hipError_t launch_fused_kind<double, false, SLAB_LINEAR>(const SimDev<double>*, const int2*, int, long long, long long, int, int, int, const BeamDev<double>*, const int2*, int, hipStream_t, hipEvent_t, hipEvent_t);
-extern template hipError_t launch_fused_kind<double, false, SLAB_KERR_E>(const SimDev<double>*, const int2*, int, long long, long long, int, int, int, const BeamDev<double>*, const int2*, int, hipStream_t, hipEvent_t, hipEvent_t);
-extern template hipError_t launch_fused_kind<double, false, SLAB_KERR_REF>(const SimDev<double>*, const int2*, int, long long, long long, int, int, int, const BeamDev<double>*, const int2*, int, hipStream_t, hipEvent_t, hipEvent_t);
+extern template hipError_t launch_fused_kind<float, true, SLAB_DIEL>(const SimDev<float>*, const int2*, int, long long, long long, int, int, int, const BeamDev<float>*, const int2*, int, int, hipStream_t, hipEvent_t, hipEvent_t);
+extern template hipError_t launch_fused_kind<float, true, SLAB_LINEAR>(const SimDev<float>*, const int2*, int, long long, long long, int, int, int, const BeamDev<float>*, const int2*, int, int, hipStream_t, hipEvent_t, hipEvent_t);
+extern template hipError_t launch_fused_kind<float, true, SLAB_KERR_E>(const SimDev<float>*, const int2*, int, long long, long long, int, int, int, const BeamDev<float>*, const int2*, int, int, hipStream_t, hipEvent_t, hipEvent_t);
+extern template hipError_t launch_fused_kind<double, false, SLAB_DIEL>(const SimDev<double>*, const int2*, int, long long, long long, int, int, int, const BeamDev<double>*, const int2*, int, int, hipStream_t, hipEvent_t, hipEvent_t);
+extern template hipError_t launch_fused_kind<double, false, SLAB_LINEAR>(const SimDev<double>*, const int2*, int, long long, long long, int, int, int, const BeamDev<double>*, const int2*, int, int, hipStream_t, hipEvent_t, hipEvent_t);
+extern template hipError_t launch_fused_kind<double, false, SLAB_KERR_E>(const SimDev<double>*, const int2*, int, long long, long long, int, int, int, const BeamDev<double>*, const int2*, int, int, hipStream_t, hipEvent_t, hipEvent_t);
+extern template hipError_t launch_fused_kind<double, false, SLAB_KERR_REF>(const SimDev<double>*, const int2*, int, long long, long long, int, int, int, const BeamDev<double>*, const int2*, int, int, hipStream_t, hipEvent_t, hipEvent_t);
 
 // kind: 0 (no D-form slab: NONE or DIEL share the DIEL kernel), SLAB_LINEAR,
 // SLAB_KERR_REF (FP64 only), SLAB_KERR_E.
@@ -79,9 +79,11 @@ template int field_capacity_per_cu<float, true>(int, int);
 template <typename R, bool KAHAN>
 hipError_t launch_field_scatter(const SimDev<R>* sims, const int2* tiles, int ntiles, long long launch_idx,
                                 long long gstep0, int nsteps, int K, int S, int kind, const BeamDev<R>* beams,
-                                const int2* sblocks, int nsblocks, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
+                                const int2* sblocks, int nsblocks, int nfirst, hipStream_t st, hipEvent_t ev0,
+                                hipEvent_t ev1) {
 #define FDTD_FUSED(KIND) \
-    launch_fused_kind<R, KAHAN, KIND>(sims, tiles, ntiles, launch_idx, gstep0, nsteps, K, S, beams, sblocks, nsblocks, st, ev0, ev1)
+    launch_fused_kind<R, KAHAN, KIND>(sims, tiles, ntiles, launch_idx, gstep0, nsteps, K, S, beams, sblocks, nsblocks, \
+                                      nfirst, st, ev0, ev1)
     switch (kind) {
         case SLAB_NONE:
         case SLAB_DIEL: return FDTD_FUSED(SLAB_DIEL);
@@ -97,9 +99,9 @@ hipError_t launch_field_scatter(const SimDev<R>* sims, const int2* tiles, int nt
 
 template hipError_t launch_field_scatter<double, false>(const SimDev<double>*, const int2*, int, long long, long long,
                                                         int, int, int, int, const BeamDev<double>*, const int2*, int,
-                                                        hipStream_t, hipEvent_t, hipEvent_t);
+                                                        int, hipStream_t, hipEvent_t, hipEvent_t);
 template hipError_t launch_field_scatter<float, true>(const SimDev<float>*, const int2*, int, long long, long long,
                                                       int, int, int, int, const BeamDev<float>*, const int2*, int,
-                                                      hipStream_t, hipEvent_t, hipEvent_t);
+                                                      int, hipStream_t, hipEvent_t, hipEvent_t);
 
 }  // namespace fdtd

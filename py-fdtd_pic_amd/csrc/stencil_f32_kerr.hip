@@ -6,5 +6,5 @@
 namespace fdtd {
 template hipError_t launch_kind<float, true, SLAB_KERR_E>(const SimDev<float>*, const int2*, int, long long, long long, int, int, int, const PersistArgs&, hipStream_t, hipEvent_t, hipEvent_t);
 template int field_blocks_per_cu<float, true, SLAB_KERR_E>(int);
-template hipError_t launch_fused_kind<float, true, SLAB_KERR_E>(const SimDev<float>*, const int2*, int, long long, long long, int, int, int, const BeamDev<float>*, const int2*, int, hipStream_t, hipEvent_t, hipEvent_t);
+template hipError_t launch_fused_kind<float, true, SLAB_KERR_E>(const SimDev<float>*, const int2*, int, long long, long long, int, int, int, const BeamDev<float>*, const int2*, int, int, hipStream_t, hipEvent_t, hipEvent_t);
 }  // namespace fdtd

@@ -7,7 +7,7 @@ Cases (config-1 slab, thermal electron beam drifting at 0.1c):
   absorbed_right  the same at the right boundary
 FP64: fields, probes, particle displacement and u within 1e-10 relative L2;
 FP32: within 1e-5; the same for one-cell groups at mixed cell densities
-(both of push/count's write paths).  Also: bitwise invariance of the run under a permutation
+(push/count's register and LDS-held chunks).  Also: bitwise invariance of the run under a permutation
 of the input particles, agreement between sort/deposit wave widths, the
 fused field + scatter launch bitwise equal to separate launches, and a
 reduced config 3 (N = 65,536, 1e5 particles, 300 steps) in FP32.
@@ -95,9 +95,11 @@ def test_pic_parity(oracles, kind, prec, tol):
 @pytest.mark.parametrize("prec,tol", [("f64", 1e-10), ("f32", 1e-5)])
 def test_one_cell_groups_mixed_density(prec, tol):
     """One-cell groups (push/count writes the new state) on a beam of about
-    125 particles per cell: cells of up to 128 particles take the one-pass
-    path, fuller ones the counting pass plus the writing pass, in one run;
-    against the oracle at the parity tolerances."""
+    125 particles per cell: cells of up to 128 particles are held in
+    registers, fuller ones (up to 158 here) push a third chunk whose state
+    waits in LDS, in one run; against the oracle at the parity tolerances.
+    Cells above 192 (the counting pass plus the writing pass) are the
+    absorbed cases of test_pic_parity (about 350 per cell)."""
     p = _params("driven", steps=300, n=25000)
     p["beam"].update(cell_lo=1200, cell_hi=1400)
     o = _oracle(p)
