@@ -714,7 +714,9 @@ static int rebuild_t(fdtd_ctx* c) {
         // fused launch); particle blocks for multi-cell groups only:
         // push/count moves the particles of one-cell groups, and the node
         // blocks the rest
-        for (int k = 0; k < bh->nnode; ++k) pblocks.push_back(make_int2(bi, -(k + 1)));
+        // one-cell beams: as many again for the staged movers and parked particles
+        const int nodes = bh->G == 1 ? 2 * bh->nnode : bh->nnode;
+        for (int k = 0; k < nodes; ++k) pblocks.push_back(make_int2(bi, -(k + 1)));
         if (bh->G != 1)
             for (int64_t p = 0; p < bh->np; p += 256 * kScatterPer) xblocks.push_back(make_int2(bi, (int)p));
     }

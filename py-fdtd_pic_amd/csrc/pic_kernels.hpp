@@ -844,9 +844,65 @@ __device__ __forceinline__ int count_in(gptr<const int32_t> a, int c, int cmin, 
     return (cmin <= cmax && c >= cmin - 1 && c <= cmax + 1) ? a[c] : 0;
 }
 
+// The second half of a one-cell beam's node blocks (block k of nnode): the
+// movers push/count staged at start[c] + rank of this buffer, i.e. the
+// down-movers of the first cell of each push/count block of 2 * kPushWaves
+// cells and the up-movers of its last cell (kXMove threads per block
+// boundary, each taking every kXMove-th mover of both cells), and the
+// particles parked at the previous step, [park[0], start[2]) and
+// [start[N-2], park[1]), copied to the other buffer at the same index (from
+// then on they are in both buffers and never copied again).
+template <typename R>
+__device__ __forceinline__ void move_block(const BeamDev<R>& b, int par, int k, int t, int cmin, int cmax) {
+    const int po = par ^ 1;
+    const int N = b.N, np = b.np, nnode = b.nnode;
+    const gptr<const int32_t> st = gp((const int32_t*)b.start[par]);
+    const gptr<const int32_t> cup = gp((const int32_t*)b.cup), cdown = gp((const int32_t*)b.cdown);
+    const gptr<const int32_t> park = gp((const int32_t*)b.park);
+    const gptr<int32_t> err = gp(b.err);
+    const StateOut<R> dstb{gp(b.id[po]), gp(b.frac[po]), gp(b.ux[po]), gp(b.uy[po]), gp(b.uz[po])};
+    const StateIn<R> srcb{gp((const int32_t*)b.id[par]), gp((const R*)b.frac[par]), gp((const R*)b.ux[par]),
+                          gp((const R*)b.uy[par]), gp((const R*)b.uz[par])};
+    constexpr int kXMove = 8;
+    constexpr int kBlkCells = 2 * kPushWaves;
+    if (cmin <= cmax) {
+        const int lo_cell = max(cmin - 1, 0), hi_cell = min(cmax + 1, N);
+        const int nblk = (hi_cell - lo_cell + kBlkCells) / kBlkCells;
+        for (int j = k * 256 + t; j < nblk * kXMove; j += nnode * 256) {
+            const int md = lo_cell + (j / kXMove) * kBlkCells, mu = md + kBlkCells - 1;
+            const int r0 = j % kXMove;
+            if (md >= 2 && md <= N - 3) {
+                const int n = cdown[md];
+                const int src0 = st[md], dst0 = st[md] - count_in(cup, md - 1, cmin, cmax);
+                if (src0 < 0 || src0 + n > np || dst0 < 0 || dst0 + n > np) err[0] = 1;
+                else
+                    for (int r = r0; r < n; r += kXMove)
+                        copy_state<R>(srcb, (unsigned)(src0 + r), dstb, (unsigned)(dst0 + r));
+            }
+            if (mu >= 2 && mu <= min(hi_cell, N - 3)) {
+                const int n = cup[mu];
+                const int src0 = st[mu];
+                const int dst0 = st[mu + 1] - count_in(cup, mu, cmin, cmax) + count_in(cdown, mu + 1, cmin, cmax);
+                if (src0 < 0 || src0 + n > np || dst0 < 0 || dst0 + n > np) err[0] = 1;
+                else
+                    for (int r = r0; r < n; r += kXMove)
+                        copy_state<R>(srcb, (unsigned)(src0 + r), dstb, (unsigned)(dst0 + r));
+            }
+        }
+    }
+    const int a0 = park[0], a1 = max(a0, st[2]);
+    const int b1 = park[1], b0 = min(st[N - 2], b1);
+    const int na = a1 - a0, nb = b1 - b0;
+    for (int j = k * 256 + t; j < na + nb; j += nnode * 256) {
+        const unsigned i = (unsigned)(j < na ? a0 + j : b0 + (j - na));
+        copy_state<R>(srcb, i, dstb, i);
+    }
+}
+
 // One 256-thread scatter block: entry bl = (beam, first particle) for a
-// particle block of 256 * PER particles, (beam, -(k + 1)) for node block k;
-// t is the thread's index in the block.  Shared by pic_scatter and the fused
+// particle block of 256 * PER particles, (beam, -(k + 1)) for node block k
+// (k >= nnode: move_block k - nnode of a one-cell beam); t is the thread's
+// index in the block.  Shared by pic_scatter and the fused
 // field + scatter launch.
 template <typename R, int PER>
 __device__ __forceinline__ void scatter_block(const SimDev<R>* __restrict__ sims,
@@ -930,8 +986,16 @@ __device__ __forceinline__ void scatter_block(const SimDev<R>* __restrict__ sims
         }
         return;
     }
-    // node threads: J', new start[] and the next live range over the union range
-    const int k = -bl.y - 1;
+    // node blocks k < nnode: J', new start[] and the next live range over
+    // the union range; one-cell beams add nnode blocks for the staged
+    // movers and the parked particles (a chain of loads of their own, run
+    // beside the first instead of after it)
+    const int kk = -bl.y - 1;
+    if (kk >= nnode) {
+        if (G == 1) move_block<R>(b, par, kk - nnode, t, cmin, cmax);
+        return;
+    }
+    const int k = kk;
     const int u0 = urng_p[0], u1 = urng_p[1];
     const int e0 = max(u0 - 3, 0), e1 = min(u1 + 4, N);
     const int nlo = st[2] + count_in(cdown, 2, cmin, cmax);      // new start of cell 2 (cell 1 sends nothing up)
@@ -953,53 +1017,6 @@ __device__ __forceinline__ void scatter_block(const SimDev<R>* __restrict__ sims
                 }
             }
         }
-    }
-    if (G != 1) return;
-    // the staged movers sit in this buffer (push/count wrote them over the
-    // start of their old cell's range)
-    const StateOut<R> dstb{ido, pfo, pxo, pyo, pzo};
-    const StateIn<R> srcb{idp, pf, px, py, pz};
-    // One-cell groups (push/count wrote every other particle's new state):
-    // the movers staged at start[c] + rank of this buffer, i.e. the down-movers of the first
-    // cell of each push/count block of 2 * kPushWaves cells and the up-movers
-    // of its last cell.  kXMove threads per block boundary, each taking every
-    // kXMove-th mover of both cells.
-    constexpr int kXMove = 8;
-    constexpr int kBlkCells = 2 * kPushWaves;
-    if (cmin <= cmax) {
-        const int lo_cell = max(cmin - 1, 0), hi_cell = min(cmax + 1, N);
-        const int nblk = (hi_cell - lo_cell + kBlkCells) / kBlkCells;
-        for (int j = k * 256 + t; j < nblk * kXMove; j += nnode * 256) {
-            const int md = lo_cell + (j / kXMove) * kBlkCells, mu = md + kBlkCells - 1;
-            const int r0 = j % kXMove;
-            if (md >= 2 && md <= N - 3) {
-                const int n = cdown[md];
-                const int src0 = st[md], dst0 = st[md] - count_in(cup, md - 1, cmin, cmax);
-                if (src0 < 0 || src0 + n > np || dst0 < 0 || dst0 + n > np) err[0] = 1;
-                else
-                    for (int r = r0; r < n; r += kXMove)
-                        copy_state<R>(srcb, (unsigned)(src0 + r), dstb, (unsigned)(dst0 + r));
-            }
-            if (mu >= 2 && mu <= min(hi_cell, N - 3)) {
-                const int n = cup[mu];
-                const int src0 = st[mu];
-                const int dst0 = st[mu + 1] - count_in(cup, mu, cmin, cmax) + count_in(cdown, mu + 1, cmin, cmax);
-                if (src0 < 0 || src0 + n > np || dst0 < 0 || dst0 + n > np) err[0] = 1;
-                else
-                    for (int r = r0; r < n; r += kXMove)
-                        copy_state<R>(srcb, (unsigned)(src0 + r), dstb, (unsigned)(dst0 + r));
-            }
-        }
-    }
-    // the particles parked at the previous step, [park[0], start[2]) and
-    // [start[N-2], park[1]): copied to the other buffer at the same index
-    // (from then on they are in both buffers and never copied again)
-    const int a0 = park[0], a1 = max(a0, st[2]);
-    const int b1 = park[1], b0 = min(st[N - 2], b1);
-    const int na = a1 - a0, nb = b1 - b0;
-    for (int j = k * 256 + t; j < na + nb; j += nnode * 256) {
-        const unsigned i = (unsigned)(j < na ? a0 + j : b0 + (j - na));
-        copy_state<R>(srcb, i, dstb, i);
     }
 }
 

@@ -204,6 +204,8 @@ struct TileCtx {
     double* Hprev;          // energy: H' before the launch's last H update, or null
     const double* pS;       // beam CIC sums (J' formed here), or null: J' read from Jc
     int pcmin, pcmax;
+    bool jpre;              // J' of the lane's cells formed at set-up (K <= 2): Jpre
+    double Jpre[2];
     R *probe_out, *snap;
 };
 
@@ -322,7 +324,10 @@ __device__ __forceinline__ void run_steps(TILE_PARAMS, const TileCtx<R>& c, cons
 #pragma unroll
         for (int j = 0; j < K; ++j) Jv[j] = R(0);
         if constexpr (GEN) {
-            if (c.pS) {
+            if (c.pS && c.jpre) {
+#pragma unroll
+                for (int j = 0; j < K; ++j) Jv[j] = (R)c.Jpre[j < 2 ? j : 0];
+            } else if (c.pS) {
 #pragma unroll
                 for (int j = 0; j < K; ++j) {
                     const int n = c.base + j;
@@ -508,9 +513,43 @@ __device__ __forceinline__ void tile_body(const SimDev<R>& sd, int out0_, long l
     c.pS = sd.pS;
     c.pcmin = 1;
     c.pcmax = 0;
+    c.jpre = false;
+    c.Jpre[0] = c.Jpre[1] = 0.0;
     if (c.pS) {
+        // J' of a launch is the same at each of its steps (the sums of this
+        // launch's push).  With K <= 2 it is formed here: the sums come in
+        // from clamped cells without waiting for the live range, which only
+        // masks them afterwards (the sum of j_from_sums, in its order).
+        double raw[K <= 2 ? K : 1][4];
+        if constexpr (K <= 2 && !PERSIST) {
+#pragma unroll
+            for (int j = 0; j < K; ++j)
+#pragma unroll
+                for (int w = 0; w < 4; ++w) {
+                    const int cc = min(max(c.base + j - 2 + w, 0), N + 1);
+                    raw[j][w] = c.pS[4 * (size_t)cc + (3 - w)];
+                }
+        }
         c.pcmin = sd.prange[par][0];
         c.pcmax = sd.prange[par][1];
+        if constexpr (K <= 2 && !PERSIST) {
+            c.jpre = true;
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                const int n = c.base + j;
+                double t = 0.0;
+                if (n >= 0 && n <= N) {
+#pragma unroll
+                    for (int w = 0; w < 4; ++w) {
+                        const int cc = n - 2 + w;
+                        const double v = (c.pcmin <= c.pcmax && cc >= c.pcmin - 1 && cc <= c.pcmax + 1) ? raw[j][w]
+                                                                                                       : 0.0;
+                        t = w == 0 ? v : t + v;
+                    }
+                }
+                c.Jpre[j] = t;
+            }
+        }
     }
 
     R E[K], H[K], cE[K], cH[K], ke[K], kh[K];
