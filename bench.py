@@ -184,7 +184,7 @@ def pic_bench(a):
     n, T, N = p["beam"]["n"], p["steps"], p["N"]
     pushes, cells = n * T, N * T
     bpc = configs.bytes_per_cell_update(p, "f32")
-    bpp = 40.0          # cell int32 + frac + ux, uy, uz (FP32), read and written
+    bpp = 40.0          # id int32 + frac + ux, uy, uz (FP32), read and written (one-cell beams keep no cell array)
     # per-step time of the whole timed run
     step_ms = wall * 1e3 / T
     achieved = (n * bpp + N * bpc) / (step_ms / 1e3) / 1e9
