@@ -18,6 +18,8 @@ ones checked only through runs that compose them:
     eps < 0 (no real index) gives 1.
   physical.dft: X(f) = dt sum x_t exp(-2 pi i f t dt) of a cosine on a DFT
     bin is A T dt / 2; linear.
+  gaussian_src (BaseFDTD11.Gaussian, :86-96): the peak 2A/Sc at step 440,
+    1/e of it 200 steps either side, the exponent's alternating sign at f = 0.5.
   mat_setup (Material_Def.matSetup, Material_Def.py:21-120): the grid of a
     5 GHz call worked by hand: dz = c0/(200 f), delT = 0.99 dz/c0 =
     0.99e-12 s, 900 cells, source at 250, slab 450..749, probes at 350 and
@@ -176,3 +178,33 @@ def test_mat_setup_hand_worked_5ghz():
     assert (eL, mL, ca, cb, da, db) == (0, 0, 1.0, 1.0, 1.0, 1.0)
     assert abs(dz - c0 / (200 * 5e9)) < 1e-18 and abs(dt - 0.99e-12) < 1e-25
     assert abs(cn - 0.99) < 1e-15 and abs(period - 2e-10) < 1e-25 and abs(lam - c0 / 5e9 * 0.1) < 1e-15
+
+
+# ---------------------------------------------------------------------------
+# gaussian_src (BaseFDTD11.Gaussian, :86-96): 2 A exp(-(t - 440)^2 / 200^2
+# cos(2 pi f (t - 440))) / Sc, t in steps: the cosine sits inside the
+# exponent (the reference's form), so with f = 1 it is a plain Gaussian and
+# with f = 0.5 the exponent's sign alternates step by step
+# ---------------------------------------------------------------------------
+def _gauss(fn, f):
+    P = SimpleNamespace(timeSteps=1000, freq_in=f, Amplitude=1.0, courantNo=0.5)
+    return np.asarray(fn(P))
+
+
+def _gauss_ok(g1, gh):
+    e = math.exp(1.0)
+    return (abs(g1[440] - 4.0) < 1e-15 and abs(g1[640] - 4.0 / e) < 1e-14 and abs(g1[240] - 4.0 / e) < 1e-14
+            and abs(gh[441] - 4.0 * math.exp(1.0 / 40000.0)) < 1e-14
+            and abs(gh[442] - 4.0 * math.exp(-4.0 / 40000.0)) < 1e-14)
+
+
+def test_gaussian_source_known_values():
+    assert _gauss_ok(_gauss(FO.gaussian_src, 1.0), _gauss(FO.gaussian_src, 0.5))
+
+
+def test_planted_modulated_gaussian_fails():
+    def textbook(P):                                   # cos as a carrier outside the exponent
+        t = np.arange(P.timeSteps)
+        env = np.exp(-((t - 440.0) ** 2) / 40000.0)
+        return P.Amplitude * 2 * env * np.cos(2 * np.pi * P.freq_in * (t - 440)) / P.courantNo
+    assert not _gauss_ok(_gauss(textbook, 1.0), _gauss(textbook, 0.5))
