@@ -73,13 +73,14 @@ struct PersistArgs {
 // start[c] is the first particle of cell c (start[N+1] = np).  Live
 // particles occupy cells [2, N-3]; cells 1 and N-2 park absorbed ones, which
 // keep their index from then on.
-constexpr int kPicGMax = 64;        // cells per push/count group (LDS tables)
-constexpr int kScatterPer = 2;      // particles per scatter thread (a block covers 256 * kScatterPer)
+constexpr int kPicGMax = 64;        // cells per push/count group (LDS tables; the last group of a
+                                    // range may take one more)
 // push/count waves per workgroup (a CU holds a limited number of workgroups;
 // single-wave workgroups would cap the resident waves; 8 measured slower:
 // config 3 push/count 13.0 against 11.6 us); the host pads the work list to
 // whole workgroups of this many waves
 constexpr int kPushWaves = 4;
+constexpr int kXMove = 8;           // move_block threads per push/count block boundary
 template <typename R>
 struct BeamDev {
     int64_t launch_begin;  // the simulation's, copied: the parity without a load of the sim descriptor
@@ -87,24 +88,18 @@ struct BeamDev {
     int32_t nwaves;      // push/count waves of this beam (grid-stride over cell groups; one-cell
                          // groups: whole workgroups of kPushWaves waves)
     int32_t nnode;       // scatter node blocks of this beam (grid-stride over nodes)
+    int32_t nmove;       // node blocks for its staged movers and parked particles (move_block)
     R a;                 // q dt / (2 m c)
     R jscale;            // Z0 q w c: J' of one particle per unit u_x/gamma
     R Sc;                // c dt / dz
-    int32_t* cell[2];
+    int32_t* cell[2];    // multi-cell beams only (a one-cell beam's cells follow from start[])
     R *frac[2], *ux[2], *uy[2], *uz[2];
     int32_t* id[2];
     int32_t* start[2];   // N+2
     int32_t* range[2];   // [cmin, cmax] of the live particles' cells (empty: cmin > cmax)
     int32_t* urange[2];  // union of the live ranges so far (where J and start[] can change)
-    // multi-cell groups: per-step record at the old (sorted) index,
-    // push/count -> scatter (the scatter re-runs the push for the new state,
-    // bitwise the same)
-    int32_t* trank;      // rank << 2 | (d + 1), d = new cell - old cell, rank among (cell, d)
     int32_t* park;       // [2]: the previous step's start[2] and start[N-2] (push/count -> scatter):
                          // particles below / at or above them are parked and in both buffers already
-    R* hold;             // H^{n-1/2} on the live range +-1, copied by push/count for the scatter's
-                         // push of multi-cell groups (the field step of the same launch may
-                         // overwrite that buffer)
     // per-cell results of the push/count pass, valid on the live range +-1
     int32_t *cup, *cdown;  // particles moving up / down out of the cell
     double* S;             // 4 per cell: J' sums onto nodes c-1, c, c+1, c+2

@@ -1,16 +1,12 @@
 // One launch for the field step and the particle scatter of a PIC step.
 //
-// The scatter reads nothing the field step writes (H^{n-1/2} comes from
-// push/count's copy in hold, and the field step forms J' from the CIC sums
-// itself), so both can share one grid of 256-thread workgroups: the
-// scatter's node blocks first (few, with short dependent load chains that
-// would otherwise trail the launch), then the field workgroups, four tiles
-// each, one per wave, so the tiles' long step chains start at once, then the
-// scatter's particle blocks (multi-cell beams), which fill the remaining
-// wave slots.  One launch instead of two saves the second launch's ramp and
-// drain.  The kernel has the field tile's register count, which lowers the
-// scatter's occupancy, so the host fuses only small field launches
-// (kFuseMaxTiles).
+// The scatter's node blocks read nothing the field step writes (the field
+// step forms J' from the CIC sums itself), so both can share one grid of
+// 256-thread workgroups: the node blocks first (few, with short dependent
+// load chains that would otherwise trail the launch), then the field
+// workgroups, four tiles each, one per wave.  One launch instead of two
+// saves the second launch's ramp and drain.  The host fuses only small field
+// launches (kFuseMaxTiles): the kernel has the field tile's register count.
 #pragma once
 #include "pic_kernels.hpp"
 #include "stencil_kernels.hpp"
@@ -28,7 +24,7 @@ __global__ __launch_bounds__(256) void field_scatter(const SimDev<R>* __restrict
     const int nfb = (ntiles + kFusedTileWaves - 1) / kFusedTileWaves;
     const int b = (int)blockIdx.x;
     if (b < nfirst || b >= nfirst + nfb) {
-        scatter_block<R, kScatterPer>(sims, beams, sblocks[b < nfirst ? b : b - nfb], launch_idx, (int)threadIdx.x);
+        scatter_block<R>(sims, beams, sblocks[b < nfirst ? b : b - nfb], launch_idx, (int)threadIdx.x);
         return;
     }
     // wave-uniform tile index, so the descriptors come in through scalar loads

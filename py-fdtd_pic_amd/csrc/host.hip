@@ -487,7 +487,7 @@ static int staged_upload(fdtd_ctx* c, const std::vector<Upload>& ups) {
 // particle beams
 // ------------------------------------------------------------------------
 enum { BO_CELL = 0, BO_FRAC = 2, BO_UX = 4, BO_UY = 6, BO_UZ = 8, BO_ID = 10, BO_START = 12, BO_RANGE = 14,
-       BO_URANGE = 16, BO_TRANK = 18, BO_CUP, BO_CDOWN, BO_S, BO_HOLD, BO_PARK, BO_END };
+       BO_URANGE = 16, BO_CUP = 18, BO_CDOWN, BO_S, BO_PARK, BO_END };
 
 template <typename R>
 static BeamDev<R> fill_beam_dev(BeamHost& b, int32_t* err, int64_t launch_begin) {
@@ -496,7 +496,7 @@ static BeamDev<R> fill_beam_dev(BeamHost& b, int32_t* err, int64_t launch_begin)
     char* blk = (char*)b.block;
     d.launch_begin = launch_begin;
     d.sim = b.sim; d.N = b.N; d.np = (int32_t)b.np; d.G = b.G;
-    d.nwaves = b.nwaves; d.nnode = b.nnode;
+    d.nwaves = b.nwaves; d.nnode = b.nnode; d.nmove = b.nmove;
     d.a = (R)b.a; d.jscale = (R)b.jscale; d.Sc = (R)b.Sc;
     for (int k = 0; k < 2; ++k) {
         d.cell[k] = (int32_t*)(blk + b.off[BO_CELL + k]);
@@ -509,8 +509,6 @@ static BeamDev<R> fill_beam_dev(BeamHost& b, int32_t* err, int64_t launch_begin)
         d.range[k] = (int32_t*)(blk + b.off[BO_RANGE + k]);
         d.urange[k] = (int32_t*)(blk + b.off[BO_URANGE + k]);
     }
-    d.trank = (int32_t*)(blk + b.off[BO_TRANK]);
-    d.hold = (R*)(blk + b.off[BO_HOLD]);
     d.park = (int32_t*)(blk + b.off[BO_PARK]);
     d.cup = (int32_t*)(blk + b.off[BO_CUP]);
     d.cdown = (int32_t*)(blk + b.off[BO_CDOWN]);
@@ -530,7 +528,8 @@ static int upload_beam(BeamHost& b, int N, hipStream_t st) {
         ux[i] = (R)b.u0[3 * i]; uy[i] = (R)b.u0[3 * i + 1]; uz[i] = (R)b.u0[3 * i + 2];
     }
     const size_t rn = sizeof(R) * n, in = sizeof(int32_t) * n, sn = sizeof(int32_t) * (N + 2);
-    HIPCHK(hipMemcpyAsync(blk + b.off[BO_CELL], b.cell0.data(), in, hipMemcpyHostToDevice, st));
+    if (b.G != 1)   // one-cell beams keep no cell array
+        HIPCHK(hipMemcpyAsync(blk + b.off[BO_CELL], b.cell0.data(), in, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(blk + b.off[BO_FRAC], fr.data(), rn, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(blk + b.off[BO_UX], ux.data(), rn, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(blk + b.off[BO_UY], uy.data(), rn, hipMemcpyHostToDevice, st));
@@ -629,20 +628,23 @@ static int attach_t(fdtd_ctx* c, int id, SimHost& s, const fdtd_beam_desc_t* bd)
         b->nwaves = (int)std::min<int64_t>(32768, std::max<int64_t>(1, (span + b->G - 1) / b->G));
     }
     b->nnode = std::min(64, std::max(1, (span + 8 + 255) / 256 + 1));
+    // move_block: kXMove threads per push/count block boundary (blocks of
+    // 2 * kPushWaves cells for one-cell beams, groups of G cells otherwise)
+    const int bcells = b->G == 1 ? 2 * kPushWaves : b->G;
+    const int nbound = (span + bcells - 1) / bcells + 2;
+    b->nmove = std::min(256, std::max(1, (nbound * kXMove + 255) / 256));
     // device block
     Layout L;
     const size_t rn = sizeof(R) * n, in = sizeof(int32_t) * n, cn = sizeof(int32_t) * (N + 2);
-    for (int k = 0; k < 2; ++k) L.add(in);                       // cell
+    const bool one = b->G == 1;
+    for (int k = 0; k < 2; ++k) L.add(one ? 0 : in);             // cell (multi-cell beams)
     for (int k = 0; k < 2; ++k) L.add(rn);                       // frac
     for (int k = 0; k < 6; ++k) L.add(rn);                       // ux0 ux1 uy0 uy1 uz0 uz1 (BO_UX/UY/UZ + k)
     for (int k = 0; k < 2; ++k) L.add(in);                       // id
     for (int k = 0; k < 2; ++k) L.add(cn);                       // start
     for (int k = 0; k < 4; ++k) L.add(2 * sizeof(int32_t));      // range x2, urange x2
-    const bool one = b->G == 1;
-    L.add(one ? 0 : in);                                         // trank (multi-cell groups)
     L.add(cn); L.add(cn);                                        // cup cdown
     L.add(4 * sizeof(double) * (N + 2));                         // S
-    L.add(one ? 0 : sizeof(R) * (N + 2));                        // hold (multi-cell groups)
     L.add(2 * sizeof(int32_t));                                  // park
     for (int k = 0; k < L.n; ++k) b->off[k] = L.off[k];
     b->bytes = L.total;
@@ -698,7 +700,7 @@ static int rebuild_t(fdtd_ctx* c) {
     }
     // particle beams of the active simulations
     std::vector<BeamDev<R>> bdev;
-    std::vector<int2> pblocks, xblocks, pitems, citems;
+    std::vector<int2> pblocks, pitems, citems;
     for (int i : c->tile_sims) {
         BeamHost* bh = i < (int)c->beams.size() ? c->beams[i] : nullptr;
         if (!bh) continue;
@@ -710,20 +712,14 @@ static int rebuild_t(fdtd_ctx* c) {
         } else {
             for (int w = 0; w < bh->nwaves; ++w) pitems.push_back(make_int2(bi, w));
         }
-        // node blocks first (they go ahead of the field workgroups in the
-        // fused launch); particle blocks for multi-cell groups only:
-        // push/count moves the particles of one-cell groups, and the node
-        // blocks the rest
-        // one-cell beams: as many again for the staged movers and parked particles
-        const int nodes = bh->G == 1 ? 2 * bh->nnode : bh->nnode;
-        for (int k = 0; k < nodes; ++k) pblocks.push_back(make_int2(bi, -(k + 1)));
-        if (bh->G != 1)
-            for (int64_t p = 0; p < bh->np; p += 256 * kScatterPer) xblocks.push_back(make_int2(bi, (int)p));
+        // node blocks (push/count writes every particle that stays within
+        // its push/count block): nnode for J', start[] and the range, nmove
+        // for the staged movers and the parked particles
+        for (int k = 0; k < bh->nnode + bh->nmove; ++k) pblocks.push_back(make_int2(bi, -(k + 1)));
     }
     while (pitems.size() % kPushWaves) pitems.push_back(make_int2(-1, 0));   // whole push/count workgroups
     c->has_pic = !bdev.empty();
     c->npnode = (int)pblocks.size();
-    pblocks.insert(pblocks.end(), xblocks.begin(), xblocks.end());
     c->npblocks = (int)pblocks.size();
     c->npitems = (int)pitems.size();
     c->npcell = (int)citems.size();

@@ -35,6 +35,7 @@ struct BeamHost {
     int64_t np = 0;
     int G = 1;                   // cells per push/count group
     int nwaves = 1, nnode = 1;   // push/count waves, scatter node blocks
+    int nmove = 1;               // scatter blocks for staged movers and parked particles
     int32_t range0[2] = {0, -1}; // live cell range at attach
     double q = 0, m = 0, w = 0;  // charge, mass, weight per unit transverse area
     double a = 0, jscale = 0, Sc = 0;

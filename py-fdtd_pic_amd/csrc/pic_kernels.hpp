@@ -8,34 +8,32 @@
 // deposit writes J' = Z0*Jx (J_x*dz in the reference's units).
 //
 // Step n: push/count, then the field launch of step n (which forms
-// J^{n+1/2} from the CIC sums) and the scatter, which need not wait for it:
-//   pic_push_count  one wave per group of G cells of the live range (grid-
-//                   stride): per particle, gather E^n at x^n and
-//                   B^n = -(H^{n-1/2} + H^{n+1/2})/(2c) at x^n - 1/2, Boris,
-//                   x^{n+1} = x^n + Sc u_z/gamma; per cell, the particles
-//                   moving down / up (|d| <= 1 since Sc*beta < 1), each
-//                   particle's stable rank among (cell, d), and four FP64
-//                   sums of the CIC weights of J' at x^{n+1/2} onto nodes
-//                   c-1 .. c+2 (one-cell groups: per-lane sums and a wave
-//                   scan; larger groups: segmented DPP scans; a fixed order
-//                   either way).  The new index of a particle is
+// J^{n+1/2} from the CIC sums) and the scatter's node blocks, which need not
+// wait for it:
+//   pic_push_cell   dense beams (one-cell groups): workgroups of four waves
+//                   take blocks of 8 cells, two cells per wave; per particle,
+//                   gather E^n at x^n and B^n = -(H^{n-1/2} + H^{n+1/2})/(2c)
+//                   at x^n - 1/2, Boris, x^{n+1} = x^n + Sc u_z/gamma; per
+//                   cell, the particles moving down / up (|d| <= 1 since
+//                   Sc*beta < 1), each particle's stable rank r among
+//                   (cell, d), and four FP64 sums of the CIC weights of J' at
+//                   x^{n+1/2} onto nodes c-1 .. c+2.  The new index is
 //                      d=-1: start[c] - up(c-1) + r
 //                      d= 0: start[c] + down(c) + r
 //                      d=+1: start[c+1] - up(c) + down(c+1) + r
-//                   One-cell groups (dense beams; two cells per wave) write
-//                   the new state of every particle whose index needs only
-//                   the wave's own counts, i.e. all but the movers into a
-//                   neighbouring wave's cells, which are staged; multi-cell
-//                   groups write each particle's (rank, d) record instead.
-//   pic_scatter     node threads: J'[m] = S+2(m-2) + S+1(m-1) + S0(m) +
+//                   and with the block's counts exchanged through LDS, every
+//                   particle whose new cell lies in the block is written to
+//                   the other buffer; the movers out of the block are staged.
+//   pic_push_count  sparser beams (groups of G cells, one wave each): the
+//                   same in two passes over the group (counts and sums, then
+//                   the pushes again with ranks and writes); movers out of
+//                   the group are staged.
+//   pic_scatter     node blocks: J'[m] = S+2(m-2) + S+1(m-1) + S0(m) +
 //                   S-1(m+1), the new start[] and the next live range (a
 //                   single writer each: the first and the last non-empty
-//                   cell); one-cell groups: the staged movers to their new
-//                   index, and the particles parked at the previous step
-//                   copied to the other buffer.  Multi-cell groups add one
-//                   thread per particle: the push again (bitwise the same:
-//                   no contraction in push_one) and the new state at the new
-//                   index (parked particles keep theirs).
+//                   cell); then the staged movers to their new index, and the
+//                   particles parked at the previous step copied to the
+//                   other buffer.
 // Absorbed particles stay in the array (cells 1 and N-2), so the particle
 // count is constant and no global scan is needed.
 #pragma once
@@ -190,10 +188,10 @@ struct PushArgs {
     int N;
 };
 
-// Both PIC kernels run it on the same inputs (push/count for the moves, ranks
-// and current; scatter for the new state) and must agree bit for bit, so no
-// FMA contraction here: a contraction decision can depend on the caller
-// (a product with two uses in one kernel and one in the other).
+// The multi-cell push/count runs it twice on the same particles (a counting
+// pass and a writing pass), which must agree bit for bit, so no FMA
+// contraction here: a contraction decision can depend on how the results
+// are used at each call.
 // The push after the gather: Eg, Hg are the fields at the particle.
 template <typename R>
 __device__ __forceinline__ void push_at(const PushArgs<R>& pa, int& c, R& f, R& ux, R& uy, R& uz, R Eg, R Hg,
@@ -233,25 +231,9 @@ __device__ __forceinline__ void push_at(const PushArgs<R>& pa, int& c, R& f, R& 
     f = nf;
 }
 
-// Linear gather of E at x and of the time-averaged H at x - 1/2.
-template <typename R>
-__device__ __forceinline__ void push_one(const PushArgs<R>& pa, int& c, R& f, R& ux, R& uy, R& uz, int& d,
-                                         int& off, R& a0, R& a1) {
-#pragma clang fp contract(off)
-    const gptr<const R> E = pa.E, Hc = pa.Hc, Hp = pa.Hp;
-    const R Eg = (R(1) - f) * ldo(E, c) + f * ldo(E, c + 1);
-    int j;
-    R g;
-    if (f >= R(0.5)) { j = c; g = f - R(0.5); } else { j = c - 1; g = f + R(0.5); }
-    const R h0 = R(0.5) * (ldo(Hp, j) + ldo(Hc, j));
-    const R h1 = R(0.5) * (ldo(Hp, j + 1) + ldo(Hc, j + 1));
-    const R Hg = (R(1) - g) * h0 + g * h1;
-    push_at<R>(pa, c, f, ux, uy, uz, Eg, Hg, d, off, a0, a1);
-}
-
 // The fields a particle of cell c can gather, wave-uniform when c is: E at
-// c and c+1, the time-averaged H at c-1, c and c+1 (each formed as push_one
-// forms it).  Loaded once per one-cell group, ahead of the particles.
+// c and c+1, and the time-averaged H, (H^{n-1/2} + H^{n+1/2})/2, at c-1, c
+// and c+1.  Loaded once per one-cell group, ahead of the particles.
 template <typename R>
 struct CellFields {
     R E0, E1, hm, h0, hp;
@@ -266,8 +248,9 @@ __device__ __forceinline__ CellFields<R> cell_fields(const PushArgs<R>& pa, int 
     return CellFields<R>{e0, e1, R(0.5) * (pm + qm), R(0.5) * (p0 + q0), R(0.5) * (pp + qp)};
 }
 
-// push_one for a particle of the cell cf was loaded for: the same values
-// and operations (bitwise the same result), no per-particle gather.
+// The push of a particle of the cell cf was loaded for: the linear gather of
+// E at x and of the time-averaged H at x - 1/2 (nodes c-1, c below the cell
+// centre, c, c+1 above it), then push_at.
 template <typename R>
 __device__ __forceinline__ void push_cell(const PushArgs<R>& pa, const CellFields<R>& cf, int& c, R& f, R& ux,
                                           R& uy, R& uz, int& d, int& off, R& a0, R& a1) {
@@ -431,7 +414,7 @@ __device__ __forceinline__ void g1_put_cell(const StateOut<R>& nxt, const StateO
 }
 
 // The fields of a multi-cell group, staged in the wave's LDS once per group:
-// E and the time-averaged H (formed as push_one forms it) at the nodes of
+// E and the time-averaged H (formed as cell_fields forms it) at the nodes of
 // cells c0 - 1 .. c0 + n (entry k holds node c0 - 1 + k, clamped to the grid).
 template <typename R>
 __device__ __forceinline__ void stage_fields(const PushArgs<R>& pa, R* Et, R* Ht, int c0, int n, int lane) {
@@ -443,8 +426,8 @@ __device__ __forceinline__ void stage_fields(const PushArgs<R>& pa, R* Et, R* Ht
     }
 }
 
-// push_one for a particle of cell c in [c0, c0 + n) with the staged fields:
-// the same values and operations (bitwise the same result).
+// The push of a particle of cell c in [c0, c0 + n) with the staged fields:
+// the gather of push_cell, then push_at.
 template <typename R>
 __device__ __forceinline__ void push_tab(const PushArgs<R>& pa, const R* Et, const R* Ht, int c0, int& c, R& f,
                                          R& ux, R& uy, R& uz, int& d, int& off, R& a0, R& a1) {
@@ -474,6 +457,77 @@ __device__ __forceinline__ void begin_step(const BeamDev<R>& b, int par, int N) 
     gp(b.park)[1] = gp((const int32_t*)b.start[po])[N - 2];
 }
 
+// Groups of a multi-cell beam over the count range [lo, hi] (at least three
+// cells): G cells each, the last taking a remainder of one cell (G + 1
+// cells), so every group has a first and a last cell of its own.
+__device__ __forceinline__ int mc_groups(int lo, int hi, int G) {
+    const int n = hi - lo + 1;
+    const int g = (n + G - 1) / G;
+    return (n % G == 1 && g > 1) ? g - 1 : g;
+}
+__device__ __forceinline__ int2 mc_group(int lo, int hi, int G, int ngroups, int j) {
+    const int g0 = lo + j * G;
+    return make_int2(g0, j == ngroups - 1 ? hi + 1 : g0 + G);
+}
+
+// kBatch chunks of a multi-cell group in registers (cell, state; id only
+// for the writing pass).
+template <typename R>
+struct McBatch {
+    int c[kBatch], id[kBatch];
+    R f[kBatch], ux[kBatch], uy[kBatch], uz[kBatch];
+};
+template <typename R, bool ID>
+__device__ __forceinline__ void mc_fetch(gptr<const int32_t> cell, gptr<const int32_t> pid, gptr<const R> pf,
+                                         gptr<const R> px, gptr<const R> py, gptr<const R> pz, int base, int np,
+                                         int lane, McBatch<R>& m) {
+#pragma unroll
+    for (int q = 0; q < kBatch; ++q) {
+        const unsigned i = (unsigned)min(base + 64 * q + lane, np - 1);
+        m.c[q] = ldo(cell, i);
+        m.id[q] = ID ? ldo(pid, i) : 0;
+        m.f[q] = ldo(pf, i);
+        m.ux[q] = ldo(px, i);
+        m.uy[q] = ldo(py, i);
+        m.uz[q] = ldo(pz, i);
+    }
+}
+
+// The new state of a multi-cell group's particle (cell c0 + k of the group's
+// n cells, move d, rank r; stl: start[] of the group's cells, tot: their
+// totals).  New index:
+//   d=-1: start[c] - up(c-1) + r      d=0: start[c] + down(c) + r
+//   d=+1: start[c+1] - up(c) + down(c+1) + r
+// The down-movers of the first cell and the up-movers of the last need a
+// neighbouring group's counts: they are staged at start[c] + r of this
+// buffer (inside the cell's own range, at or below the mover's own index,
+// which this wave has already read) and moved by the node blocks.
+template <typename R>
+__device__ __forceinline__ void mc_put(const StateOut<R>& nxt, const StateOut<R>& stg, gptr<int32_t> cello,
+                                       gptr<int32_t> cellp, const int* stl, const int (*tot)[3], int n, int k,
+                                       int d, int r, int c, const Pushed<R>& s, int np, gptr<int32_t> err) {
+    int ni;
+    bool staged = false;
+    if (d == 0) {
+        ni = stl[k] + tot[k][0] + r;
+    } else if (d < 0) {
+        staged = k == 0;
+        ni = staged ? stl[k] + r : stl[k] - tot[k - 1][2] + r;
+    } else {
+        staged = k == n - 1;
+        ni = staged ? stl[k] + r : stl[k + 1] - tot[k][2] + tot[k + 1][0] + r;
+    }
+    if ((unsigned)ni >= (unsigned)np) {
+        err[0] = 1;
+    } else if (staged) {
+        g1_put<R>(stg, ni, np, s, err);
+        sto(cellp, (unsigned)ni, c);
+    } else {
+        g1_put<R>(nxt, ni, np, s, err);
+        sto(cello, (unsigned)ni, c);
+    }
+}
+
 // Multi-cell groups (G > 1): kPushWaves independent waves per workgroup,
 // each with its own LDS tables; item kPushWaves * block + wave, items with
 // x < 0 are padding.
@@ -483,102 +537,100 @@ __global__ __launch_bounds__(64 * kPushWaves) void pic_push_count(const SimDev<R
                                                                   const BeamDev<R>* __restrict__ beams,
                                                                   const int2* __restrict__ items,
                                                                   long long launch_idx) {
-    __shared__ int cnt_all[kPushWaves][kPicGMax][3];    // down, stay, up so far per group cell (rank carry)
-    __shared__ double S_all[kPushWaves][kPicGMax][4];   // J' sums onto nodes c-1 .. c+2
-    __shared__ R fld_all[kPushWaves][2][kPicGMax + 2];  // the group's E and averaged H (stage_fields)
+    __shared__ int cnt_all[kPushWaves][kPicGMax + 1][3];   // down, stay, up so far per group cell (rank carry)
+    __shared__ int tot_all[kPushWaves][kPicGMax + 1][3];   // the group's totals (second pass)
+    __shared__ int stl_all[kPushWaves][kPicGMax + 2];      // start[] of the group's cells g0 .. g1
+    __shared__ double S_all[kPushWaves][kPicGMax + 1][4];  // J' sums onto nodes c-1 .. c+2
+    __shared__ R fld_all[kPushWaves][2][kPicGMax + 3];     // the group's E and averaged H (stage_fields)
     // wave-uniform by construction; readfirstlane lets the compiler see it, so
     // the item and the descriptors come in through scalar loads
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    int (&cnt)[kPicGMax][3] = cnt_all[wv];
-    double (&Sl)[kPicGMax][4] = S_all[wv];
+    int (&cnt)[kPicGMax + 1][3] = cnt_all[wv];
+    int (&tot)[kPicGMax + 1][3] = tot_all[wv];
+    int* const stl = stl_all[wv];
+    double (&Sl)[kPicGMax + 1][4] = S_all[wv];
     R* const Et = fld_all[wv][0];
     R* const Ht = fld_all[wv][1];
     const int2 it = items[blockIdx.x * kPushWaves + wv];
     if (it.x < 0) return;
-    const BeamDev<R>& b = beams[it.x];
-    const SimDev<R>& sd = sims[b.sim];
+    const cref<BeamDev<R>> b = ((cptr<BeamDev<R>>)beams)[it.x];
+    const cref<SimDev<R>> sd = ((cptr<SimDev<R>>)sims)[b.sim];
     const int par = (int)((launch_idx - b.launch_begin) & 1);
+    const int po = par ^ 1;
     const int lane = threadIdx.x & 63;
-    // every descriptor field is read here, ahead of the kernel's first store
     const int N = b.N, np = b.np, nwaves = b.nwaves;
-    const int G = max(1, min(b.G, kPicGMax));
+    const int G = max(2, min(b.G, kPicGMax));
     const PushArgs<R> pa{gp((const R*)sd.E[par]), gp((const R*)sd.H[par]), gp((const R*)sd.H[par ^ 1]),
                          b.a, b.jscale, b.Sc, N};
     const gptr<const int32_t> rng_p = gp((const int32_t*)b.range[par]);
     const gptr<const int32_t> st = gp((const int32_t*)b.start[par]);
-    const gptr<const int32_t> cell = gp((const int32_t*)b.cell[par]);
+    const gptr<const int32_t> cell = gp((const int32_t*)b.cell[par]), pid = gp((const int32_t*)b.id[par]);
     const gptr<const R> pf = gp((const R*)b.frac[par]), px = gp((const R*)b.ux[par]);
     const gptr<const R> py = gp((const R*)b.uy[par]), pz = gp((const R*)b.uz[par]);
-    const gptr<int32_t> trank = gp(b.trank), cup = gp(b.cup), cdown = gp(b.cdown);
-    const gptr<R> hold = gp(b.hold);
-    const gptr<double> Sg = gp(b.S);
     const int cmin = rng_p[0], cmax = rng_p[1];
-    if (it.y == 0 && lane == 0) begin_step<R>(b, par, N);
+    if (it.y == 0 && lane == 0) begin_step<R>(beams[it.x], par, N);
     if (cmin > cmax) return;
     const int lo_cell = max(cmin - 1, 0), hi_cell = min(cmax + 1, N);   // counts cover the range +-1
-    const int ngroups = (hi_cell - lo_cell + G) / G;
-    auto clampi = [&](int v) { return min(max(v, 0), np); };
+    const int ngroups = mc_groups(lo_cell, hi_cell, G);
     for (int gi = it.y; gi < ngroups; gi += nwaves) {
-        const int g0 = lo_cell + gi * G;
-        const int g1 = min(g0 + G, hi_cell + 1);
-        // live cells only, as above; both bounds come in with one load (lane 0
-        // and lane 1), read back with readlane
+        const int2 gb = mc_group(lo_cell, hi_cell, G, ngroups, gi);
+        const int g0 = gb.x, g1 = gb.y, n = g1 - g0;
+        // live cells only (the graveyard cells 1 and N-2 are skipped); both
+        // bounds come in with one load (lane 0 and lane 1), read back with readlane
         const int sb = ldo(st, (unsigned)(lane == 0 ? max(g0, 2) : min(g1, N - 2)));
-        for (int k = lane; k < G; k += 64) {
+        for (int k = lane; k < n; k += 64) {
             cnt[k][0] = cnt[k][1] = cnt[k][2] = 0;
             Sl[k][0] = Sl[k][1] = Sl[k][2] = Sl[k][3] = 0.0;
         }
-        stage_fields<R>(pa, Et, Ht, g0, g1 - g0, lane);
+        for (int k = lane; k <= n; k += 64) stl[k] = ldo(st, (unsigned)min(g0 + k, N + 1));
+        stage_fields<R>(pa, Et, Ht, g0, n, lane);
         wave_sync();
-        const int plo = clampi(__builtin_amdgcn_readlane(sb, 0));
-        const int phi = max(plo, clampi(__builtin_amdgcn_readlane(sb, 1)));
-        // particle loads one batch ahead: the next batch is in flight while
-        // this one is processed (the wave fences below do not wait for it);
-        // unconditional loads at an in-range index (np >= 1), selects on use
-        int nc[kBatch];
-        R nf[kBatch], nx[kBatch], ny[kBatch], nz[kBatch];
-        auto fetch = [&](int bb) {
+        const int plo = min(max(__builtin_amdgcn_readlane(sb, 0), 0), np);
+        const int phi = max(plo, min(max(__builtin_amdgcn_readlane(sb, 1), 0), np));
+        // A group of up to 64 * kBatch live particles is done in one pass:
+        // its ranks come out of the counting pass and its pushed state waits
+        // in registers for the totals (a lone sparse beam's groups of ~100
+        // particles are latency-bound, so a second pass would double their
+        // time).  Larger groups take two passes (below).
+        const bool one = phi - plo <= 64 * kBatch;
+        Pushed<R> hs[kBatch];
+        int hc[kBatch], hk[kBatch], hd[kBatch], hr[kBatch];
 #pragma unroll
-            for (int q = 0; q < kBatch; ++q) {
-                const unsigned ic = (unsigned)min(bb + 64 * q + lane, np - 1);
-                nc[q] = ldo(cell, ic);
-                nf[q] = ldo(pf, ic);
-                nx[q] = ldo(px, ic);
-                ny[q] = ldo(py, ic);
-                nz[q] = ldo(pz, ic);
-            }
-        };
-        if (plo < phi) fetch(plo);
+        for (int q = 0; q < kBatch; ++q) hk[q] = -1;      // -1: no particle held
+        // Pass 1: the moves, counts and CIC sums.  Particle loads one batch
+        // ahead: the next batch is in flight while this one is processed
+        // (the wave fences below do not wait for it); unconditional loads at
+        // an in-range index (np >= 1), selects on use.
+        McBatch<R> nb;
+        if (one) mc_fetch<R, true>(cell, pid, pf, px, py, pz, plo, np, lane, nb);
+        else mc_fetch<R, false>(cell, pid, pf, px, py, pz, plo, np, lane, nb);
         for (int bb = plo; bb < phi; bb += 64 * kBatch) {
-            int cc[kBatch];
-            R ff[kBatch], xx[kBatch], yy[kBatch], zz[kBatch];
-#pragma unroll
-            for (int q = 0; q < kBatch; ++q) {
-                const bool v = bb + 64 * q + lane < phi;
-                cc[q] = v ? nc[q] : 0x7fffffff;
-                ff[q] = v ? nf[q] : R(0);
-                xx[q] = v ? nx[q] : R(0);
-                yy[q] = v ? ny[q] : R(0);
-                zz[q] = v ? nz[q] : R(0);
-            }
-            if (bb + 64 * kBatch < phi) fetch(bb + 64 * kBatch);
+            McBatch<R> cb = nb;
+            if (bb + 64 * kBatch < phi) mc_fetch<R, false>(cell, pid, pf, px, py, pz, bb + 64 * kBatch, np, lane, nb);
 #pragma unroll
             for (int q = 0; q < kBatch; ++q) {
                 const int base = bb + 64 * q;
                 if (base >= phi) break;
                 const int i = base + lane;
-                const int c_old = cc[q];
+                const int c_old = i < phi ? cb.c[q] : 0x7fffffff;
                 const Chunk k = chunk_info(c_old, i, phi, lane);
                 int c = c_old, d = 0, off = 1;
-                R f = ff[q], ux = xx[q], uy = yy[q], uz = zz[q];
+                R f = cb.f[q], ux = cb.ux[q], uy = cb.uy[q], uz = cb.uz[q];
                 R a0 = R(0), a1 = R(0);
-                const bool live = k.valid && c_old >= 2 && c_old <= N - 3;
-                if (live) push_tab<R>(pa, Et, Ht, g0, c, f, ux, uy, uz, d, off, a0, a1);
+                if (k.valid) push_tab<R>(pa, Et, Ht, g0, c, f, ux, uy, uz, d, off, a0, a1);
                 const unsigned long long seg = lanes_le(lane) & ~lanes_lt(k.fl);
                 const unsigned long long bd0 = __ballot(k.valid && d == -1);
                 const unsigned long long bd1 = __ballot(k.valid && d == 0);
                 const unsigned long long bd2 = __ballot(k.valid && d == 1);
-                const unsigned long long mine = d < 0 ? bd0 : (d == 0 ? bd1 : bd2);
+                if (one && k.valid) {
+                    const unsigned long long mine = d < 0 ? bd0 : (d == 0 ? bd1 : bd2);
+                    hk[q] = c_old - g0;
+                    hd[q] = d;
+                    hr[q] = cnt[hk[q]][d + 1] + __popcll(mine & lanes_lt(lane) & ~lanes_lt(k.fl));
+                    hc[q] = c;
+                    hs[q].id = cb.id[q];
+                    hs[q].f = f; hs[q].ux = ux; hs[q].uy = uy; hs[q].uz = uz;
+                }
                 // the chunk's segment sums of the weights onto nodes c-1+off,
                 // c+off, in R (FP32: at most 64 weights, then added into the
                 // FP64 cell sums in a fixed order)
@@ -589,32 +641,80 @@ __global__ __launch_bounds__(64 * kPushWaves) void pic_push_count(const SimDev<R
                 v[3] = off == 2 ? a1 : R(0);
                 seg_scan4<R>(v, lane, k.fl);
                 const int lk = k.valid ? c_old - g0 : -1;
-                const bool in = lk >= 0 && lk < G;
-                int r = 0;
-                if (in) r = cnt[lk][d + 1] + __popcll(mine & lanes_lt(lane) & ~lanes_lt(k.fl));
-                wave_sync();
-                if (in && k.last) {
+                if (k.valid && k.last) {
                     cnt[lk][0] += __popcll(bd0 & seg);
                     cnt[lk][1] += __popcll(bd1 & seg);
                     cnt[lk][2] += __popcll(bd2 & seg);
 #pragma unroll
                     for (int w = 0; w < 4; ++w) Sl[lk][w] += (double)v[w];
                 }
-                if (in) sto(trank, i, (r << 2) | (d + 1));
                 wave_sync();
             }
         }
-        for (int k = lane; k < g1 - g0; k += 64) {
+        // the group's totals: out to the node blocks, and kept for pass 2
+        const gptr<int32_t> cup = gp(b.cup), cdown = gp(b.cdown);
+        const gptr<double> Sg = gp(b.S);
+        for (int k = lane; k < n; k += 64) {
             const int c = g0 + k;
             cup[c] = cnt[k][2];
             cdown[c] = cnt[k][0];
 #pragma unroll
             for (int w = 0; w < 4; ++w) Sg[4 * (size_t)c + w] = Sl[k][w];
-            // H^{n-1/2} of the group's cells for the scatter's push: the groups
-            // tile [cmin-1, cmax+1], the H nodes the live particles' gathers touch
-            hold[c] = pa.Hp[c];
+            tot[k][0] = cnt[k][0];
+            tot[k][1] = cnt[k][1];
+            tot[k][2] = cnt[k][2];
+            cnt[k][0] = cnt[k][1] = cnt[k][2] = 0;
         }
         wave_sync();
+        // The new state (mc_put): from registers for a one-pass group; else
+        // pass 2, the same pushes again (bitwise the same moves; the group's
+        // ~1,000 particles come back from L2/MALL) with the ranks.
+        const StateOut<R> nxt{gp(b.id[po]), gp(b.frac[po]), gp(b.ux[po]), gp(b.uy[po]), gp(b.uz[po])};
+        const StateOut<R> stg{gp(b.id[par]), gp(b.frac[par]), gp(b.ux[par]), gp(b.uy[par]), gp(b.uz[par])};
+        const gptr<int32_t> cello = gp(b.cell[po]), cellp = gp(b.cell[par]);
+        const gptr<int32_t> err = gp(b.err);
+        if (one) {
+#pragma unroll
+            for (int q = 0; q < kBatch; ++q)
+                if (hk[q] >= 0)
+                    mc_put<R>(nxt, stg, cello, cellp, stl, tot, n, hk[q], hd[q], hr[q], hc[q], hs[q], np, err);
+            continue;
+        }
+        if (plo < phi) mc_fetch<R, true>(cell, pid, pf, px, py, pz, plo, np, lane, nb);
+        for (int bb = plo; bb < phi; bb += 64 * kBatch) {
+            McBatch<R> cb = nb;
+            if (bb + 64 * kBatch < phi) mc_fetch<R, true>(cell, pid, pf, px, py, pz, bb + 64 * kBatch, np, lane, nb);
+#pragma unroll
+            for (int q = 0; q < kBatch; ++q) {
+                const int base = bb + 64 * q;
+                if (base >= phi) break;
+                const int i = base + lane;
+                const int c_old = i < phi ? cb.c[q] : 0x7fffffff;
+                const Chunk k = chunk_info(c_old, i, phi, lane);
+                Pushed<R> s;
+                s.id = cb.id[q];
+                s.f = cb.f[q]; s.ux = cb.ux[q]; s.uy = cb.uy[q]; s.uz = cb.uz[q];
+                int c = c_old, d = 0, off = 1;
+                R a0 = R(0), a1 = R(0);
+                if (k.valid) push_tab<R>(pa, Et, Ht, g0, c, s.f, s.ux, s.uy, s.uz, d, off, a0, a1);
+                const unsigned long long seg = lanes_le(lane) & ~lanes_lt(k.fl);
+                const unsigned long long bd0 = __ballot(k.valid && d == -1);
+                const unsigned long long bd1 = __ballot(k.valid && d == 0);
+                const unsigned long long bd2 = __ballot(k.valid && d == 1);
+                const unsigned long long mine = d < 0 ? bd0 : (d == 0 ? bd1 : bd2);
+                const int lk = k.valid ? c_old - g0 : 0;
+                int r = 0;
+                if (k.valid) r = cnt[lk][d + 1] + __popcll(mine & lanes_lt(lane) & ~lanes_lt(k.fl));
+                wave_sync();
+                if (k.valid && k.last) {
+                    cnt[lk][0] += __popcll(bd0 & seg);
+                    cnt[lk][1] += __popcll(bd1 & seg);
+                    cnt[lk][2] += __popcll(bd2 & seg);
+                }
+                if (k.valid) mc_put<R>(nxt, stg, cello, cellp, stl, tot, n, lk, d, r, c, s, np, err);
+                wave_sync();
+            }
+        }
     }
 }
 
@@ -844,18 +944,30 @@ __device__ __forceinline__ int count_in(gptr<const int32_t> a, int c, int cmin, 
     return (cmin <= cmax && c >= cmin - 1 && c <= cmax + 1) ? a[c] : 0;
 }
 
-// The second half of a one-cell beam's node blocks (block k of nnode): the
-// movers push/count staged at start[c] + rank of this buffer, i.e. the
-// down-movers of the first cell of each push/count block of 2 * kPushWaves
-// cells and the up-movers of its last cell (kXMove threads per block
-// boundary, each taking every kXMove-th mover of both cells), and the
-// particles parked at the previous step, [park[0], start[2]) and
-// [start[N-2], park[1]), copied to the other buffer at the same index (from
-// then on they are in both buffers and never copied again).
+// One particle's state from index s of one buffer to index d of the other,
+// with its cell when the beam keeps a cell array (mc).
+template <typename R>
+__device__ __forceinline__ void move_one(const StateIn<R>& a, const StateOut<R>& b, gptr<const int32_t> cells,
+                                         gptr<int32_t> celld, bool mc, unsigned s, unsigned d) {
+    copy_state<R>(a, s, b, d);
+    if (mc) sto(celld, d, ldo(cells, s));
+}
+
+// The second part of a beam's node blocks (block k of nmove): the movers
+// push/count staged at start[c] + rank of this buffer, i.e. the down-movers
+// of the first cell of each push/count block (one-cell beams: blocks of
+// 2 * kPushWaves cells; multi-cell beams: the groups of mc_group) and the
+// up-movers of its last cell (kXMove threads per block boundary, each taking
+// every kXMove-th mover of both cells), and the particles parked at the
+// previous step, [park[0], start[2]) and [start[N-2], park[1]), copied to the
+// other buffer at the same index (from then on they are in both buffers and
+// never copied again).  Multi-cell beams carry their cell array along.
 template <typename R>
 __device__ __forceinline__ void move_block(const BeamDev<R>& b, int par, int k, int t, int cmin, int cmax) {
     const int po = par ^ 1;
-    const int N = b.N, np = b.np, nnode = b.nnode;
+    const int N = b.N, np = b.np, nmove = b.nmove;
+    const bool mc = b.G != 1;
+    const int G = max(2, min(b.G, kPicGMax));
     const gptr<const int32_t> st = gp((const int32_t*)b.start[par]);
     const gptr<const int32_t> cup = gp((const int32_t*)b.cup), cdown = gp((const int32_t*)b.cdown);
     const gptr<const int32_t> park = gp((const int32_t*)b.park);
@@ -863,21 +975,30 @@ __device__ __forceinline__ void move_block(const BeamDev<R>& b, int par, int k, 
     const StateOut<R> dstb{gp(b.id[po]), gp(b.frac[po]), gp(b.ux[po]), gp(b.uy[po]), gp(b.uz[po])};
     const StateIn<R> srcb{gp((const int32_t*)b.id[par]), gp((const R*)b.frac[par]), gp((const R*)b.ux[par]),
                           gp((const R*)b.uy[par]), gp((const R*)b.uz[par])};
-    constexpr int kXMove = 8;
+    const gptr<const int32_t> cells = gp((const int32_t*)b.cell[par]);
+    const gptr<int32_t> celld = gp(b.cell[po]);
     constexpr int kBlkCells = 2 * kPushWaves;
     if (cmin <= cmax) {
         const int lo_cell = max(cmin - 1, 0), hi_cell = min(cmax + 1, N);
-        const int nblk = (hi_cell - lo_cell + kBlkCells) / kBlkCells;
-        for (int j = k * 256 + t; j < nblk * kXMove; j += nnode * 256) {
-            const int md = lo_cell + (j / kXMove) * kBlkCells, mu = md + kBlkCells - 1;
-            const int r0 = j % kXMove;
+        const int nblk = mc ? mc_groups(lo_cell, hi_cell, G) : (hi_cell - lo_cell + kBlkCells) / kBlkCells;
+        for (int j = k * 256 + t; j < nblk * kXMove; j += nmove * 256) {
+            const int g = j / kXMove, r0 = j % kXMove;
+            int md, mu;
+            if (mc) {
+                const int2 gb = mc_group(lo_cell, hi_cell, G, nblk, g);
+                md = gb.x;
+                mu = gb.y - 1;
+            } else {
+                md = lo_cell + g * kBlkCells;
+                mu = md + kBlkCells - 1;
+            }
             if (md >= 2 && md <= N - 3) {
                 const int n = cdown[md];
                 const int src0 = st[md], dst0 = st[md] - count_in(cup, md - 1, cmin, cmax);
                 if (src0 < 0 || src0 + n > np || dst0 < 0 || dst0 + n > np) err[0] = 1;
                 else
                     for (int r = r0; r < n; r += kXMove)
-                        copy_state<R>(srcb, (unsigned)(src0 + r), dstb, (unsigned)(dst0 + r));
+                        move_one<R>(srcb, dstb, cells, celld, mc, (unsigned)(src0 + r), (unsigned)(dst0 + r));
             }
             if (mu >= 2 && mu <= min(hi_cell, N - 3)) {
                 const int n = cup[mu];
@@ -886,25 +1007,26 @@ __device__ __forceinline__ void move_block(const BeamDev<R>& b, int par, int k, 
                 if (src0 < 0 || src0 + n > np || dst0 < 0 || dst0 + n > np) err[0] = 1;
                 else
                     for (int r = r0; r < n; r += kXMove)
-                        copy_state<R>(srcb, (unsigned)(src0 + r), dstb, (unsigned)(dst0 + r));
+                        move_one<R>(srcb, dstb, cells, celld, mc, (unsigned)(src0 + r), (unsigned)(dst0 + r));
             }
         }
     }
     const int a0 = park[0], a1 = max(a0, st[2]);
     const int b1 = park[1], b0 = min(st[N - 2], b1);
     const int na = a1 - a0, nb = b1 - b0;
-    for (int j = k * 256 + t; j < na + nb; j += nnode * 256) {
+    for (int j = k * 256 + t; j < na + nb; j += nmove * 256) {
         const unsigned i = (unsigned)(j < na ? a0 + j : b0 + (j - na));
-        copy_state<R>(srcb, i, dstb, i);
+        move_one<R>(srcb, dstb, cells, celld, mc, i, i);
     }
 }
 
-// One 256-thread scatter block: entry bl = (beam, first particle) for a
-// particle block of 256 * PER particles, (beam, -(k + 1)) for node block k
-// (k >= nnode: move_block k - nnode of a one-cell beam); t is the thread's
-// index in the block.  Shared by pic_scatter and the fused
-// field + scatter launch.
-template <typename R, int PER>
+// One 256-thread scatter block (node work; push/count has written every
+// particle that stays within its push/count block): entry bl = (beam,
+// -(k + 1)); k < nnode: J', the new start[] and the next live range over the
+// union range; nnode <= k < nnode + nmove: move_block k - nnode.  t is the
+// thread's index in
+// the block.  Shared by pic_scatter and the fused field + scatter launch.
+template <typename R>
 __device__ __forceinline__ void scatter_block(const SimDev<R>* __restrict__ sims,
                                               const BeamDev<R>* __restrict__ beams, const int2 bl,
                                               long long launch_idx, const int t) {
@@ -913,86 +1035,20 @@ __device__ __forceinline__ void scatter_block(const SimDev<R>* __restrict__ sims
     const int par = (int)((launch_idx - b.launch_begin) & 1);
     const int po = par ^ 1;
     // every descriptor field is read here, ahead of the kernel's first store
-    // (scalar loads), as in pic_push_count
-    const int N = b.N, np = b.np, nnode = b.nnode, G = b.G;
-    // H^{n-1/2} from push/count's copy: the field step of this launch may run
-    // alongside and overwrite H[par ^ 1]
-    const PushArgs<R> pa{gp((const R*)sd.E[par]), gp((const R*)sd.H[par]), gp((const R*)b.hold),
-                         b.a, b.jscale, b.Sc, N};
+    // (scalar loads)
+    const int N = b.N, nnode = b.nnode;
     const gptr<const int32_t> st = gp((const int32_t*)b.start[par]);
     const gptr<const int32_t> cup = gp((const int32_t*)b.cup), cdown = gp((const int32_t*)b.cdown);
-    const gptr<const int32_t> trank = gp((const int32_t*)b.trank);
-    const gptr<const int32_t> cell = gp((const int32_t*)b.cell[par]), idp = gp((const int32_t*)b.id[par]);
-    const gptr<const R> pf = gp((const R*)b.frac[par]), px = gp((const R*)b.ux[par]);
-    const gptr<const R> py = gp((const R*)b.uy[par]), pz = gp((const R*)b.uz[par]);
-    const gptr<int32_t> cello = gp(b.cell[po]), ido = gp(b.id[po]), st_o = gp(b.start[po]);
-    const gptr<R> pfo = gp(b.frac[po]), pxo = gp(b.ux[po]), pyo = gp(b.uy[po]), pzo = gp(b.uz[po]);
+    const gptr<int32_t> st_o = gp(b.start[po]);
     const gptr<const double> Sg = gp((const double*)b.S);
-    const gptr<int32_t> err = gp(b.err);
-    const gptr<const int32_t> park = gp((const int32_t*)b.park);
     const gptr<const int32_t> rng_p = gp((const int32_t*)b.range[par]), urng_p = gp((const int32_t*)b.urange[par]);
     const gptr<int32_t> rng_o = gp(b.range[po]), urng_o = gp(b.urange[po]);
     const gptr<R> J = gp(const_cast<R*>(sd.J));
     const int cmin = rng_p[0], cmax = rng_p[1];
     // counts and sums exist on [cmin-1, cmax+1] this step; zero elsewhere
-    if (bl.y >= 0) {
-        // PER particles per thread, 256 apart: every load of all of
-        // them is issued before the first store (the stores go to the other
-        // buffer, which the compiler cannot tell apart from this one)
-        const int p0 = park[0], p1 = park[1];
-        int ii[PER], c0[PER], id[PER], tr[PER];
-        R f[PER], ux[PER], uy[PER], uz[PER];
-        bool todo[PER];
-#pragma unroll
-        for (int k = 0; k < PER; ++k) {
-            const int i = bl.y + 256 * k + t;
-            // parked since the previous step: already in both buffers at index i
-            todo[k] = i < np && i >= p0 && i < p1;
-            const unsigned ic = (unsigned)min(i, np - 1);
-            ii[k] = i;
-            c0[k] = ldo(cell, ic);
-            f[k] = ldo(pf, ic);
-            ux[k] = ldo(px, ic);
-            uy[k] = ldo(py, ic);
-            uz[k] = ldo(pz, ic);
-            id[k] = ldo(idp, ic);
-            tr[k] = ldo(trank, ic);
-        }
-#pragma unroll
-        for (int k = 0; k < PER; ++k) {
-            if (!todo[k]) continue;
-            const int i = ii[k], cc = c0[k];
-            int c = cc, ni = i;           // parked: unchanged, same index
-            if (cc >= 2 && cc <= N - 3) {
-                const int d = (tr[k] & 3) - 1, r = tr[k] >> 2;
-                int dn, off;
-                R a0, a1;
-                push_one<R>(pa, c, f[k], ux[k], uy[k], uz[k], dn, off, a0, a1);   // the push of pic_push_count
-                if (d < 0) ni = st[cc] - count_in(cup, cc - 1, cmin, cmax) + r;
-                else if (d == 0) ni = st[cc] + count_in(cdown, cc, cmin, cmax) + r;
-                else
-                    ni = st[cc + 1] - count_in(cup, cc, cmin, cmax) + count_in(cdown, cc + 1, cmin, cmax) + r;
-                if (dn != d || ni < 0 || ni >= np) {
-                    err[0] = 1;
-                    continue;
-                }
-            }
-            sto(cello, ni, c);
-            sto(pfo, ni, f[k]);
-            sto(pxo, ni, ux[k]);
-            sto(pyo, ni, uy[k]);
-            sto(pzo, ni, uz[k]);
-            sto(ido, ni, id[k]);
-        }
-        return;
-    }
-    // node blocks k < nnode: J', new start[] and the next live range over
-    // the union range; one-cell beams add nnode blocks for the staged
-    // movers and the parked particles (a chain of loads of their own, run
-    // beside the first instead of after it)
     const int kk = -bl.y - 1;
     if (kk >= nnode) {
-        if (G == 1) move_block<R>(b, par, kk - nnode, t, cmin, cmax);
+        move_block<R>(b, par, kk - nnode, t, cmin, cmax);
         return;
     }
     const int k = kk;
@@ -1024,7 +1080,7 @@ template <typename R>
 __global__ __launch_bounds__(256) void pic_scatter(const SimDev<R>* __restrict__ sims,
                                                    const BeamDev<R>* __restrict__ beams,
                                                    const int2* __restrict__ blocks, long long launch_idx) {
-    scatter_block<R, kScatterPer>(sims, beams, blocks[blockIdx.x], launch_idx, (int)threadIdx.x);
+    scatter_block<R>(sims, beams, blocks[blockIdx.x], launch_idx, (int)threadIdx.x);
 }
 
 }  // namespace fdtd

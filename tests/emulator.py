@@ -295,3 +295,87 @@ def emulate_one_cell_writes(cell, d_live, start, prev_start, crange, N, waves=4)
     for i in list(range(a0, a1)) + list(range(b0, b1)):
         put(dst, i, i)
     return dst, cup, cdown
+
+
+def mc_groups(lo, hi, G):
+    """Groups of a multi-cell beam over [lo, hi] (pic_kernels.hpp: mc_groups,
+    mc_group): G cells each, the last taking a one-cell remainder."""
+    n = hi - lo + 1
+    g = (n + G - 1) // G
+    ng = g - 1 if (n % G == 1 and g > 1) else g
+    return [(lo + j * G, hi + 1 if j == ng - 1 else lo + j * G + G) for j in range(ng)]
+
+
+def emulate_multi_cell_writes(cell, d_live, start, prev_start, crange, N, G):
+    """The writes of one step on the multi-cell path (pic_kernels.hpp,
+    pic_push_count and move_block): per group, the totals of a counting
+    pass, then each particle's rank and new index in the writing pass; the
+    down-movers of a group's first cell and the up-movers of its last are
+    staged at start[c] + r and moved by the node blocks.  Returns
+    (dst, cup, cdown) as emulate_one_cell_writes does."""
+    n = len(cell)
+    live = (cell >= 2) & (cell <= N - 3)
+    d = np.where(live, d_live, 0)
+    cup = np.zeros(N + 2, dtype=np.int64)
+    cdown = np.zeros(N + 2, dtype=np.int64)
+    dst = np.full(n, -1, dtype=np.int64)
+    staged = np.full(n, -1, dtype=np.int64)
+
+    def put(arr, j, src):
+        assert 0 <= j < n, (j, n)
+        assert arr[j] == -1, ("written twice", j)
+        arr[j] = src
+
+    cmin, cmax = crange
+    lo_cell, hi_cell = max(cmin - 1, 0), min(cmax + 1, N)
+    groups = mc_groups(lo_cell, hi_cell, G) if cmin <= cmax else []
+    for g0, g1 in groups:
+        m = g1 - g0
+        assert m >= 2
+        plo = start[max(g0, 2)]
+        phi = max(plo, start[min(g1, N - 2)])
+        tot = np.zeros((m, 3), dtype=np.int64)
+        for i in range(plo, phi):
+            tot[cell[i] - g0, d[i] + 1] += 1
+        stl = [start[min(g0 + k, N + 1)] for k in range(m + 1)]
+        carry = np.zeros((m, 3), dtype=np.int64)
+        for i in range(plo, phi):
+            k, cls = cell[i] - g0, d[i] + 1
+            r = carry[k, cls]
+            carry[k, cls] += 1
+            if d[i] == 0:
+                put(dst, stl[k] + tot[k, 0] + r, i)
+            elif d[i] < 0:
+                if k == 0:
+                    assert stl[k] + r <= i
+                    put(staged, stl[k] + r, i)
+                else:
+                    put(dst, stl[k] - tot[k - 1, 2] + r, i)
+            else:
+                if k == m - 1:
+                    assert stl[k] + r <= i
+                    put(staged, stl[k] + r, i)
+                else:
+                    put(dst, stl[k + 1] - tot[k, 2] + tot[k + 1, 0] + r, i)
+        for k in range(m):
+            cup[g0 + k], cdown[g0 + k] = tot[k, 2], tot[k, 0]
+    inr = lambda c: cmin <= cmax and cmin - 1 <= c <= cmax + 1
+    up = lambda c: cup[c] if inr(c) else 0
+    down = lambda c: cdown[c] if inr(c) else 0
+    moved = set()
+    for g0, g1 in groups:
+        md, mu = g0, g1 - 1
+        if 2 <= md <= N - 3:
+            for r in range(cdown[md]):
+                put(dst, start[md] - up(md - 1) + r, staged[start[md] + r])
+                moved.add(int(staged[start[md] + r]))
+        if 2 <= mu <= min(hi_cell, N - 3):
+            for r in range(cup[mu]):
+                put(dst, start[mu + 1] - up(mu) + down(mu + 1) + r, staged[start[mu] + r])
+                moved.add(int(staged[start[mu] + r]))
+    assert moved == set(staged[staged >= 0].tolist())
+    a0, b1 = prev_start[2], prev_start[N - 2]
+    a1, b0 = max(a0, start[2]), min(start[N - 2], b1)
+    for i in list(range(a0, a1)) + list(range(b0, b1)):
+        put(dst, i, i)
+    return dst, cup, cdown

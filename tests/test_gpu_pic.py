@@ -128,11 +128,13 @@ def test_permutation_invariance_bitwise():
 
 
 def test_wave_width_independence():
-    """Cells per push/count group change only the order of the FP64 CIC sums."""
+    """Cells per push/count group change only the order of the FP64 CIC sums.
+    Groups of 3 cells (~100 particles) take push/count's one-pass path, 5 and
+    64 its two passes, 1 the one-cell kernel."""
     p = _params("driven", steps=300)
     ref = _device(p, "f64", group=0)
     x0 = _oracle_inputs(p)
-    for g in (1, 5, 64):
+    for g in (1, 3, 5, 64):
         d = _device(p, "f64", group=g)
         assert rel_l2(d["E"], ref["E"]) < 1e-13, g
         assert rel_l2(d["x"] - x0, ref["x"] - x0) < 1e-13, g

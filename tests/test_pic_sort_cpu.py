@@ -10,7 +10,7 @@ through the steps."""
 import numpy as np
 import pytest
 
-from emulator import emulate_one_cell_writes, emulate_pic_step
+from emulator import emulate_multi_cell_writes, emulate_one_cell_writes, emulate_pic_step, mc_groups
 
 
 def _moves(rng, cell, N):
@@ -110,6 +110,7 @@ def test_one_cell_pairs_cover_each_cell_once(N, lo, hi, n):
         assert sum(p1 - p0 for _, p0, p1 in got) == int(live.sum())
 
 
+@pytest.mark.parametrize("G", [1, 2, 5, 64])
 @pytest.mark.parametrize("N,n,lo,hi,steps", [
     (200, 3000, 2, 198, 5),          # dense: up to ~40 per cell
     (80, 9000, 20, 40, 4),           # cells of several hundred (the two-pass case)
@@ -118,11 +119,12 @@ def test_one_cell_pairs_cover_each_cell_once(N, lo, hi, n):
     (60, 3000, 54, 58, 8),           # most of the beam parked in the right graveyard
     (64, 1, 30, 31, 3),              # one particle
 ])
-def test_one_cell_path_buffers(N, n, lo, hi, steps):
+def test_write_path_buffers(N, n, lo, hi, steps, G):
     """Two ping-pong buffers through several steps of random moves on the
-    one-cell-group path: after every step the buffer written holds every
-    particle at its sorted index (the indices of emulate_pic_step), parked
-    particles included, although the long-parked ones are never rewritten."""
+    one-cell path (G = 1) and the multi-cell path: after every step the
+    buffer written holds every particle at its sorted index (the indices of
+    emulate_pic_step), parked particles included, although the long-parked
+    ones are never rewritten."""
     rng = np.random.default_rng(N * 11 + n)
     cell = np.sort(rng.integers(lo, hi, n))
     start = np.searchsorted(cell, np.arange(N + 2), side="left").astype(np.int64)
@@ -138,8 +140,11 @@ def test_one_cell_path_buffers(N, n, lo, hi, steps):
         off = np.ones(n, dtype=np.int64)
         zero = np.zeros(n)
         new_idx, nstart, _, nrange, nurange, cup_ref, cdown_ref = emulate_pic_step(
-            cell, d, off, zero, zero, start, crange, urange, 1, N)
-        dst, cup, cdown = emulate_one_cell_writes(cell, d, start, prev_start, crange, N)
+            cell, d, off, zero, zero, start, crange, urange, max(G, 1), N)
+        if G == 1:
+            dst, cup, cdown = emulate_one_cell_writes(cell, d, start, prev_start, crange, N)
+        else:
+            dst, cup, cdown = emulate_multi_cell_writes(cell, d, start, prev_start, crange, N, G)
         lo_cell, hi_cell = max(crange[0] - 1, 0), min(crange[1] + 1, N)
         if crange[0] <= crange[1]:
             sl = slice(lo_cell, hi_cell + 1)
@@ -157,3 +162,13 @@ def test_one_cell_path_buffers(N, n, lo, hi, steps):
         cell, crange, urange = newc[order], nrange, nurange
         ids = ids[order]
         assert np.array_equal(buf[par ^ 1], ids)
+
+
+@pytest.mark.parametrize("lo,hi,G", [(1, 3, 2), (1, 4, 2), (5, 70, 64), (5, 69, 64), (2, 200, 7), (10, 12, 64)])
+def test_multi_cell_groups_tile_the_range(lo, hi, G):
+    """Groups cover [lo, hi] once, in order, each with at least two cells
+    (a one-cell remainder joins the last group), at most G + 1."""
+    g = mc_groups(lo, hi, G)
+    assert g[0][0] == lo and g[-1][1] == hi + 1
+    assert all(a[1] == b[0] for a, b in zip(g, g[1:]))
+    assert all(2 <= b - a <= G + 1 for a, b in g)
