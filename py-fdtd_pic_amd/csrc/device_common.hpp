@@ -26,7 +26,8 @@ struct SimDev {
     const R *s_ke, *s_kh, *s_bE, *s_cE, *s_CbE, *s_bH, *s_cH, *s_C2H;
     const R *sl_A, *sl_B, *sl_C, *sl_e;         // FP64: A,B,C ; FP32: -B, 1-A-B, C
     const R *srcE, *srcH;
-    R *probe_out;                               // nprobe x T
+    double *probe_out;                          // nprobe x T, FP64 (an FP32 field as hi - c of its
+                                                // Kahan pair)
     R *snap;                                    // snap_rows x (N+1)
     double* Hprev;                              // N+1 or null: H' before the last H update of a launch
                                                 // (compensated, FP64), for the energy sum

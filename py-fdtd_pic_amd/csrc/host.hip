@@ -26,8 +26,7 @@ struct OpArgs {
     int lo[2], hi[2];
 };
 hipError_t launch_step_op(int op, const OpArgs& g, hipStream_t st);
-template <typename R>
-hipError_t launch_dft(const R* const* series, const int* lens, int nseries, const double* freqs, int nf, double dt,
+hipError_t launch_dft(const double* const* series, const int* lens, int nseries, const double* freqs, int nf, double dt,
                       double* out, hipStream_t st);
 template <typename R>
 hipError_t launch_energy(const EnergyJob<R>* jobs, int n, hipStream_t st);
@@ -110,7 +109,7 @@ static void fill_dev(SimHost& s, SimDev<R>& d) {
         d.D[b] = P(k++); d.P[b] = P(k++); d.Pm[b] = P(k++); d.cD[b] = P(k++); d.cP[b] = P(k++);
         d.psiE[b] = P(k++); d.psiH[b] = P(k++);
     }
-    d.probe_out = P(k++);
+    d.probe_out = (double*)P(k++);
     d.snap = P(k++);
     d.J = q.has_current ? P(k++) : nullptr;
     d.Hprev = s.e_every > 0 ? (double*)s.eblock : nullptr;
@@ -134,7 +133,7 @@ static int build_sim(fdtd_ctx* ctx, SimHost& s, const fdtd_sim_desc_t* q) {
         for (int i = 0; i < 5; ++i) L.add(rs * std::max(ns, 1));      // D P Pm cD cP
         for (int i = 0; i < 2; ++i) L.add(rs * std::max(nstrip, 1));  // psiE psiH
     }
-    L.add(rs * (size_t)std::max(q->nprobe, 1) * T);                   // probes
+    L.add(sizeof(double) * (size_t)std::max(q->nprobe, 1) * T);       // probes (FP64 in both precisions)
     L.add(rs * (size_t)std::max(q->snap_rows, 1) * (N + 1));           // snapshots
     if (q->has_current) L.add(rs * (N + 1));                           // J
     for (int i = 0; i < L.n; ++i) s.lay_off[i] = L.off[i];
@@ -1355,7 +1354,7 @@ int fdtd_read_probes(fdtd_ctx* c, int id, double* out, int64_t len) {
     const int64_t n = (int64_t)s->d.nprobe * s->d.steps;
     if (len != n) return fdtd_fail(FDTD_E_ARG, "length mismatch: expected " + std::to_string(n));
     if (c->prec == FDTD_F64) { SimDev<double> d; fill_dev<double>(*s, d); return read_plain<double>(c, d.probe_out, n, out); }
-    SimDev<float> d; fill_dev<float>(*s, d); return read_plain<float>(c, d.probe_out, n, out);
+    SimDev<float> d; fill_dev<float>(*s, d); return read_plain<double>(c, d.probe_out, n, out);
 }
 
 int fdtd_read_snapshots(fdtd_ctx* c, int id, double* out, int64_t len) {
@@ -1449,10 +1448,10 @@ int pic_read(fdtd_ctx* c, int id, double* x, double* u, int8_t* alive, int64_t n
 // ------------------------------------------------------------------------
 // spectra of probe series (FP64 DFT on the device)
 // ------------------------------------------------------------------------
-// The series of probe `probe` of each listed simulation (device pointers
-// and lengths, host side).
+// The series of probe `probe` of each listed simulation (device pointers to
+// its FP64 samples, and lengths; host side).
 template <typename R>
-static int spectra_series(fdtd_ctx* c, int nsims, const int* ids, int probe, const R** ptrs, int* lens) {
+static int spectra_series(fdtd_ctx* c, int nsims, const int* ids, int probe, const double** ptrs, int* lens) {
     for (int k = 0; k < nsims; ++k) {
         SimHost* s = get_sim(c, ids[k]);
         if (!s) return fdtd_fail(FDTD_E_ARG, "bad sim id " + std::to_string(ids[k]));
@@ -1467,15 +1466,15 @@ static int spectra_series(fdtd_ctx* c, int nsims, const int* ids, int probe, con
 
 // Uploads the tables into the device buffers given, runs the DFT and copies
 // the results to out; synchronous.
-template <typename R>
-static int spectra_run(fdtd_ctx* c, int nsims, const R* const* ptrs, const int* lens, const double* freqs, int nf,
-                       double dt, double* out, const R** d_ptrs, int* d_lens, double* d_freqs, double* d_out) {
+static int spectra_run(fdtd_ctx* c, int nsims, const double* const* ptrs, const int* lens, const double* freqs,
+                       int nf, double dt, double* out, const double** d_ptrs, int* d_lens, double* d_freqs,
+                       double* d_out) {
     HIPCHK(hipMemcpyAsync(d_ptrs, ptrs, sizeof(void*) * nsims, hipMemcpyHostToDevice, c->st));
     HIPCHK(hipMemcpyAsync(d_lens, lens, sizeof(int) * nsims, hipMemcpyHostToDevice, c->st));
     HIPCHK(hipMemcpyAsync(d_freqs, freqs, sizeof(double) * nf, hipMemcpyHostToDevice, c->st));
     for (int k0 = 0; k0 < nsims; k0 += 32768) {
         const int nk = std::min(32768, nsims - k0);
-        hipError_t e = launch_dft<R>(d_ptrs + k0, d_lens + k0, nk, d_freqs, nf, dt, d_out + 2 * (size_t)k0 * nf, c->st);
+        hipError_t e = launch_dft(d_ptrs + k0, d_lens + k0, nk, d_freqs, nf, dt, d_out + 2 * (size_t)k0 * nf, c->st);
         if (e != hipSuccess) return fdtd_fail(FDTD_E_HIP, std::string("dft launch: ") + hipGetErrorString(e));
     }
     HIPCHK(hipMemcpyAsync(out, d_out, sizeof(double) * 2 * (size_t)nsims * nf, hipMemcpyDeviceToHost, c->st));
@@ -1486,7 +1485,7 @@ static int spectra_run(fdtd_ctx* c, int nsims, const R* const* ptrs, const int* 
 template <typename R>
 static int spectra_t(fdtd_ctx* c, int nsims, const int* ids, int probe, const double* freqs, int nf, double dt,
                      double* out) {
-    std::vector<const R*> ptrs(nsims);
+    std::vector<const double*> ptrs(nsims);
     std::vector<int> lens(nsims);
     int rc = spectra_series<R>(c, nsims, ids, probe, ptrs.data(), lens.data());
     if (rc) return rc;
@@ -1502,8 +1501,8 @@ static int spectra_t(fdtd_ctx* c, int nsims, const int* ids, int probe, const do
         c->d_scratch_cap = need;
     }
     char* base = (char*)c->d_scratch;
-    return spectra_run<R>(c, nsims, ptrs.data(), lens.data(), freqs, nf, dt, out, (const R**)base,
-                          (int*)(base + o_lens), (double*)(base + o_freqs), (double*)(base + o_out));
+    return spectra_run(c, nsims, ptrs.data(), lens.data(), freqs, nf, dt, out, (const double**)base,
+                       (int*)(base + o_lens), (double*)(base + o_freqs), (double*)(base + o_out));
 }
 
 // The same spectra through the library's static scratch array of this
@@ -1513,12 +1512,12 @@ static int spectra_noalloc_t(fdtd_ctx* c, int nsims, const int* ids, int probe, 
                              double dt, double* out) {
     char* base = (char*)noalloc_scratch(c->device);
     if (!base) return fdtd_fail(FDTD_E_STATE, "no-allocation scratch not resolved on this device");
-    const R* ptrs[kNaMaxSims];
+    const double* ptrs[kNaMaxSims];
     int lens[kNaMaxSims];
     int rc = spectra_series<R>(c, nsims, ids, probe, ptrs, lens);
     if (rc) return rc;
-    return spectra_run<R>(c, nsims, ptrs, lens, freqs, nf, dt, out, (const R**)base, (int*)(base + kNaOffLens),
-                          (double*)(base + kNaOffFreqs), (double*)(base + kNaOffOut));
+    return spectra_run(c, nsims, ptrs, lens, freqs, nf, dt, out, (const double**)base, (int*)(base + kNaOffLens),
+                       (double*)(base + kNaOffFreqs), (double*)(base + kNaOffOut));
 }
 
 extern "C" int fdtd_probe_spectra(fdtd_ctx* c, int nsims, const int* sim_ids, int probe, const double* freqs, int nf,

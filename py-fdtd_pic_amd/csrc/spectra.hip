@@ -7,8 +7,9 @@
 // at the transmission probe, the two-pass protocol of
 // Solver_Engine.py:275-371) need only these per-frequency sums.
 //
-// One block per (simulation, frequency chunk of 64): the block stages the
-// probe series through LDS in tiles, each thread owns one frequency and
+// The series are FP64 in both precisions (the step kernel writes an FP32
+// field as hi - c of its Kahan pair).  One block per (simulation, frequency
+// chunk of 64): the block stages the probe series through LDS in tiles, each thread owns one frequency and
 // advances its twiddle by a complex multiply per sample, re-anchored with an
 // exact sincos every 64 samples so the recurrence error stays ~1e-15.
 #include <hip/hip_runtime.h>
@@ -19,14 +20,13 @@ namespace fdtd {
 
 constexpr int kDftTile = 256;
 
-template <typename R>
-__global__ __launch_bounds__(64) void probe_dft(const R* const* __restrict__ series, const int* __restrict__ lens,
+__global__ __launch_bounds__(64) void probe_dft(const double* const* __restrict__ series, const int* __restrict__ lens,
                                                 const double* __restrict__ freqs, int nf, double dt,
                                                 double* __restrict__ out) {
     __shared__ double xs[kDftTile];
     const int s = blockIdx.y;
     const int f = blockIdx.x * 64 + threadIdx.x;
-    const R* x = series[s];
+    const double* x = series[s];
     const int T = lens[s];
     const double w = f < nf ? -2.0 * M_PI * freqs[f] * dt : 0.0;
     double re = 0.0, im = 0.0;
@@ -35,7 +35,7 @@ __global__ __launch_bounds__(64) void probe_dft(const R* const* __restrict__ ser
     for (int t0 = 0; t0 < T; t0 += kDftTile) {
         const int n = min(kDftTile, T - t0);
         __syncthreads();
-        for (int k = threadIdx.x; k < n; k += 64) xs[k] = (double)x[t0 + k];
+        for (int k = threadIdx.x; k < n; k += 64) xs[k] = x[t0 + k];
         __syncthreads();
         for (int k = 0; k < n; ++k) {
             if ((k & 63) == 0) sincos(w * (double)(t0 + k), &ci, &cr);
@@ -53,18 +53,11 @@ __global__ __launch_bounds__(64) void probe_dft(const R* const* __restrict__ ser
     }
 }
 
-template <typename R>
-hipError_t launch_dft(const R* const* series, const int* lens, int nseries, const double* freqs, int nf, double dt,
-                      double* out, hipStream_t st) {
+hipError_t launch_dft(const double* const* series, const int* lens, int nseries, const double* freqs, int nf,
+                      double dt, double* out, hipStream_t st) {
     if (nseries <= 0 || nf <= 0) return hipSuccess;
-    hipLaunchKernelGGL((probe_dft<R>), dim3((nf + 63) / 64, nseries), dim3(64), 0, st, series, lens, freqs, nf, dt,
-                       out);
+    hipLaunchKernelGGL(probe_dft, dim3((nf + 63) / 64, nseries), dim3(64), 0, st, series, lens, freqs, nf, dt, out);
     return hipGetLastError();
 }
-
-template hipError_t launch_dft<double>(const double* const*, const int*, int, const double*, int, double, double*,
-                                       hipStream_t);
-template hipError_t launch_dft<float>(const float* const*, const int*, int, const double*, int, double, double*,
-                                      hipStream_t);
 
 }  // namespace fdtd

@@ -206,7 +206,8 @@ struct TileCtx {
     int pcmin, pcmax;
     bool jpre;              // J' of the lane's cells formed at set-up (K <= 2): Jpre
     double Jpre[2];
-    R *probe_out, *snap;
+    double* probe_out;      // FP64 samples (FP32 fields: hi - c of the Kahan pair)
+    R* snap;
 };
 
 // What a tile's step loop has to handle (compile-time, one loop per class).
@@ -298,7 +299,8 @@ __device__ __forceinline__ void run_steps(TILE_PARAMS, const TileCtx<R>& c, cons
     // The source and TFSF cells enter through 0/1 masks per lane and cell:
     // E + 1*v is E + v exactly and E + 0*v is E, so the injection needs no
     // branch on the lane or the cell.
-    R srcEc[kChunks], srcHc[kChunks], pacc[2], ms[K], mt[K];
+    R srcEc[kChunks], srcHc[kChunks], ms[K], mt[K];
+    double pacc[2];
     if constexpr (SRC) {
 #pragma unroll
         for (int j = 0; j < K; ++j) {
@@ -314,7 +316,7 @@ __device__ __forceinline__ void run_steps(TILE_PARAMS, const TileCtx<R>& c, cons
         }
     }
     if constexpr (PROBE) {
-        pacc[0] = pacc[1] = R(0);
+        pacc[0] = pacc[1] = 0.0;
     }
     // the step body, called twice per loop trip: with two copies the
     // loop-carried registers alternate instead of being moved back each step
@@ -395,7 +397,10 @@ __device__ __forceinline__ void run_steps(TILE_PARAMS, const TileCtx<R>& c, cons
 #pragma unroll
             for (int k = 0; k < 2; ++k)
                 if (k < c.npr) {
-                    const R v = lane_read(pick<R, K>(E, c.pr_rel[k] % K), c.pr_rel[k] / K);
+                    // the sample in FP64: an FP32 field as hi - c of its Kahan
+                    // pair (the value fdtd_read_field returns)
+                    double v = (double)lane_read(pick<R, K>(E, c.pr_rel[k] % K), c.pr_rel[k] / K);
+                    if (KAHAN) v -= (double)lane_read(pick<R, K>(cE, c.pr_rel[k] % K), c.pr_rel[k] / K);
                     pacc[k] = lane_write(v, ix & 63, pacc[k]);
                 }
         }
@@ -422,7 +427,10 @@ __device__ __forceinline__ void run_steps(TILE_PARAMS, const TileCtx<R>& c, cons
                     if (!((c.probe_bits >> p) & 1u)) continue;
                     if (t < sd.probe_t0[p] || t > sd.probe_t1[p]) continue;
                     const int rel = sd.probe_cell[p] - c.t0;
-                    if (c.lane == rel / K) c.probe_out[(long long)p * c.T + t] = pick<R, K>(E, rel % K);
+                    if (c.lane == rel / K)
+                        c.probe_out[(long long)p * c.T + t] =
+                            KAHAN ? (double)pick<R, K>(E, rel % K) - (double)pick<R, K>(cE, rel % K)
+                                  : (double)pick<R, K>(E, rel % K);
                 }
             }
             if (c.snap_interval > 0 && t > 0 && (t % c.snap_interval) == 0) {

@@ -73,6 +73,22 @@ def test_fp32_within_tolerance():
         assert v <= 1e-5, (k, v)
 
 
+@pytest.mark.parametrize("K,S,Ks", [(1, 1, 1), (4, 8, 4), (8, 64, 4)])
+def test_fp32_probes_are_the_compensated_field(K, S, Ks):
+    """FP32 fields are Kahan pairs (value = hi - c); the probe series holds
+    that difference in FP64, so the last sample equals the field read back
+    (sim.read returns hi - c in FP64) bitwise, and the series is not a set of
+    FP32 values (a series stored as hi alone would be)."""
+    spec = _spec(chi3=1e-3, src_amp=2.0, steps=1300)
+    print("K", K, "S", S, "Ks", Ks, flush=True)
+    d = _device(spec, "f32", K=K, S=S, K_slab=Ks)
+    pr, E = d["probes"], d["E"]
+    assert pr.dtype == np.float64 and pr.shape == (2, 1300)
+    assert np.array_equal(pr[:, -1], E[spec["probes"]])
+    nz = pr[pr != 0]
+    assert nz.size > 500 and np.mean(nz != nz.astype(np.float32).astype(np.float64)) > 0.5
+
+
 @pytest.mark.parametrize("K,S,Ks", [(1, 3, 1), (1, 31, 1), (2, 16, 2), (4, 8, 4), (4, 100, 4), (8, 64, 8),
                                     (8, 255, 8), (4, 48, 2), (2, 20, 1), (8, 100, 4)])
 def test_tilings_agree(K, S, Ks):
