@@ -25,9 +25,11 @@
 //                   particle whose new cell lies in the block is written to
 //                   the other buffer; the movers out of the block are staged.
 //   pic_push_count  sparser beams (groups of G cells, one wave each): the
-//                   same in two passes over the group (counts and sums, then
-//                   the pushes again with ranks and writes); movers out of
-//                   the group are staged.
+//                   same; a group of up to 128 particles is pushed once and
+//                   held in registers across its counts, a larger one takes
+//                   a counting pass and then a pass that pushes again
+//                   (bitwise the same) and writes; movers out of the group
+//                   are staged.
 //   pic_scatter     node blocks: J'[m] = S+2(m-2) + S+1(m-1) + S0(m) +
 //                   S-1(m+1), the new start[] and the next live range (a
 //                   single writer each: the first and the last non-empty

@@ -299,8 +299,9 @@ __device__ __forceinline__ void run_steps(TILE_PARAMS, const TileCtx<R>& c, cons
     // The source and TFSF cells enter through 0/1 masks per lane and cell:
     // E + 1*v is E + v exactly and E + 0*v is E, so the injection needs no
     // branch on the lane or the cell.
-    R srcEc[kChunks], srcHc[kChunks], ms[K], mt[K];
-    double pacc[2];
+    // probe samples of the current 64 steps, lane j holding step j: the field
+    // and (FP32) its Kahan compensation, combined in FP64 when stored
+    R srcEc[kChunks], srcHc[kChunks], pacc[2], pcc[2], ms[K], mt[K];
     if constexpr (SRC) {
 #pragma unroll
         for (int j = 0; j < K; ++j) {
@@ -316,7 +317,7 @@ __device__ __forceinline__ void run_steps(TILE_PARAMS, const TileCtx<R>& c, cons
         }
     }
     if constexpr (PROBE) {
-        pacc[0] = pacc[1] = 0.0;
+        pacc[0] = pacc[1] = pcc[0] = pcc[1] = R(0);
     }
     // the step body, called twice per loop trip: with two copies the
     // loop-carried registers alternate instead of being moved back each step
@@ -397,11 +398,12 @@ __device__ __forceinline__ void run_steps(TILE_PARAMS, const TileCtx<R>& c, cons
 #pragma unroll
             for (int k = 0; k < 2; ++k)
                 if (k < c.npr) {
-                    // the sample in FP64: an FP32 field as hi - c of its Kahan
-                    // pair (the value fdtd_read_field returns)
-                    double v = (double)lane_read(pick<R, K>(E, c.pr_rel[k] % K), c.pr_rel[k] / K);
-                    if (KAHAN) v -= (double)lane_read(pick<R, K>(cE, c.pr_rel[k] % K), c.pr_rel[k] / K);
+                    const R v = lane_read(pick<R, K>(E, c.pr_rel[k] % K), c.pr_rel[k] / K);
                     pacc[k] = lane_write(v, ix & 63, pacc[k]);
+                    if (KAHAN) {
+                        const R cv = lane_read(pick<R, K>(cE, c.pr_rel[k] % K), c.pr_rel[k] / K);
+                        pcc[k] = lane_write(cv, ix & 63, pcc[k]);
+                    }
                 }
         }
         // end of a 64-step chunk: flush probe samples, rotate source chunks
@@ -412,7 +414,10 @@ __device__ __forceinline__ void run_steps(TILE_PARAMS, const TileCtx<R>& c, cons
 #pragma unroll
                     for (int k = 0; k < 2; ++k)
                         if (k < c.npr && c.lane <= (ix & 63) && tt >= c.pr_t0[k] && tt <= c.pr_t1[k])
-                            c.probe_out[(long long)c.pr_id[k] * c.T + tt] = pacc[k];
+                            // FP64 in both precisions: an FP32 field as hi - c of
+                            // its Kahan pair (the value fdtd_read_field returns)
+                            c.probe_out[(long long)c.pr_id[k] * c.T + tt] =
+                                KAHAN ? (double)pacc[k] - (double)pcc[k] : (double)pacc[k];
                 }
                 if constexpr (SRC) {
 #pragma unroll
