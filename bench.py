@@ -249,16 +249,17 @@ def config4_bench(a):
     # per-simulation loop: a seeded sample from each stratum (with / without a
     # beam) run alone one after the other; each stratum's time per unit of
     # work (cell-updates + 4 x particle-pushes) scales to its total work.  A
-    # beam alone gets the push/count group size best for one beam (~96
-    # particles per group: enough waves to fill the GPU), not the larger
-    # groups the library picks by default for a full batched context
+    # beam alone gets the push/count group size best for one beam (~48
+    # particles per group: enough waves to fill the GPU; 12 seeded lone beam
+    # jobs took 2.46 s against 2.51 s at ~96 and 2.63 s at ~128), not the
+    # larger groups the library picks by default for a full batched context
     def lone(p):
         if not p.get("beam"):
             return p
         b = p["beam"]
         ppc = b["n"] / (b["cell_hi"] - b["cell_lo"])
         q = dict(p)
-        q["beam"] = dict(b, group=max(1, min(64, int(round(96.0 / ppc)))))
+        q["beam"] = dict(b, group=max(1, min(64, int(round(48.0 / ppc)))))
         return q
 
     import numpy as np
@@ -295,7 +296,13 @@ def config5_bench(a):
     arr, ps = configs.config5_requests()
     n = a.c5_requests
     srv = serve.Server("f32", budget_bytes=200 << 30)
-    jobs, wall = srv.replay(arr[:n], ps[:n])
+    last = [time.perf_counter()]
+
+    def log(msg):                        # completions, at most one line per 20 s
+        if time.perf_counter() - last[0] >= 20.0:
+            last[0] = time.perf_counter()
+            progress(f"config5 {msg}")
+    jobs, wall = srv.replay(arr[:n], ps[:n], log=log)
     srv.close()
     out = serve.summarize(jobs, wall)
     out.update(metric="config5 served stream (trace prefix replay)",

@@ -163,6 +163,24 @@ int fdtd_reset_sim(fdtd_ctx *ctx, int sim_id);
 /* Steps a simulation has completed. */
 int fdtd_sim_progress(fdtd_ctx *ctx, int sim_id, int64_t *steps_done);
 
+/* ---- checkpoint / resume ---------------------------------------------- */
+/* The reference holds a run's state in memory only (Ex_History,
+ * Solver_Engine.py:57-68, allocated at MasterController.py:160-166) and has
+ * no checkpoint.  A checkpoint is one simulation's device state (both
+ * ping-pong buffers of fields, ADE and CPML state, probes and snapshot rows
+ * so far, the attached beam's particles and per-cell results, the energy
+ * rows) plus its step position, written into a caller buffer of
+ * fdtd_state_size bytes.  fdtd_state_save drains the context stream first.
+ * fdtd_state_load copies one into a simulation built from the same
+ * description in a context of the same precision (with the same beam
+ * attached and the same energy setting; FDTD_E_ARG otherwise): it continues
+ * from the saved step, and in a context of the same tiling, with launches
+ * of whole S-step slices, its results are bitwise those of the run that was
+ * saved.  Coefficients are not part of a checkpoint. */
+int fdtd_state_size(fdtd_ctx *ctx, int sim_id, int64_t *bytes);
+int fdtd_state_save(fdtd_ctx *ctx, int sim_id, void *buf, int64_t len);
+int fdtd_state_load(fdtd_ctx *ctx, int sim_id, const void *buf, int64_t len);
+
 /* ---- stepping --------------------------------------------------------- */
 /* Advances every active, unfinished simulation by up to nsteps steps
  * (the `for counts in range(P.timeSteps)` bodies of Solver_Engine.py:167,236,294).

@@ -1110,6 +1110,134 @@ int fdtd_sync(fdtd_ctx* c) {
     return c->has_pic ? check_pic_flag(c) : FDTD_OK;
 }
 
+}  // extern "C"
+
+// Checkpoint of one simulation: a header, the energy rows' step indices, then
+// the device bytes of the simulation block's state region (from the first
+// state sub-array: fields of both ping-pong buffers, ADE and CPML state,
+// probes, snapshot rows, J), of the beam block and of the energy block, as
+// the device holds them.  The coefficients are not saved: a checkpoint is
+// loaded into a simulation built from the same description.
+namespace {
+struct CkptHeader {
+    char magic[8];
+    int32_t prec, N, steps, nprobe;
+    int64_t state_bytes, beam_bytes, e_bytes, beam_np;
+    int64_t steps_done, launches, rel_launch, rel_step;
+    int64_t e_count, e_launches, n_esteps;
+};
+constexpr char kCkptMagic[8] = {'F', 'D', 'T', 'D', 'C', 'K', 'P', '1'};
+
+struct CkptParts {
+    size_t state0 = 0, state_bytes = 0, beam_bytes = 0, e_bytes = 0;
+    const BeamHost* b = nullptr;
+    int64_t total = 0;
+};
+
+CkptParts ckpt_parts(fdtd_ctx* c, int id, const SimHost* s) {
+    CkptParts p;
+    p.state0 = s->lay_off[14];
+    p.state_bytes = s->bytes - p.state0;
+    p.b = (id < (int)c->beams.size()) ? c->beams[id] : nullptr;
+    p.beam_bytes = p.b ? p.b->bytes : 0;
+    p.e_bytes = s->eblock ? s->ebytes : 0;
+    p.total = (int64_t)(sizeof(CkptHeader) + sizeof(int64_t) * s->e_steps.size() + p.state_bytes + p.beam_bytes +
+                        p.e_bytes);
+    return p;
+}
+}  // namespace
+
+extern "C" {
+
+static SimHost* ckpt_sim(fdtd_ctx* c, int id) {
+    return (c && id >= 0 && id < (int)c->sims.size()) ? c->sims[id] : nullptr;
+}
+
+int fdtd_state_size(fdtd_ctx* c, int id, int64_t* bytes) {
+    SimHost* s = ckpt_sim(c, id);
+    if (!s || !bytes) return fdtd_fail(FDTD_E_ARG, "bad sim id or null output");
+    *bytes = ckpt_parts(c, id, s).total;
+    return FDTD_OK;
+}
+
+int fdtd_state_save(fdtd_ctx* c, int id, void* buf, int64_t len) {
+    SimHost* s = ckpt_sim(c, id);
+    if (!s || !buf) return fdtd_fail(FDTD_E_ARG, "bad sim id or null buffer");
+    const CkptParts p = ckpt_parts(c, id, s);
+    if (len < p.total) return fdtd_fail(FDTD_E_ARG, "checkpoint buffer too small (fdtd_state_size)");
+    if (int rc = fdtd_sync(c)) return rc;           // the state after every queued launch
+    CkptHeader h{};
+    std::memcpy(h.magic, kCkptMagic, sizeof(h.magic));
+    h.prec = c->prec;
+    h.N = s->d.N;
+    h.steps = s->d.steps;
+    h.nprobe = s->d.nprobe;
+    h.state_bytes = (int64_t)p.state_bytes;
+    h.beam_bytes = (int64_t)p.beam_bytes;
+    h.e_bytes = (int64_t)p.e_bytes;
+    h.beam_np = p.b ? p.b->np : -1;
+    h.steps_done = s->steps_done;
+    h.launches = s->launches;
+    h.rel_launch = c->launch_idx - s->launch_begin;    // ping-pong parity of the next launch
+    h.rel_step = c->gstep - s->t_begin;               // the simulation's time index of the next step
+    h.e_count = s->e_count;
+    h.e_launches = s->e_launches;
+    h.n_esteps = (int64_t)s->e_steps.size();
+    char* o = (char*)buf;
+    std::memcpy(o, &h, sizeof(h));
+    o += sizeof(h);
+    if (!s->e_steps.empty()) std::memcpy(o, s->e_steps.data(), sizeof(int64_t) * s->e_steps.size());
+    o += sizeof(int64_t) * s->e_steps.size();
+    HIPCHK(hipMemcpy(o, (const char*)s->block + p.state0, p.state_bytes, hipMemcpyDeviceToHost));
+    o += p.state_bytes;
+    if (p.beam_bytes) HIPCHK(hipMemcpy(o, p.b->block, p.beam_bytes, hipMemcpyDeviceToHost));
+    o += p.beam_bytes;
+    if (p.e_bytes) HIPCHK(hipMemcpy(o, s->eblock, p.e_bytes, hipMemcpyDeviceToHost));
+    return FDTD_OK;
+}
+
+int fdtd_state_load(fdtd_ctx* c, int id, const void* buf, int64_t len) {
+    SimHost* s = ckpt_sim(c, id);
+    if (!s || !buf) return fdtd_fail(FDTD_E_ARG, "bad sim id or null buffer");
+    CkptHeader h;
+    if (len < (int64_t)sizeof(h)) return fdtd_fail(FDTD_E_ARG, "checkpoint: buffer shorter than its header");
+    std::memcpy(&h, buf, sizeof(h));
+    if (std::memcmp(h.magic, kCkptMagic, sizeof(h.magic)) != 0) return fdtd_fail(FDTD_E_ARG, "checkpoint: bad magic");
+    const CkptParts p = ckpt_parts(c, id, s);
+    if (h.prec != c->prec || h.N != s->d.N || h.steps != s->d.steps || h.nprobe != s->d.nprobe ||
+        h.state_bytes != (int64_t)p.state_bytes)
+        return fdtd_fail(FDTD_E_ARG, "checkpoint: precision or simulation shape differs from the target");
+    if (h.beam_bytes != (int64_t)p.beam_bytes || h.beam_np != (p.b ? p.b->np : -1))
+        return fdtd_fail(FDTD_E_ARG, "checkpoint: beam differs from the target's (attach the same beam first)");
+    if (h.e_bytes != (int64_t)p.e_bytes)
+        return fdtd_fail(FDTD_E_ARG, "checkpoint: energy recording differs from the target's (fdtd_set_energy)");
+    if (h.n_esteps != h.e_count || h.n_esteps < 0 ||
+        len < (int64_t)(sizeof(h) + sizeof(int64_t) * h.n_esteps + p.state_bytes + p.beam_bytes + p.e_bytes))
+        return fdtd_fail(FDTD_E_ARG, "checkpoint: buffer shorter than the checkpoint it describes");
+    if (h.steps_done < 0 || h.steps_done > h.steps) return fdtd_fail(FDTD_E_ARG, "checkpoint: bad step count");
+    const char* o = (const char*)buf + sizeof(h);
+    std::vector<int64_t> esteps((size_t)h.n_esteps);
+    if (h.n_esteps) std::memcpy(esteps.data(), o, sizeof(int64_t) * h.n_esteps);
+    o += sizeof(int64_t) * h.n_esteps;
+    // ordered after the target's queued launches; the host buffer is
+    // pageable, so the copies finish before returning
+    HIPCHK(hipMemcpyAsync((char*)s->block + p.state0, o, p.state_bytes, hipMemcpyHostToDevice, c->st));
+    o += p.state_bytes;
+    if (p.beam_bytes) HIPCHK(hipMemcpyAsync(p.b->block, o, p.beam_bytes, hipMemcpyHostToDevice, c->st));
+    o += p.beam_bytes;
+    if (p.e_bytes) HIPCHK(hipMemcpyAsync(s->eblock, o, p.e_bytes, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    s->steps_done = h.steps_done;
+    s->launches = h.launches;
+    s->launch_begin = c->launch_idx - h.rel_launch;
+    s->t_begin = c->gstep - h.rel_step;
+    s->e_count = (int)h.e_count;
+    s->e_launches = h.e_launches;
+    s->e_steps = std::move(esteps);
+    c->dirty = true;
+    return FDTD_OK;
+}
+
 int fdtd_set_energy(fdtd_ctx* c, int id, int every, int rows) {
     if (!c || id < 0 || id >= (int)c->sims.size() || !c->sims[id]) return fdtd_fail(FDTD_E_ARG, "bad sim id");
     if (every < 0 || rows < 0 || (every > 0 && rows < 1))

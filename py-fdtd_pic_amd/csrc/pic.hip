@@ -43,9 +43,9 @@ hipError_t launch_push(const SimDev<R>* sims, const BeamDev<R>* beams, const int
     return hipSuccess;
 }
 
-// The scatter reads nothing the field step of the same launch writes
-// (H^{n-1/2} comes from the copy in hold, and both form J' from the CIC
-// sums), so it may follow that step or run beside it on another stream.
+// The scatter (node and move blocks) reads nothing the field step of the
+// same launch writes (both form J' from the CIC sums; the movers come from
+// the particle buffers), so it may follow that step or share its launch.
 template <typename R>
 hipError_t launch_scatter(const SimDev<R>* sims, const BeamDev<R>* beams, const int2* scatter_blocks,
                           int nscatter, long long launch_idx, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {

@@ -74,6 +74,9 @@ SIGNATURES = {
     "fdtd_remove_sim": (C.c_int, [_vp, C.c_int]),
     "fdtd_sim_progress": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int64)]),
     "fdtd_reset_sim": (C.c_int, [_vp, C.c_int]),
+    "fdtd_state_size": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int64)]),
+    "fdtd_state_save": (C.c_int, [_vp, C.c_int, _vp, C.c_int64]),
+    "fdtd_state_load": (C.c_int, [_vp, C.c_int, _vp, C.c_int64]),
     "fdtd_run": (C.c_int, [_vp, C.c_int64]),
     "fdtd_run_all": (C.c_int, [_vp]),
     "fdtd_sync": (C.c_int, [_vp]),

@@ -160,6 +160,21 @@ class Sim:
     def reset(self):
         check(self.ctx.lib.fdtd_reset_sim(self.ctx.h, self.id))
 
+    def save_state(self):
+        """Checkpoint of this simulation (and its beam) after every queued
+        launch, as bytes (fdtd_state_save)."""
+        n = C.c_int64()
+        check(self.ctx.lib.fdtd_state_size(self.ctx.h, self.id, C.byref(n)))
+        buf = C.create_string_buffer(n.value)
+        check(self.ctx.lib.fdtd_state_save(self.ctx.h, self.id, buf, n.value))
+        return buf.raw
+
+    def load_state(self, blob):
+        """Continue from a checkpoint of a simulation built from the same spec
+        (same precision, beam and energy setting; fdtd_state_load)."""
+        buf = C.create_string_buffer(bytes(blob), len(blob))
+        check(self.ctx.lib.fdtd_state_load(self.ctx.h, self.id, buf, len(blob)))
+
     def remove(self):
         check(self.ctx.lib.fdtd_remove_sim(self.ctx.h, self.id))
 
