@@ -45,6 +45,8 @@ def parse():
     ap.add_argument("--persistent", default="off", choices=["on", "off"],
                     help="config 2: persistent field launches (many blocks of S steps per launch; measured "
                          "slower than one-block launches at K=2 S=48, faster at K=1 and K=4 shapes)")
+    ap.add_argument("--halo", type=int, default=0,
+                    help="config 2: workgroup halo exchange every H steps (0: off)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU baseline sample length")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-pic", action="store_true", help="skip the config 3 (PIC) measurement")
@@ -327,8 +329,10 @@ def config2_line(a, precision):
     S = a.S or (S0 if K == K0 else 8 * Ks)
     ctx.set_tiling(K, S, Ks)
     ctx.set_persistent(persistent)
+    ctx.set_halo_exchange(a.halo)
     sim = ctx.add(engine.physical_spec(p))
     persistent = ctx.persistent_active()
+    halo = a.halo if ctx.halo_exchange_active() else 0
     for _ in range(a.warmup):
         sim.reset()
         ctx.run_all()
@@ -363,7 +367,7 @@ def config2_line(a, precision):
         "data": "synthetic (seeded config 2 parameters)",
         "config": {"workload": "config2: N=65536 cells, 50000 steps, Lorentz+Kerr slab [20000,40000)",
                    "sim_steps": p["steps"], "K": K, "K_slab": Ks, "S": S, "precision": precision,
-                   "persistent": persistent},
+                   "persistent": persistent, "halo_exchange": halo},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(precision, K, S, Ks),

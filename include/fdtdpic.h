@@ -115,8 +115,10 @@ typedef struct {
  * Solver_Engine.py:294-316 with one device-resident loop. */
 int fdtd_ctx_create(int precision, fdtd_ctx **out);
 int fdtd_ctx_destroy(fdtd_ctx *ctx);
-/* Temporal tiling: K cells per lane (1,2,4,8), S steps per launch (<= 32K-1).
- * S = 0 picks the library default for K. */
+/* Temporal tiling: K cells per lane (1,2,4,8), S steps per launch (<= 32K-1;
+ * up to 64K-h-1 with the workgroup halo exchange of period h, below; the run
+ * fails with FDTD_E_ARG when neither holds).  S = 0 picks the library
+ * default for K. */
 int fdtd_set_tiling(fdtd_ctx *ctx, int K, int S);
 /* Mixed widths in one launch: tiles whose window meets a Lorentz/Kerr slab
  * hold K_slab = K/2 cells per lane, the others K (K_slab = K: uniform).  The
@@ -148,6 +150,20 @@ int fdtd_set_pic_fusion(fdtd_ctx *ctx, int on);
 int fdtd_set_persistent(fdtd_ctx *ctx, int on);
 /* *active = 1 when the next fdtd_run uses persistent launches. */
 int fdtd_persistent_active(fdtd_ctx *ctx, int *active);
+/* Workgroup halo exchange (h > 0; 0 turns it off, the default): the four
+ * waves of a field workgroup take adjacent windows of one simulation that
+ * overlap by 2h cells and, every h steps, replace each other's inner halos
+ * with the owners' values through LDS, so only the workgroup's two outer
+ * windows carry S-cell halos (a window otherwise recomputes S cells on each
+ * side, 2S of its 64K).  No counterpart in the reference, whose loop updates
+ * each cell once per step (Solver_Engine.py:297-316).  Used when the tiling
+ * is uniform with K = 2 or 4, h is a multiple of K with h/K <= 8, h <= S and
+ * 64K - h - S >= 1, no beam is attached and persistent launches are off;
+ * otherwise declined (fdtd_halo_exchange_active reports 0).  Results equal
+ * the oracle to the same tolerances as the default tiles; they differ from
+ * other tilings at the rounding level, as tilings do among themselves. */
+int fdtd_set_halo_exchange(fdtd_ctx *ctx, int h);
+int fdtd_halo_exchange_active(fdtd_ctx *ctx, int *active);
 
 /* ---- simulations ------------------------------------------------------ */
 /* Uploads coefficients, allocates zeroed state (FieldInit + CPML_FieldInit,

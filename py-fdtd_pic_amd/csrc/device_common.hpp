@@ -67,6 +67,7 @@ struct PersistArgs {
     const int2* nbr;
     int32_t* err;
     long long spin_ticks;
+    int halo;        // > 0: the workgroup halo-exchange instance, exchange period (cells = steps)
 };
 
 // Particle beam of one simulation.  Particles are kept sorted by cell

@@ -348,7 +348,10 @@ int fdtd_set_tiling_mixed(fdtd_ctx* c, int K, int K_slab, int S) {
     if (K != 1 && K != 2 && K != 4 && K != 8) return fdtd_fail(FDTD_E_ARG, "K must be 1, 2, 4 or 8");
     if (K_slab != K && !(K >= 2 && K_slab == K / 2)) return fdtd_fail(FDTD_E_ARG, "K_slab must be K or K/2");
     if (S == 0) S = default_S(K_slab);
-    if (S < 1 || 64 * K_slab - 2 * S < 1) return fdtd_fail(FDTD_E_ARG, "S must satisfy 1 <= S <= 32*K_slab-1");
+    // S <= 32 K_slab - 1 for the default tiles, checked when the tile list is
+    // built: with the workgroup halo exchange S may reach 64 K - h - 1
+    if (S < 1 || S > 255 || 64 * K_slab - 2 - S < 1)
+        return fdtd_fail(FDTD_E_ARG, "S must satisfy 1 <= S <= min(255, 64*K_slab-3)");
     if ((int64_t)c->sims.size() >= (1 << 24)) return fdtd_fail(FDTD_E_ARG, "too many simulations");
     c->K = K; c->K_slab = K_slab; c->S = S; c->dirty = true;
     return FDTD_OK;
@@ -674,6 +677,16 @@ static int rebuild_t(fdtd_ctx* c) {
     c->S_run = pic ? 1 : c->S;
     const int U = 64 * c->K - 2 * c->S_run;              // plain tiles
     const int Un = 64 * (c->K / 2) - 2 * c->S_run;       // narrow (slab) tiles, when K_slab = K/2
+    // workgroup halo exchange: four adjacent windows per workgroup, 64K - 2h
+    // apart, each workgroup keeping L - 2S of its L = 4 * 64K - 6h cells
+    const int hx = c->halo;
+    const int Lg = 4 * 64 * c->K - 6 * hx;
+    const int Ug = Lg - 2 * c->S_run;
+    c->grouped = hx > 0 && !pic && !c->persist && c->K_slab == c->K && (c->K == 2 || c->K == 4) && hx % c->K == 0 &&
+                 hx / c->K <= 8 && c->S_run >= hx && 64 * c->K - hx - c->S_run >= 1 && Ug >= 1;
+    if (!c->grouped && (U < 1 || Un < 1 && c->K_slab != c->K))
+        return fdtd_fail(FDTD_E_ARG, "S = " + std::to_string(c->S_run) + " needs S <= 32*K_slab-1 without the "
+                                     "workgroup halo exchange");
     c->tile_sims.clear();
     for (int i = 0; i < nsim; ++i) {
         SimHost* s = c->sims[i];
@@ -691,6 +704,12 @@ static int rebuild_t(fdtd_ctx* c) {
         // a tile whose plain-width window [o - S, o + U + S) meets a D-form
         // slab is narrow (K/2 cells per lane) when the tiling asks for it
         const bool slabby = c->K_slab != c->K && s->d.slab_kind >= FDTD_SLAB_LINEAR && s->d.R > s->d.F;
+        if (c->grouped) {
+            // entries hold window starts; a workgroup's four entries stay together
+            for (int ws0 = -c->S_run; ws0 + c->S_run <= s->d.N; ws0 += Ug)
+                for (int w = 0; w < 4; ++w) by_kind[kind].push_back(make_int2(i, ws0 + w * (64 * c->K - 2 * hx)));
+            continue;
+        }
         for (int o = 0; o <= s->d.N;) {
             const bool narrow = slabby && o - c->S_run < s->d.R && o + U + c->S_run > s->d.F;
             by_kind[kind].push_back(make_int2(i | (narrow ? 1 << 24 : 0), o));
@@ -736,14 +755,17 @@ static int rebuild_t(fdtd_ctx* c) {
     // launches against 19.8 us in grid order).
     for (int k = 0; k < 5; ++k) {
         std::vector<int2> heavy, light;
-        for (const int2& t : by_kind[k]) {
+        const size_t step = c->grouped ? 4 : 1;       // whole workgroups in grouped lists
+        for (size_t t0 = 0; t0 < by_kind[k].size(); t0 += step) {
+            const int2 t = by_kind[k][t0];
             const SimHost* sh = c->sims[t.x & 0xffffff];
             const int Ut = (t.x >> 24) ? Un : U;
-            const int lo = t.y - c->S_run, hi = t.y + Ut + c->S_run;
+            const int lo = c->grouped ? t.y : t.y - c->S_run;
+            const int hi = c->grouped ? t.y + Lg : t.y + Ut + c->S_run;
             bool h = (sh->d.slab_kind >= FDTD_SLAB_LINEAR && lo < sh->d.R && hi > sh->d.F) || lo < sh->d.Ls ||
                      hi > sh->d.Rs || (sh->d.src_cell >= lo - 1 && sh->d.src_cell < hi);
             for (int p = 0; p < sh->d.nprobe && !h; ++p) h = sh->d.probe_cell[p] >= lo && sh->d.probe_cell[p] < hi;
-            (h ? heavy : light).push_back(t);
+            for (size_t q = t0; q < t0 + step; ++q) (h ? heavy : light).push_back(by_kind[k][q]);
         }
         by_kind[k].swap(heavy);
         by_kind[k].insert(by_kind[k].end(), light.begin(), light.end());
@@ -844,8 +866,8 @@ static EvPair* next_event(fdtd_ctx* c, int kind) {
 // runs in one launch with the particle scatter (stamped by fev instead).
 static int launch_once(fdtd_ctx* c, int n, EvPair* ev, hipStream_t st, bool fused = false,
                        EvPair* fev = nullptr, int nblocks = 1) {
-    PersistArgs pa{1, nullptr, nullptr, nullptr, 0};
-    if (nblocks > 1) pa = PersistArgs{nblocks, c->d_flags, c->d_nbr, c->d_perr, kPersistSpinTicks};
+    PersistArgs pa{1, nullptr, nullptr, nullptr, 0, c->grouped ? c->halo : 0};
+    if (nblocks > 1) pa = PersistArgs{nblocks, c->d_flags, c->d_nbr, c->d_perr, kPersistSpinTicks, 0};
     const size_t ng = c->groups.size();
     const size_t nplain = fused ? ng - 1 : ng;
     for (size_t gi = 0; gi < ng; ++gi) {
@@ -1303,6 +1325,26 @@ int fdtd_persistent_active(fdtd_ctx* c, int* active) {
         if (rc) return rc;
     }
     *active = c->persist_ok ? 1 : 0;
+    return FDTD_OK;
+}
+
+int fdtd_set_halo_exchange(fdtd_ctx* c, int h) {
+    if (!c) return fdtd_fail(FDTD_E_ARG, "null ctx");
+    if (h < 0 || h > 32) return fdtd_fail(FDTD_E_ARG, "halo exchange period must be 0 (off) .. 32");
+    c->halo = h;
+    c->dirty = true;
+    return FDTD_OK;
+}
+
+int fdtd_halo_exchange_active(fdtd_ctx* c, int* active) {
+    if (!c || !active) return fdtd_fail(FDTD_E_ARG, "null argument");
+    if (c->dirty) {
+        int rc = close_bracket(c);
+        if (rc) return rc;
+        rc = c->prec == FDTD_F64 ? rebuild_t<double>(c) : rebuild_t<float>(c);
+        if (rc) return rc;
+    }
+    *active = c->grouped ? 1 : 0;
     return FDTD_OK;
 }
 

@@ -70,6 +70,8 @@ struct fdtd_ctx {
     bool stage_pending = false;
     int K = 2, S = 16;
     int K_slab = 2;             // cells per lane of tiles meeting a D-form slab (K or K/2)
+    int halo = 0;               // requested workgroup halo exchange period (0: off)
+    bool grouped = false;       // the last rebuild's tile list is in workgroup windows
     int S_run = 16;             // steps per launch of the current tile list (1 with a beam)
     std::vector<SimHost*> sims;
     void* d_sims = nullptr;

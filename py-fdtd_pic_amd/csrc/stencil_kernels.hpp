@@ -208,6 +208,11 @@ struct TileCtx {
     double Jpre[2];
     double* probe_out;      // FP64 samples (FP32 fields: hi - c of the Kahan pair)
     R* snap;
+    // workgroup halo exchange (M_GROUP): this wave's place in its workgroup,
+    // the exchange period in steps (= halo cells on an inner side) and the
+    // emulated vacuum cells (E-form Kerr slab-boundary tiles)
+    int wv, hx;
+    unsigned emu_mask;
 };
 
 // What a tile's step loop has to handle (compile-time, one loop per class).
@@ -218,7 +223,68 @@ enum : int {
     M_SRC = 8,         // soft source / TFSF cell in the tile
     M_PROBE = 16,      // up to two probes in the tile's interior
     M_GENERAL = 32,    // current J, snapshots, more than two probes
+    M_GROUP = 64,      // the workgroup's four windows exchange inner halos every hx steps
 };
+
+// Inner halos of a workgroup's four adjacent windows (M_GROUP).  Window w
+// starts 64K - 2hx cells after window w - 1, so its first hx cells are the
+// owned cells hx .. 2hx - 1 from the end of window w - 1, and its last hx
+// cells are the owned cells hx .. 2hx - 1 of window w + 1.  Between
+// exchanges each window recomputes its halos itself; the invalid front that
+// enters at a window edge moves one cell per step, so after hx steps it has
+// just reached the owned cells and the halos are replaced by the owners'
+// values.  Every state array of a cell goes across (E, H and their Kahan
+// compensations, D, P, Pm, cD, cP, psi_E, psi_H); an emulated vacuum cell
+// (D-form Kerr tiles) sends the compensation of its E, E - (D - cD), as the
+// write-back stores it, and the receiver emulates its halo cells again.
+// Double buffered: a buffer is written again only after the next barrier,
+// which every wave reaches after reading it.  All four waves run the same
+// steps (one simulation per workgroup), so they pass the same barriers.
+constexpr int kTileWaves = 4;        // tiles (waves) per workgroup
+constexpr int kHaloMaxLanes = 8;     // hx / K lanes per inner side
+template <typename R, int K, bool KAHAN>
+__device__ __forceinline__ void halo_exchange(TILE_PARAMS, const TileCtx<R>& c, int buf) {
+    constexpr int NV = 11;
+    __shared__ R xb[2][4][2][NV][kHaloMaxLanes][K];
+    constexpr int st = kHaloMaxLanes * K;    // between the arrays of one (buffer, wave, side)
+    const int hl = c.hx / K;
+    const int lane = c.lane;
+    const int w = c.wv;
+    // side 0: cells hx .. 2hx-1 (the left neighbour's right halo); side 1:
+    // cells 64K-2hx .. 64K-hx-1 (the right neighbour's left halo)
+    int side = -1, q = 0;
+    if (w > 0 && lane >= hl && lane < 2 * hl) { side = 0; q = lane - hl; }
+    if (w < 3 && lane >= 64 - 2 * hl && lane < 64 - hl) { side = 1; q = lane - (64 - 2 * hl); }
+    if (side >= 0) {
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            R ce = cE[j];
+            if (KAHAN && ((c.emu_mask >> j) & 1u)) ce = (R)((double)E[j] - ((double)D[j] - (double)cD[j]));
+            R* o = &xb[buf][w][side][0][q][j];
+            o[0 * st] = E[j]; o[1 * st] = H[j]; o[2 * st] = ce; o[3 * st] = cH[j];
+            o[4 * st] = D[j]; o[5 * st] = P[j]; o[6 * st] = Pm[j]; o[7 * st] = cD[j];
+            o[8 * st] = cP[j]; o[9 * st] = psE[j]; o[10 * st] = psH[j];
+        }
+    }
+    __syncthreads();
+    int from = -1, fside = 0, fq = 0;
+    if (w > 0 && lane < hl) { from = w - 1; fside = 1; fq = lane; }
+    if (w < 3 && lane >= 64 - hl) { from = w + 1; fside = 0; fq = lane - (64 - hl); }
+    if (from >= 0) {
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const R* o = &xb[buf][from][fside][0][fq][j];
+            E[j] = o[0 * st]; H[j] = o[1 * st]; cE[j] = o[2 * st]; cH[j] = o[3 * st];
+            D[j] = o[4 * st]; P[j] = o[5 * st]; Pm[j] = o[6 * st]; cD[j] = o[7 * st];
+            cP[j] = o[8 * st]; psE[j] = o[9 * st]; psH[j] = o[10 * st];
+            if ((c.emu_mask >> j) & 1u) {
+                D[j] = E[j];
+                cD[j] = cE[j];
+                P[j] = Pm[j] = cP[j] = R(0);
+            }
+        }
+    }
+}
 
 // Slab E half for all K cells of a pure-slab lane (straight-line: the K FP64
 // chains interleave).
@@ -291,6 +357,8 @@ __device__ __forceinline__ void run_steps(TILE_PARAMS, const TileCtx<R>& c, cons
     constexpr bool SRC = (MODE & M_SRC) != 0;
     constexpr bool PROBE = (MODE & M_PROBE) != 0;
     constexpr bool GEN = (MODE & M_GENERAL) != 0;
+    constexpr bool GROUP = (MODE & M_GROUP) != 0;
+    int xleft = c.hx, xbuf = 0;      // steps to the next halo exchange, its buffer
     // source values of this launch, one step per lane (read back with
     // v_readlane), and probe samples collected with lane selects and stored
     // once after the loop: no memory access on the per-step path
@@ -448,6 +516,15 @@ __device__ __forceinline__ void run_steps(TILE_PARAMS, const TileCtx<R>& c, cons
                 }
             }
         }
+        if constexpr (GROUP) {
+            if (--xleft == 0) {
+                xleft = c.hx;
+                if (s + 1 < c.s_hi) {
+                    halo_exchange<R, K, KAHAN>(TILE_ARGS, c, xbuf);
+                    xbuf ^= 1;
+                }
+            }
+        }
     };
     int s = c.s_lo;
     for (; s + 1 < c.s_hi; s += 2) {
@@ -497,13 +574,21 @@ __device__ __forceinline__ bool wait_flag(unsigned long long* flag, unsigned lon
 // PERSIST, pa.nblocks blocks of S steps with the neighbour exchange between
 // them (a separate instance: the one-block kernel keeps its own register
 // allocation).
-template <typename R, int K, bool KAHAN, int KIND, bool PERSIST>
+template <typename R, int K, bool KAHAN, int KIND, bool PERSIST, bool GROUP = false>
 __device__ __forceinline__ void tile_body(const SimDev<R>& sd, int out0_, long long launch_idx, long long gstep0,
                                           int nsteps, int S, int tid, const PersistArgs& pa) {
     constexpr bool DFORM = KIND >= SLAB_LINEAR;
     const int lane = wave_lane();
-    const int U = 64 * K - 2 * S;
-    const int out0 = out0_;
+    // the window [t0, t0 + 64K) and the cells it owns (writes back, probes):
+    // a tile entry holds the first owned cell, the window reaching S cells
+    // to either side; with GROUP it holds the window start, and wave w of the
+    // workgroup owns all but its outer halo of S cells (first and last wave)
+    // and its inner halos of hx cells
+    const int wv = GROUP ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) : 0;
+    const int hx = GROUP ? pa.halo : 0;
+    const int t0 = GROUP ? out0_ : out0_ - S;
+    const int out0 = GROUP ? t0 + (wv == 0 ? S : hx) : out0_;
+    const int U = GROUP ? 64 * K - (wv == 0 ? S : hx) - (wv == kTileWaves - 1 ? S : hx) : 64 * K - 2 * S;
     const int N = sd.N;
     const int par = (int)((launch_idx - sd.launch_begin) & 1);
     const int F = sd.F, Rr = sd.Rr, Ls = sd.Ls, Rs = sd.Rs;
@@ -513,7 +598,10 @@ __device__ __forceinline__ void tile_body(const SimDev<R>& sd, int out0_, long l
     // descriptor, so the compiler would otherwise reload it every step.
     TileCtx<R> c;
     c.lane = lane;
-    c.t0 = out0 - S;
+    c.t0 = t0;
+    c.wv = wv;
+    c.hx = hx;
+    c.emu_mask = 0;
     c.base = c.t0 + lane * K;
     c.N = N;
     c.T = sd.T;
@@ -714,16 +802,18 @@ __device__ __forceinline__ void tile_body(const SimDev<R>& sd, int out0_, long l
         c.s_lo = (int)max(0LL, c.t_begin - gs);
         c.s_hi = (int)min((long long)nsteps, (long long)c.T + c.t_begin - gs);
         c.t_first = (int)(gs + c.s_lo - c.t_begin);
+        constexpr int GM = GROUP ? M_GROUP : 0;
+        c.emu_mask = emu_mask;
         if (gen)
-            run_steps<R, K, KAHAN, KIND, M_MIXED | M_STRIP | M_SRC | M_GENERAL>(TILE_ARGS, c, sd);
+            run_steps<R, K, KAHAN, KIND, M_MIXED | M_STRIP | M_SRC | M_GENERAL | GM>(TILE_ARGS, c, sd);
         else if ((pure_slab || emu_tile) && !extras)
-            run_steps<R, K, KAHAN, KIND, M_PURE_SLAB>(TILE_ARGS, c, sd);
+            run_steps<R, K, KAHAN, KIND, M_PURE_SLAB | GM>(TILE_ARGS, c, sd);
         else if (!any_slab && !extras)
-            run_steps<R, K, KAHAN, KIND, 0>(TILE_ARGS, c, sd);
+            run_steps<R, K, KAHAN, KIND, GM>(TILE_ARGS, c, sd);
         else if (!any_slab)
-            run_steps<R, K, KAHAN, KIND, M_STRIP | M_SRC | M_PROBE>(TILE_ARGS, c, sd);
+            run_steps<R, K, KAHAN, KIND, M_STRIP | M_SRC | M_PROBE | GM>(TILE_ARGS, c, sd);
         else
-            run_steps<R, K, KAHAN, KIND, M_MIXED | M_STRIP | M_SRC | M_PROBE>(TILE_ARGS, c, sd);
+            run_steps<R, K, KAHAN, KIND, M_MIXED | M_STRIP | M_SRC | M_PROBE | GM>(TILE_ARGS, c, sd);
 
         // ---------------- write back the interior ----------------
         // (persistent: device-scope stores, which the neighbours' halo loads see)
@@ -774,13 +864,16 @@ __device__ __forceinline__ void tile_body(const SimDev<R>& sd, int out0_, long l
 // interior cell): narrow tiles hold KW/2 cells per lane, the others KW, so
 // tiles with costly slab cells can be made narrower than plain ones within
 // one launch.
-constexpr int kTileWaves = 4;
-template <typename R, int KW, bool KAHAN, int KIND, bool PERSIST = false>
+template <typename R, int KW, bool KAHAN, int KIND, bool PERSIST = false, bool GROUP = false>
 __device__ __forceinline__ void field_tile(const SimDev<R>* __restrict__ sims, const int2* __restrict__ tiles,
                                            int tid, long long launch_idx, long long gstep0, int nsteps, int S,
                                            const PersistArgs& pa) {
     const int2 tl = tiles[tid];
     const SimDev<R>& sd = sims[tl.x & 0xffffff];
+    if constexpr (GROUP) {
+        tile_body<R, KW, KAHAN, KIND, false, true>(sd, tl.y, launch_idx, gstep0, nsteps, S, tid, pa);
+        return;
+    }
     if constexpr (KW > 1 && !PERSIST) {
         if (tl.x >> 24) {
             tile_body<R, KW / 2, KAHAN, KIND, false>(sd, tl.y, launch_idx, gstep0, nsteps, S, tid, pa);
@@ -790,7 +883,10 @@ __device__ __forceinline__ void field_tile(const SimDev<R>* __restrict__ sims, c
     tile_body<R, KW, KAHAN, KIND, PERSIST>(sd, tl.y, launch_idx, gstep0, nsteps, S, tid, pa);
 }
 
-template <typename R, int KW, bool KAHAN, int KIND, bool PERSIST>
+// GROUP: the halo-exchange instance; its tile list holds whole workgroups
+// (ntiles a multiple of kTileWaves, every wave of a workgroup from one
+// simulation), so every wave reaches the exchange barriers.
+template <typename R, int KW, bool KAHAN, int KIND, bool PERSIST, bool GROUP = false>
 __global__ __launch_bounds__(64 * kTileWaves) void stencil_tiles(const SimDev<R>* __restrict__ sims,
                                                                  const int2* __restrict__ tiles, int ntiles,
                                                                  long long launch_idx, long long gstep0,
@@ -798,23 +894,24 @@ __global__ __launch_bounds__(64 * kTileWaves) void stencil_tiles(const SimDev<R>
     // wave-uniform tile index, so the descriptors come in through scalar loads
     const int tid = kTileWaves * (int)blockIdx.x + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     if (tid >= ntiles) return;
-    field_tile<R, KW, KAHAN, KIND, PERSIST>(sims, tiles, tid, launch_idx, gstep0, nsteps, S, pa);
+    field_tile<R, KW, KAHAN, KIND, PERSIST, GROUP>(sims, tiles, tid, launch_idx, gstep0, nsteps, S, pa);
 }
 
 // Host-side launcher (called from host.hip).
 // With profiling events the launch goes through hipExtLaunchKernelGGL, which
 // stamps ev0/ev1 from the kernel's own dispatch packet (kernel execution time
 // only, the figure a kernel trace reports).
-template <typename R, int K, bool KAHAN, int KIND, bool PERSIST>
+template <typename R, int K, bool KAHAN, int KIND, bool PERSIST, bool GROUP = false>
 static hipError_t launch_kp(const SimDev<R>* sims, const int2* tiles, int ntiles, long long launch_idx,
                             long long gstep0, int nsteps, int S, const PersistArgs& pa, hipStream_t st,
                             hipEvent_t ev0, hipEvent_t ev1) {
+    if (GROUP && ntiles % kTileWaves) return hipErrorInvalidValue;
     const int grid = (ntiles + kTileWaves - 1) / kTileWaves;
     if (ev0 || ev1)
-        hipExtLaunchKernelGGL((stencil_tiles<R, K, KAHAN, KIND, PERSIST>), dim3(grid), dim3(64 * kTileWaves), 0, st,
-                              ev0, ev1, 0, sims, tiles, ntiles, launch_idx, gstep0, nsteps, S, pa);
+        hipExtLaunchKernelGGL((stencil_tiles<R, K, KAHAN, KIND, PERSIST, GROUP>), dim3(grid), dim3(64 * kTileWaves), 0,
+                              st, ev0, ev1, 0, sims, tiles, ntiles, launch_idx, gstep0, nsteps, S, pa);
     else
-        hipLaunchKernelGGL((stencil_tiles<R, K, KAHAN, KIND, PERSIST>), dim3(grid), dim3(64 * kTileWaves), 0, st,
+        hipLaunchKernelGGL((stencil_tiles<R, K, KAHAN, KIND, PERSIST, GROUP>), dim3(grid), dim3(64 * kTileWaves), 0, st,
                            sims, tiles, ntiles, launch_idx, gstep0, nsteps, S, pa);
     return hipGetLastError();
 }
@@ -826,6 +923,15 @@ template <typename R, int K, bool KAHAN, int KIND>
 static hipError_t launch_k(const SimDev<R>* sims, const int2* tiles, int ntiles, long long launch_idx,
                            long long gstep0, int nsteps, int S, const PersistArgs& pa, hipStream_t st,
                            hipEvent_t ev0, hipEvent_t ev1) {
+    if (pa.halo > 0) {
+        // halo-exchange instances: K = 2 and 4 (hx / K <= kHaloMaxLanes)
+        if constexpr (K == 2 || K == 4) {
+            if (pa.halo % K || pa.halo / K > kHaloMaxLanes || pa.nblocks > 1) return hipErrorInvalidValue;
+            return launch_kp<R, K, KAHAN, KIND, false, true>(sims, tiles, ntiles, launch_idx, gstep0, nsteps, S, pa,
+                                                            st, ev0, ev1);
+        }
+        return hipErrorInvalidValue;
+    }
     if (pa.nblocks > 1) {
         if constexpr (K <= kPersistMaxK)
             return launch_kp<R, K, KAHAN, KIND, true>(sims, tiles, ntiles, launch_idx, gstep0, nsteps, S, pa, st,

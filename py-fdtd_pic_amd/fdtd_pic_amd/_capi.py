@@ -70,6 +70,8 @@ SIGNATURES = {
     "fdtd_set_pic_fusion": (C.c_int, [_vp, C.c_int]),
     "fdtd_set_persistent": (C.c_int, [_vp, C.c_int]),
     "fdtd_persistent_active": (C.c_int, [_vp, C.POINTER(C.c_int)]),
+    "fdtd_set_halo_exchange": (C.c_int, [_vp, C.c_int]),
+    "fdtd_halo_exchange_active": (C.c_int, [_vp, C.POINTER(C.c_int)]),
     "fdtd_add_sim": (C.c_int, [_vp, C.POINTER(SimDesc), C.POINTER(C.c_int)]),
     "fdtd_remove_sim": (C.c_int, [_vp, C.c_int]),
     "fdtd_sim_progress": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int64)]),

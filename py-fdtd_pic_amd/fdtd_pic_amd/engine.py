@@ -212,6 +212,17 @@ class Context:
         check(self.lib.fdtd_persistent_active(self.h, C.byref(v)))
         return bool(v.value)
 
+    def set_halo_exchange(self, h=8):
+        """Workgroup halo exchange every h steps between the four adjacent
+        windows of a field workgroup (0: off); declined where it does not apply."""
+        check(self.lib.fdtd_set_halo_exchange(self.h, int(h)))
+
+    def halo_exchange_active(self):
+        """True when the next run() uses the workgroup halo exchange."""
+        v = C.c_int()
+        check(self.lib.fdtd_halo_exchange_active(self.h, C.byref(v)))
+        return bool(v.value)
+
     def add(self, spec, beam=None):
         """Add a simulation; beam (optional) is a Beam whose current couples to it."""
         sid = C.c_int()
