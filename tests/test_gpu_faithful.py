@@ -128,3 +128,30 @@ def test_looped_sim_batched_input_frequency_sweep():
         o = objs_o[k][:3]
         p = objs[k][:3]
         _cmp(o, p, FIELDS + [(0, 0, "Dx"), (0, 0, "polarisationCurr"), (0, 0, "Ex_History")])
+
+
+def test_main_driver_matches_oracle():
+    """MasterController.__Main__ (MasterController.py:620-681): its set-up
+    (envSetup at 9 GHz with the Kerr slab (nonLinMed), a sine source with
+    TFSF, 1,000 periods, attenAmount 10, vidInterval 50) and LoopedSim's
+    single run (loop=False), at a 0.15 m domain and 400-600 steps (5,776
+    cells, 1,647 steps; the reference's own values, 0.7 m and 7,000-8,000
+    steps, give 11,555 cells and 19,250 steps).  The oracle builds the same
+    objects with its restatement and runs its Controller; every array the
+    run leaves behind matches to 1e-10."""
+    import MasterController as MC
+    from oracle import faithful as FO
+    V, P, C_P, C_V, Rep = MC.__Main__(freq_in=FREQ, domainSize=0.15, lowLimTim=400, highLimTim=600)
+    Po = FO.make_params(FO.env_setup(FREQ, 0.15, 400, 600, nonLinMed=True), False, 0.15, FREQ, 20,
+                        nonLinMed=True, SineCont=True, TFSF=True)
+    Po.vidInterval = 50
+    Vo = FO.make_variables(Po.Nz, Po.timeSteps, Po.vidInterval, 10)
+    CPo, Co = FO.make_cpml_params(Po.dz), FO.make_cpml_variables(Po.Nz, Po.timeSteps)
+    Po.atten, Po.epsRe, Po.CPMLXp, Po.CPMLXm, Po.TFSF = False, 1, True, True, True
+    Po.Gaussian, Po.SineCont, Po.Periods = False, True, 1000
+    Po.LorentzMed, Po.nonLinMed, Po.FreeSpace = False, True, False
+    print(f"__Main__: N={P.Nz} T={P.timeSteps}", flush=True)
+    assert (P.Nz, P.timeSteps) == (Po.Nz, Po.timeSteps) == (5776, 1647)
+    FO.controller(Vo, Po, Co, CPo)
+    _cmp((Vo, Po, Co), (V, P, C_V), FIELDS + [(0, 0, "Dx"), (0, 0, "Ex_History")])
+    assert np.abs(V.Ex).max() > 0.5 and np.abs(V.Dx).max() > 0
