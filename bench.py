@@ -64,6 +64,11 @@ def parse():
     ap.add_argument("--c4-order", default="arrival", choices=["costly", "arrival"],
                     help="batched admission order: costly jobs first, or as listed")
     ap.add_argument("--c5-requests", type=int, default=256, help="requests of the config 5 trace replayed")
+    ap.add_argument("--c5-order", default="shortest", choices=["arrival", "shortest"],
+                    help="config 5 admission: least cost first (default), or first come first served")
+    ap.add_argument("--c5-slots", type=int, default=128,
+                    help="config 5: requests resident at most (with least-cost-first admission, 128 cut the "
+                         "256-request p50 from 70.3 s to 49 s at the same p99 and throughput as 512 in arrival order)")
     ap.add_argument("--no-f64", action="store_true", help="skip the FP64 config 2 object")
     ap.add_argument("--no-c4", action="store_true", help="skip the config 4 (batched sweep) object")
     ap.add_argument("--no-c5", action="store_true", help="skip the config 5 (served trace) object")
@@ -297,7 +302,7 @@ def config5_bench(a):
     from fdtd_pic_amd import configs, serve
     arr, ps = configs.config5_requests()
     n = a.c5_requests
-    srv = serve.Server("f32", budget_bytes=200 << 30)
+    srv = serve.Server("f32", budget_bytes=200 << 30, max_slots=a.c5_slots, order=a.c5_order)
     last = [time.perf_counter()]
 
     def log(msg):                        # completions, at most one line per 20 s
@@ -308,7 +313,8 @@ def config5_bench(a):
     srv.close()
     out = serve.summarize(jobs, wall)
     out.update(metric="config5 served stream (trace prefix replay)",
-               trace=f"first {n} of 2000 requests, Poisson 200/s, seed 5")
+               trace=f"first {n} of 2000 requests, Poisson 200/s, seed 5", admission=a.c5_order,
+               slots=a.c5_slots)
     return out
 
 

@@ -27,11 +27,24 @@ def work(p):
     return cells, pushes
 
 
+def cost(p):
+    """Relative device cost of a request: cell-updates + 4 x particle-pushes."""
+    cells, pushes = work(p)
+    return cells + 4 * pushes
+
+
 class Server:
+    """order: "arrival" admits waiting requests first come, first served;
+    "shortest" admits the waiting request of least cost first (ties in
+    arrival order)."""
+
     def __init__(self, precision="f32", budget_bytes=64 << 30, max_slots=512, freqs=None, prep_workers=8,
-                 prep_ahead=64):
+                 prep_ahead=64, order="arrival"):
+        if order not in ("arrival", "shortest"):
+            raise ValueError("order must be 'arrival' or 'shortest'")
         self.sch = sweep.Scheduler(precision, budget_bytes=budget_bytes, max_slots=max_slots, freqs=freqs)
         self.prep_workers, self.prep_ahead = int(prep_workers), int(prep_ahead)
+        self.order = order
 
     def replay(self, arrivals, params, time_scale=1.0, log=None):
         """Replay a trace in wall-clock time (arrival t is released at t * time_scale).
@@ -54,9 +67,13 @@ class Server:
 
         def release():
             now = clock()
+            grew = False
             while state["next"] < len(jobs) and jobs[state["next"]].arrival <= now:
                 waiting.append(jobs[state["next"]])
                 state["next"] += 1
+                grew = True
+            if grew and self.order == "shortest":
+                waiting.sort(key=lambda j: cost(j.p))          # stable: ties stay in arrival order
             if pool is not None:
                 for job in waiting[:self.prep_ahead]:
                     if job.prepared is None:
