@@ -1,8 +1,9 @@
 """Pins for oracle/physical.py (and its C restatement oracle/fdtd_cpu.c).
 
-Nothing in the reference pins these numerically (its two unit tests are
-broken), so each part is checked against what the
-mathematics fixes:
+Nothing in the reference pins these numerically (its two unit tests do not
+run: test_BaseFDTD.py:14-21 unpacks 6 of FieldInit's 11 returns with an
+undefined Mc, test_sandbox.py:10-11 raises NameError), so each part is
+checked against what the mathematics fixes:
   Yee vacuum update   - exact translation at Sc = 1 (magic time step);
                         numerical dispersion sin(w dt/2) = Sc sin(k dz/2);
                         discrete energy conservation in a closed cavity.
