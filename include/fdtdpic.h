@@ -134,6 +134,16 @@ int fdtd_set_tiling_mixed(fdtd_ctx *ctx, int K, int K_slab, int S);
  * counterpart in the reference (which has no particles); it selects how
  * this library schedules its own kernels. */
 int fdtd_set_pic_fusion(fdtd_ctx *ctx, int on);
+/* LDS rows per cell of the one-cell particle kernel (dense beams): a cell's
+ * particles beyond the 128 held in registers are pushed once and their new
+ * state waits in rows of 64 until the block's counts are known; a fuller
+ * cell takes a counting pass and a second, bitwise identical, pushing pass.
+ * rows = 0 (the default) gives a launch whose workgroups fit one per CU the
+ * rows its fullest beam needed at attach (plus a quarter) and a larger
+ * launch one row; 1..8 forces a count (capped by the device's shared memory,
+ * 4 in FP64).  Results are bitwise the same for every count.  No counterpart
+ * in the reference (which has no particles). */
+int fdtd_set_pic_lds_rows(fdtd_ctx *ctx, int rows);
 
 /* Persistent field launches (on = 1): when the context's tiles all fit on
  * the device at once (one slab kind, uniform tiles, S <= 64K - 2S, no beam),

@@ -83,6 +83,14 @@ constexpr int kPicGMax = 64;        // cells per push/count group (LDS tables; t
 // whole workgroups of this many waves
 constexpr int kPushWaves = 4;
 constexpr int kXMove = 8;           // move_block threads per push/count block boundary
+constexpr int kBatch = 2;           // chunks of 64 particles fetched per memory round trip
+// pic_push_cell's LDS rows (dynamic shared memory): one pushed 64-particle
+// chunk each (id, move and rank, frac, ux, uy, uz), xr rows for each of a
+// wave's two cells; at most 8 in FP32, 4 in FP64 (about 96 KiB per workgroup)
+template <typename R>
+constexpr size_t kOvfRowBytes = 64 * (2 * sizeof(int32_t) + 4 * sizeof(R));
+template <typename R>
+constexpr int kPushCellMaxRows = sizeof(R) == 4 ? 8 : 4;
 template <typename R>
 struct BeamDev {
     int64_t launch_begin;  // the simulation's, copied: the parity without a load of the sim descriptor

@@ -32,8 +32,8 @@ template <typename R>
 hipError_t launch_energy(const EnergyJob<R>* jobs, int n, hipStream_t st);
 template <typename R>
 hipError_t launch_push(const SimDev<R>* sims, const BeamDev<R>* beams, const int2* cell_items, int ncell,
-                       const int2* group_items, int ngroup, long long launch_idx, hipStream_t st, hipEvent_t e0,
-                       hipEvent_t e1);
+                       const int2* group_items, int ngroup, long long launch_idx, int xr, hipStream_t st,
+                       hipEvent_t e0, hipEvent_t e1);
 template <typename R>
 hipError_t launch_scatter(const SimDev<R>* sims, const BeamDev<R>* beams, const int2* scatter_blocks,
                           int nscatter, long long launch_idx, hipStream_t st, hipEvent_t e0, hipEvent_t e1);
@@ -312,6 +312,8 @@ int fdtd_ctx_create(int precision, fdtd_ctx** out) {
     }
     if (hipDeviceGetAttribute(&c->cu_count, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         c->cu_count = 0;
+    if (hipDeviceGetAttribute(&c->lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+        c->lds_max = 0;
     *out = c;
     return FDTD_OK;
 }
@@ -601,8 +603,16 @@ static int attach_t(fdtd_ctx* c, int id, SimHost& s, const fdtd_beam_desc_t* bd)
         for (int j = 0; j < 3; ++j) b->u0[3 * k + j] = u[3 * i + j];
         occ[cl[i]]++;
     }
-    int64_t acc = 0, occupied = 0;
-    for (int cc = 0; cc <= N + 1; ++cc) { b->start0[cc] = (int32_t)acc; acc += occ[cc]; occupied += occ[cc] > 0; }
+    int64_t acc = 0, occupied = 0, fullest = 0;
+    for (int cc = 0; cc <= N + 1; ++cc) {
+        b->start0[cc] = (int32_t)acc;
+        acc += occ[cc];
+        occupied += occ[cc] > 0;
+        fullest = std::max(fullest, occ[cc]);
+    }
+    // LDS rows per cell that hold the fullest cell's chunks beyond the
+    // register batch, with a quarter to spare as the beam evolves
+    b->xr_want = (int)std::max<int64_t>(1, (fullest + fullest / 4 + 63) / 64 - kBatch);
     // live range (cells of the particles; all live at attach)
     b->range0[0] = b->cell0.front();
     b->range0[1] = b->cell0.back();
@@ -741,6 +751,22 @@ static int rebuild_t(fdtd_ctx* c) {
     c->npblocks = (int)pblocks.size();
     c->npitems = (int)pitems.size();
     c->npcell = (int)citems.size();
+    // LDS rows of the one-cell kernel: what the launch's fullest beam wants
+    // when its workgroups fit one per CU (a lone dense beam), else one (a
+    // large launch keeps its occupancy); capped by the device's shared memory
+    {
+        int want = 1;
+        for (int i : c->tile_sims) {
+            const BeamHost* bh = i < (int)c->beams.size() ? c->beams[i] : nullptr;
+            if (bh && bh->G == 1) want = std::max(want, bh->xr_want);
+        }
+        if (c->cu_count <= 0 || c->npcell > c->cu_count) want = 1;
+        if (c->pc_rows > 0) want = c->pc_rows;
+        const long long per_row = (long long)kPushWaves * 2 * (long long)kOvfRowBytes<R>;
+        const long long room = c->lds_max > 0 ? (long long)c->lds_max - 20 * 1024 : per_row;
+        const int cap = (int)std::max<long long>(1, std::min<long long>(kPushCellMaxRows<R>, room / per_row));
+        c->pc_xr = std::min(want, cap);
+    }
     pitems.insert(pitems.end(), citems.begin(), citems.end());             // the one-cell items follow
     std::vector<Upload> ups;
     if (c->has_pic) {
@@ -1023,10 +1049,10 @@ int fdtd_run(fdtd_ctx* c, int64_t nsteps) {
             hipError_t e = c->prec == FDTD_F64
                 ? launch_push<double>((const SimDev<double>*)c->d_sims, (const BeamDev<double>*)c->d_beams,
                                       c->d_pitems + c->npitems, c->npcell, c->d_pitems, c->npitems,
-                                      c->launch_idx, c->st, e0, e1)
+                                      c->launch_idx, c->pc_xr, c->st, e0, e1)
                 : launch_push<float>((const SimDev<float>*)c->d_sims, (const BeamDev<float>*)c->d_beams,
                                      c->d_pitems + c->npitems, c->npcell, c->d_pitems, c->npitems,
-                                     c->launch_idx, c->st, e0, e1);
+                                     c->launch_idx, c->pc_xr, c->st, e0, e1);
             if (e != hipSuccess)
                 return fdtd_fail(FDTD_E_HIP, std::string("push/count launch ") + std::to_string(c->launch_idx) +
                                                  " (" + std::to_string(c->npcell) + " one-cell workgroups, " +
@@ -1325,6 +1351,14 @@ int fdtd_persistent_active(fdtd_ctx* c, int* active) {
         if (rc) return rc;
     }
     *active = c->persist_ok ? 1 : 0;
+    return FDTD_OK;
+}
+
+int fdtd_set_pic_lds_rows(fdtd_ctx* c, int rows) {
+    if (!c) return fdtd_fail(FDTD_E_ARG, "null ctx");
+    if (rows < 0 || rows > 8) return fdtd_fail(FDTD_E_ARG, "LDS rows must be 0 (chosen per launch) .. 8");
+    c->pc_rows = rows;
+    c->dirty = true;
     return FDTD_OK;
 }
 

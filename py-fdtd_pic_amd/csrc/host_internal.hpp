@@ -36,6 +36,7 @@ struct BeamHost {
     int G = 1;                   // cells per push/count group
     int nwaves = 1, nnode = 1;   // push/count waves, scatter node blocks
     int nmove = 1;               // scatter blocks for staged movers and parked particles
+    int xr_want = 1;             // one-cell beams: LDS rows per cell that hold its fullest cell at attach
     int32_t range0[2] = {0, -1}; // live cell range at attach
     double q = 0, m = 0, w = 0;  // charge, mass, weight per unit transverse area
     double a = 0, jscale = 0, Sc = 0;
@@ -111,6 +112,9 @@ struct fdtd_ctx {
     bool persist = false;                    // requested
     bool persist_ok = false;                 // the current tile list qualifies
     int cu_count = 0;
+    int lds_max = 0;            // shared memory a workgroup may use (bytes; 0: unknown)
+    int pc_rows = 0;            // forced LDS rows per cell of the one-cell PIC kernel (0: chosen per launch)
+    int pc_xr = 1;              // the rows of the current tile list's launches
     unsigned long long* d_flags = nullptr;   // per tile: last block finished (global launch index + 1)
     size_t d_flags_cap = 0;
     int2* d_nbr = nullptr;                   // per tile: left, right neighbour tile

@@ -13,19 +13,29 @@ namespace fdtd {
 // workgroup) and the multi-cell beams' kernel (one item per wave, padded to
 // whole workgroups), e0 at the start of the first launch, e1 at the end of
 // the last.
+// xr: LDS rows per cell of the one-cell kernel (1 .. kPushCellMaxRows).
 template <typename R>
 hipError_t launch_push(const SimDev<R>* sims, const BeamDev<R>* beams, const int2* cell_items, int ncell,
-                       const int2* group_items, int ngroup, long long launch_idx, hipStream_t st, hipEvent_t e0,
-                       hipEvent_t e1) {
+                       const int2* group_items, int ngroup, long long launch_idx, int xr, hipStream_t st,
+                       hipEvent_t e0, hipEvent_t e1) {
     const bool ev = e0 || e1;
     if (ncell > 0) {
+        if (xr < 1 || xr > kPushCellMaxRows<R>) return hipErrorInvalidValue;
+        const size_t shm = push_cell_lds_bytes<R>(xr);
+        static size_t allowed = 64 * 1024;   // dynamic shared memory a launch may ask for without the attribute
+        if (shm > allowed) {
+            const hipError_t a = hipFuncSetAttribute((const void*)pic_push_cell<R>,
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+            if (a != hipSuccess) return a;
+            allowed = shm;
+        }
         hipEvent_t a0 = e0, a1 = ngroup > 0 ? nullptr : e1;
         if (ev)
-            hipExtLaunchKernelGGL((pic_push_cell<R>), dim3(ncell), dim3(64 * kPushWaves), 0, st, a0, a1, 0, sims,
-                                  beams, cell_items, launch_idx);
+            hipExtLaunchKernelGGL((pic_push_cell<R>), dim3(ncell), dim3(64 * kPushWaves), shm, st, a0, a1, 0, sims,
+                                  beams, cell_items, launch_idx, xr);
         else
-            hipLaunchKernelGGL((pic_push_cell<R>), dim3(ncell), dim3(64 * kPushWaves), 0, st, sims, beams,
-                               cell_items, launch_idx);
+            hipLaunchKernelGGL((pic_push_cell<R>), dim3(ncell), dim3(64 * kPushWaves), shm, st, sims, beams,
+                               cell_items, launch_idx, xr);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -60,9 +70,9 @@ hipError_t launch_scatter(const SimDev<R>* sims, const BeamDev<R>* beams, const 
 }
 
 template hipError_t launch_push<double>(const SimDev<double>*, const BeamDev<double>*, const int2*, int,
-                                        const int2*, int, long long, hipStream_t, hipEvent_t, hipEvent_t);
+                                        const int2*, int, long long, int, hipStream_t, hipEvent_t, hipEvent_t);
 template hipError_t launch_push<float>(const SimDev<float>*, const BeamDev<float>*, const int2*, int, const int2*,
-                                       int, long long, hipStream_t, hipEvent_t, hipEvent_t);
+                                       int, long long, int, hipStream_t, hipEvent_t, hipEvent_t);
 template hipError_t launch_scatter<double>(const SimDev<double>*, const BeamDev<double>*, const int2*, int,
                                            long long, hipStream_t, hipEvent_t, hipEvent_t);
 template hipError_t launch_scatter<float>(const SimDev<float>*, const BeamDev<float>*, const int2*, int, long long,

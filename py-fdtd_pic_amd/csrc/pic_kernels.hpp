@@ -141,7 +141,6 @@ __device__ __forceinline__ Chunk chunk_info(int c, int i, int hi, int lane) {
 }
 
 
-constexpr int kBatch = 2;           // chunks of 64 particles fetched per memory round trip
 
 // Beam and field arrays are global memory, but their pointers are read from
 // descriptors, so the compiler would address them as generic (flat) memory;
@@ -756,18 +755,28 @@ __device__ __forceinline__ double pair_sums(double (*red)[64], const double (&a)
 template <typename R>
 constexpr int kPushCellOcc = sizeof(R) == 4 ? 5 : 3;
 
-// One-cell groups (G = 1): one item (beam, workgroup index) per workgroup.
+static_assert(sizeof(OvfRow<float>) == kOvfRowBytes<float> && sizeof(OvfRow<double>) == kOvfRowBytes<double>,
+              "OvfRow layout");
+template <typename R>
+constexpr size_t push_cell_lds_bytes(int xr) {
+    return (size_t)kPushWaves * 2 * (size_t)xr * sizeof(OvfRow<R>);
+}
+
+// One-cell groups (G = 1): one item (beam, workgroup index) per workgroup;
+// xr LDS rows per cell in the dynamic shared memory (push_cell_lds_bytes).
 template <typename R>
 __global__ __launch_bounds__(64 * kPushWaves, kPushCellOcc<R>) void pic_push_cell(const SimDev<R>* __restrict__ sims,
                                                                  const BeamDev<R>* __restrict__ beams,
                                                                  const int2* __restrict__ items,
-                                                                 long long launch_idx) {
+                                                                 long long launch_idx, int xr) {
     __shared__ int xcnt[2][kPushWaves][2];      // per pair, down(A) and up(B) (double buffered)
     __shared__ double red_all[kPushWaves][9][64];   // per wave: the CIC sum reduction (pair_sums)
-    __shared__ OvfRow<R> ovf_all[kPushWaves][2];     // per wave: the extra chunk of cells A and B
+    extern __shared__ __align__(16) unsigned char pc_rows[];
     // wave-uniform by construction; readfirstlane lets the compiler see it, so
     // the item and the descriptors come in through scalar loads
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    // this wave's rows: cell A's [0, xr), cell B's [xr, 2 xr)
+    OvfRow<R>* const rows = reinterpret_cast<OvfRow<R>*>(pc_rows) + (size_t)wv * 2 * xr;
     const int2 it = items[blockIdx.x];
     // the descriptors through the constant address space (the kernels never
     // write them), so their fields can be loaded where they are used, after
@@ -834,13 +843,15 @@ __global__ __launch_bounds__(64 * kPushWaves, kPushCellOcc<R>) void pic_push_cel
         const int len = max(pB - pA, pE - pB);
         // cells of up to 64 * kBatch live particles (the common case): one
         // pass, the batch held in registers across the exchange; up to
-        // 64 * (kBatch + 1): the same, each cell's extra chunk pushed after
-        // the batch and its new state parked in the wave's LDS row (a config-3
-        // beam of ~100 particles per cell has a few such cells, and a second
-        // pass there would set the launch's tail); larger ones: a counting
-        // pass, then a pass that pushes again (bitwise the same) and writes
-        const bool one = len <= 64 * (kBatch + 1);
-        const bool extra = one && len > 64 * kBatch;
+        // 64 * (kBatch + xr): the same, each cell's further chunks pushed
+        // after the batch and their new state parked in the wave's LDS rows
+        // (a config-3 beam of ~100 particles per cell has a few such cells,
+        // and a second pass there would set the launch's tail; a lone dense
+        // beam gets more rows); larger ones: a counting pass, then a pass
+        // that pushes again (bitwise the same) and writes.  Either way the
+        // chunks are pushed, ranked and summed in the same order.
+        const bool one = len <= 64 * (kBatch + xr);
+        const int nx = one ? max(0, (len - 64 * kBatch + 63) / 64) : 0;   // chunks in LDS rows, <= xr
         int tA0 = 0, tA2 = 0, tB0 = 0, tB2 = 0;        // the pair's totals: A down, A up, B down, B up
         Pushed<R> qa[kBatch], qb[kBatch];
         {
@@ -856,15 +867,15 @@ __global__ __launch_bounds__(64 * kPushWaves, kPushCellOcc<R>) void pic_push_cel
                 for (int q = 0; q < kBatch; ++q)
                     g1_push<R, true>(pa, cfB, gB, pB + o + 64 * q + lane < pE, qb[q], nB0, nB1, nB2, accB);
             }
-            if (extra) {
-                constexpr int ox = 64 * kBatch;
+            for (int e = 0; e < nx; ++e) {
+                const int ox = 64 * (kBatch + e);
                 Pushed<R> x;
                 g1_load_one<R>(pid, pf, px, py, pz, pA + ox, np, lane, x);
                 g1_push<R, true>(pa, cfA, gA, pA + ox + lane < pB, x, nA0, nA1, nA2, accA);
-                ovf_all[wv][0].put(lane, x);
+                rows[e].put(lane, x);
                 g1_load_one<R>(pid, pf, px, py, pz, pB + ox, np, lane, x);
                 g1_push<R, true>(pa, cfB, gB, pB + ox + lane < pE, x, nB0, nB1, nB2, accB);
-                ovf_all[wv][1].put(lane, x);
+                rows[xr + e].put(lane, x);
             }
             tA0 = nA0; tA2 = nA2; tB0 = nB0; tB2 = nB2;
         }
@@ -901,12 +912,12 @@ __global__ __launch_bounds__(64 * kPushWaves, kPushCellOcc<R>) void pic_push_cel
                 g1_put_cell<R>(nxt, stg, qa, pA, pB, lane, xa0, xa1, xa2, first ? 0 : -1, np, err);
                 g1_put_cell<R>(nxt, stg, qb, pB, pE, lane, xb0, xb1, xb2, last ? 2 : -1, np, err);
             }
-            if (extra) {
-                constexpr int ox = 64 * kBatch;
-                g1_put_one<R>(nxt, stg, ovf_all[wv][0].get(lane), pA + ox, pB, lane, xa0, xa1, xa2,
-                              first ? 0 : -1, np, err);
-                g1_put_one<R>(nxt, stg, ovf_all[wv][1].get(lane), pB + ox, pE, lane, xb0, xb1, xb2,
-                              last ? 2 : -1, np, err);
+            for (int e = 0; e < nx; ++e) {
+                const int ox = 64 * (kBatch + e);
+                g1_put_one<R>(nxt, stg, rows[e].get(lane), pA + ox, pB, lane, xa0, xa1, xa2, first ? 0 : -1, np,
+                              err);
+                g1_put_one<R>(nxt, stg, rows[xr + e].get(lane), pB + ox, pE, lane, xb0, xb1, xb2, last ? 2 : -1,
+                              np, err);
             }
         } else {
             int nA0 = 0, nA1 = 0, nA2 = 0, nB0 = 0, nB1 = 0, nB2 = 0;   // rank carries again

@@ -68,6 +68,7 @@ SIGNATURES = {
     "fdtd_set_tiling": (C.c_int, [_vp, C.c_int, C.c_int]),
     "fdtd_set_tiling_mixed": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     "fdtd_set_pic_fusion": (C.c_int, [_vp, C.c_int]),
+    "fdtd_set_pic_lds_rows": (C.c_int, [_vp, C.c_int]),
     "fdtd_set_persistent": (C.c_int, [_vp, C.c_int]),
     "fdtd_persistent_active": (C.c_int, [_vp, C.POINTER(C.c_int)]),
     "fdtd_set_halo_exchange": (C.c_int, [_vp, C.c_int]),

@@ -201,6 +201,11 @@ class Context:
         launch is small (default on; bitwise the same results either way)."""
         check(self.lib.fdtd_set_pic_fusion(self.h, 1 if on else 0))
 
+    def set_pic_lds_rows(self, rows=0):
+        """LDS rows per cell of the one-cell particle kernel (0: chosen per
+        launch from the beams' densities; bitwise the same results)."""
+        check(self.lib.fdtd_set_pic_lds_rows(self.h, int(rows)))
+
     def set_persistent(self, on=True):
         """Persistent field launches: many blocks of S steps per launch when
         every tile fits on the device at once (bitwise the same results)."""

@@ -116,6 +116,36 @@ def test_one_cell_groups_mixed_density(prec, tol):
         assert v <= tol, (prec, k, v)
 
 
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-10), ("f32", 1e-5)])
+def test_one_cell_lds_rows_bitwise(prec, tol):
+    """LDS rows per cell of the one-cell kernel (fdtd_set_pic_lds_rows) only
+    change where a chunk's new state waits: a beam of about 350 particles per
+    cell (the absorbed case) pushed with one row (cells above 192 take the
+    counting pass and the writing pass), with the rows chosen from its density
+    (one pass where they suffice) and with three rows gives bitwise the same
+    fields, J and particles, and matches the oracle."""
+    from fdtd_pic_amd import configs, engine
+    p = _params("absorbed")
+    x, u, q, mm, w = configs.beam_inputs(p)
+    assert np.bincount(np.floor(x).astype(int)).max() > 320
+    dt = p["Sc"] * p["dz"] / configs.C0
+    outs = []
+    for rows in (1, 0, 3):
+        print("one-cell rows", prec, rows, flush=True)
+        with engine.Context(prec) as ctx:
+            ctx.set_pic_lds_rows(rows)
+            sim = ctx.add(engine.physical_spec(p), beam=engine.Beam(x, u, q, mm, w, dt, p["dz"], 1))
+            ctx.run_all()
+            xo, uo, alive = sim.particles()
+            outs.append(dict(E=sim.read("E"), H=sim.read("H"), J=sim.read("J"), probes=sim.probes(), x=xo, u=uo,
+                             alive=alive))
+    for o in outs[1:]:
+        for k in outs[0]:
+            assert np.array_equal(o[k], outs[0][k]), k
+    ref = _oracle(p)
+    assert rel_l2(outs[0]["E"], ref["E"]) <= tol and rel_l2(outs[0]["x"] - x, ref["x"] - x) <= tol
+
+
 def test_permutation_invariance_bitwise():
     p = _params("driven", steps=200)
     rng = np.random.default_rng(5)
