@@ -32,7 +32,7 @@ template <typename R>
 hipError_t launch_energy(const EnergyJob<R>* jobs, int n, hipStream_t st);
 template <typename R>
 hipError_t launch_push(const SimDev<R>* sims, const BeamDev<R>* beams, const int2* cell_items, int ncell,
-                       const int2* group_items, int ngroup, long long launch_idx, int xr, hipStream_t st,
+                       const int2* group_items, int ngroup, long long launch_idx, int xr, int single, hipStream_t st,
                        hipEvent_t e0, hipEvent_t e1);
 template <typename R>
 hipError_t launch_scatter(const SimDev<R>* sims, const BeamDev<R>* beams, const int2* scatter_blocks,
@@ -751,6 +751,14 @@ static int rebuild_t(fdtd_ctx* c) {
     c->npblocks = (int)pblocks.size();
     c->npitems = (int)pitems.size();
     c->npcell = (int)citems.size();
+    // the one-cell items of a single beam are (beam, 0 .. npcell-1): the kernel forms them itself
+    c->pc_single = -1;
+    if (!citems.empty()) {
+        bool one_beam = true;
+        for (size_t k = 0; k < citems.size() && one_beam; ++k)
+            one_beam = citems[k].x == citems[0].x && citems[k].y == (int)k;
+        if (one_beam) c->pc_single = citems[0].x;
+    }
     // LDS rows of the one-cell kernel: what the launch's fullest beam wants
     // when its workgroups fit one per CU (a lone dense beam), else one (a
     // large launch keeps its occupancy); capped by the device's shared memory
@@ -1049,10 +1057,10 @@ int fdtd_run(fdtd_ctx* c, int64_t nsteps) {
             hipError_t e = c->prec == FDTD_F64
                 ? launch_push<double>((const SimDev<double>*)c->d_sims, (const BeamDev<double>*)c->d_beams,
                                       c->d_pitems + c->npitems, c->npcell, c->d_pitems, c->npitems,
-                                      c->launch_idx, c->pc_xr, c->st, e0, e1)
+                                      c->launch_idx, c->pc_xr, c->pc_single, c->st, e0, e1)
                 : launch_push<float>((const SimDev<float>*)c->d_sims, (const BeamDev<float>*)c->d_beams,
                                      c->d_pitems + c->npitems, c->npcell, c->d_pitems, c->npitems,
-                                     c->launch_idx, c->pc_xr, c->st, e0, e1);
+                                     c->launch_idx, c->pc_xr, c->pc_single, c->st, e0, e1);
             if (e != hipSuccess)
                 return fdtd_fail(FDTD_E_HIP, std::string("push/count launch ") + std::to_string(c->launch_idx) +
                                                  " (" + std::to_string(c->npcell) + " one-cell workgroups, " +

@@ -115,6 +115,7 @@ struct fdtd_ctx {
     int lds_max = 0;            // shared memory a workgroup may use (bytes; 0: unknown)
     int pc_rows = 0;            // forced LDS rows per cell of the one-cell PIC kernel (0: chosen per launch)
     int pc_xr = 1;              // the rows of the current tile list's launches
+    int pc_single = -1;         // the one-cell items' beam when there is one, else -1
     unsigned long long* d_flags = nullptr;   // per tile: last block finished (global launch index + 1)
     size_t d_flags_cap = 0;
     int2* d_nbr = nullptr;                   // per tile: left, right neighbour tile

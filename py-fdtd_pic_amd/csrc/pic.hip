@@ -13,10 +13,12 @@ namespace fdtd {
 // workgroup) and the multi-cell beams' kernel (one item per wave, padded to
 // whole workgroups), e0 at the start of the first launch, e1 at the end of
 // the last.
-// xr: LDS rows per cell of the one-cell kernel (1 .. kPushCellMaxRows).
+// xr: LDS rows per cell of the one-cell kernel (1 .. kPushCellMaxRows);
+// single: the one-cell items' beam when they all belong to one (items
+// (single, 0 .. ncell-1)), else -1.
 template <typename R>
 hipError_t launch_push(const SimDev<R>* sims, const BeamDev<R>* beams, const int2* cell_items, int ncell,
-                       const int2* group_items, int ngroup, long long launch_idx, int xr, hipStream_t st,
+                       const int2* group_items, int ngroup, long long launch_idx, int xr, int single, hipStream_t st,
                        hipEvent_t e0, hipEvent_t e1) {
     const bool ev = e0 || e1;
     if (ncell > 0) {
@@ -32,10 +34,10 @@ hipError_t launch_push(const SimDev<R>* sims, const BeamDev<R>* beams, const int
         hipEvent_t a0 = e0, a1 = ngroup > 0 ? nullptr : e1;
         if (ev)
             hipExtLaunchKernelGGL((pic_push_cell<R>), dim3(ncell), dim3(64 * kPushWaves), shm, st, a0, a1, 0, sims,
-                                  beams, cell_items, launch_idx, xr);
+                                  beams, cell_items, launch_idx, xr, single);
         else
             hipLaunchKernelGGL((pic_push_cell<R>), dim3(ncell), dim3(64 * kPushWaves), shm, st, sims, beams,
-                               cell_items, launch_idx, xr);
+                               cell_items, launch_idx, xr, single);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -70,9 +72,9 @@ hipError_t launch_scatter(const SimDev<R>* sims, const BeamDev<R>* beams, const 
 }
 
 template hipError_t launch_push<double>(const SimDev<double>*, const BeamDev<double>*, const int2*, int,
-                                        const int2*, int, long long, int, hipStream_t, hipEvent_t, hipEvent_t);
+                                        const int2*, int, long long, int, int, hipStream_t, hipEvent_t, hipEvent_t);
 template hipError_t launch_push<float>(const SimDev<float>*, const BeamDev<float>*, const int2*, int, const int2*,
-                                       int, long long, int, hipStream_t, hipEvent_t, hipEvent_t);
+                                       int, long long, int, int, hipStream_t, hipEvent_t, hipEvent_t);
 template hipError_t launch_scatter<double>(const SimDev<double>*, const BeamDev<double>*, const int2*, int,
                                            long long, hipStream_t, hipEvent_t, hipEvent_t);
 template hipError_t launch_scatter<float>(const SimDev<float>*, const BeamDev<float>*, const int2*, int, long long,

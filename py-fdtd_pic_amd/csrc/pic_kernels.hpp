@@ -764,11 +764,13 @@ constexpr size_t push_cell_lds_bytes(int xr) {
 
 // One-cell groups (G = 1): one item (beam, workgroup index) per workgroup;
 // xr LDS rows per cell in the dynamic shared memory (push_cell_lds_bytes).
+// single >= 0: the launch holds that beam alone and workgroup b's item is
+// (single, b), so the item list is not read (one dependent load fewer).
 template <typename R>
 __global__ __launch_bounds__(64 * kPushWaves, kPushCellOcc<R>) void pic_push_cell(const SimDev<R>* __restrict__ sims,
                                                                  const BeamDev<R>* __restrict__ beams,
                                                                  const int2* __restrict__ items,
-                                                                 long long launch_idx, int xr) {
+                                                                 long long launch_idx, int xr, int single) {
     __shared__ int xcnt[2][kPushWaves][2];      // per pair, down(A) and up(B) (double buffered)
     __shared__ double red_all[kPushWaves][9][64];   // per wave: the CIC sum reduction (pair_sums)
     extern __shared__ __align__(16) unsigned char pc_rows[];
@@ -777,7 +779,7 @@ __global__ __launch_bounds__(64 * kPushWaves, kPushCellOcc<R>) void pic_push_cel
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     // this wave's rows: cell A's [0, xr), cell B's [xr, 2 xr)
     OvfRow<R>* const rows = reinterpret_cast<OvfRow<R>*>(pc_rows) + (size_t)wv * 2 * xr;
-    const int2 it = items[blockIdx.x];
+    const int2 it = single >= 0 ? make_int2(single, (int)blockIdx.x) : items[blockIdx.x];
     // the descriptors through the constant address space (the kernels never
     // write them), so their fields can be loaded where they are used, after
     // stores too, instead of all being held from the start
