@@ -2,6 +2,8 @@
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include "pic_kernels.hpp"
 
 namespace fdtd {
@@ -24,12 +26,16 @@ hipError_t launch_push(const SimDev<R>* sims, const BeamDev<R>* beams, const int
     if (ncell > 0) {
         if (xr < 1 || xr > kPushCellMaxRows<R>) return hipErrorInvalidValue;
         const size_t shm = push_cell_lds_bytes<R>(xr);
-        static size_t allowed = 64 * 1024;   // dynamic shared memory a launch may ask for without the attribute
-        if (shm > allowed) {
+        // dynamic shared memory above 64 KiB needs the function attribute; it
+        // is raised to the largest size asked for so far (host threads with
+        // contexts of their own may launch concurrently, hence the atomic)
+        static std::atomic<size_t> allowed{64 * 1024};
+        size_t cur = allowed.load();
+        while (shm > cur) {
             const hipError_t a = hipFuncSetAttribute((const void*)pic_push_cell<R>,
                                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
             if (a != hipSuccess) return a;
-            allowed = shm;
+            if (allowed.compare_exchange_weak(cur, shm)) break;
         }
         hipEvent_t a0 = e0, a1 = ngroup > 0 ? nullptr : e1;
         if (ev)
