@@ -166,6 +166,25 @@ def test_served_trace_alone_vs_mix():
     assert ex <= 1e-4
 
 
+def test_served_least_cost_first():
+    """order="shortest": requests waiting together are admitted in order of
+    cost (cell-updates + 4 x particle-pushes), here two at a time; each still
+    gets the spectra it gets alone, bitwise."""
+    from fdtd_pic_amd import serve, sweep
+    ps = _small_instances(count=5, steps=2500, seed=19, beams=(1, 2), n_particles=2000)
+    srv = serve.Server("f32", max_slots=2, freqs=FREQS, order="shortest")
+    jobs, wall = srv.replay(np.zeros(len(ps)), ps)
+    srv.close()
+    by_admission = [j.id for j in sorted(jobs, key=lambda j: j.admitted)]
+    by_cost = sorted(range(len(ps)), key=lambda i: serve.cost(ps[i]))
+    print("admitted", by_admission, "by cost", by_cost, flush=True)
+    assert by_admission == by_cost
+    for p, j in zip(ps, jobs):
+        a = sweep.run_alone(p, "f32", freqs=FREQS)
+        for k in range(4):
+            assert np.array_equal(a.X[k], j.X[k]), (j.id, k)
+
+
 def test_jobs_of_different_dt_finishing_together():
     """Two jobs with different cell sizes (so different dt) that finish in the
     same slice each get the DFT at their own dt: bitwise their spectra alone."""
