@@ -117,7 +117,7 @@ def summarize(jobs, wall):
     lat = np.array([j.finished - j.arrival for j in jobs])
     cells = sum(work(j.p)[0] for j in jobs)
     pushes = sum(work(j.p)[1] for j in jobs)
-    return dict(requests=len(jobs), wall_s=round(wall, 3),
+    return dict(requests=len(jobs), failed=sum(1 for j in jobs if getattr(j, "failed", False)), wall_s=round(wall, 3),
                 p50_latency_s=round(float(np.percentile(lat, 50)), 4),
                 p99_latency_s=round(float(np.percentile(lat, 99)), 4),
                 requests_per_s=round(len(jobs) / wall, 3),

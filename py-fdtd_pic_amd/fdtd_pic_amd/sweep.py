@@ -86,6 +86,7 @@ class Job:
         self.admitted = self.finished = None
         self.R = self.T = None
         self.X = None            # (X0_r, X1_r, X0_t, X1_t)
+        self.failed = False      # its spectra came out non-finite (a field diverged or was corrupted)
         self.bytes = 0
         self.prepared = None     # Future of prepare(self) while queued
 
@@ -272,6 +273,10 @@ class Scheduler:
                 for k, j in enumerate(mine):
                     X0r, X1r, X0t, X1t = Xr[2 * k], Xr[2 * k + 1], Xt[2 * k], Xt[2 * k + 1]
                     j.X = (X0r, X1r, X0t, X1t)
+                    # a diverged or corrupted field shows as non-finite spectra: the
+                    # job is flagged (the reference's loop would sys.exit), and the
+                    # jobs beside it, each in simulations of its own, are unaffected
+                    j.failed = not all(np.isfinite(x).all() for x in j.X)
                     with np.errstate(divide="ignore", invalid="ignore"):   # no incident power at f: nan
                         j.R = np.abs(X1r - X0r) / np.abs(X0r)
                         j.T = np.abs(X1t) / np.abs(X0t)

@@ -185,6 +185,37 @@ def test_served_least_cost_first():
             assert np.array_equal(a.X[k], j.X[k]), (j.id, k)
 
 
+def test_nonfinite_job_flagged_and_isolated():
+    """A job whose field turns non-finite mid-run (NaN written into one cell
+    of its slab pass's E from the host after the first slice) finishes
+    flagged failed, with non-finite spectra; the jobs sharing its contexts
+    are not flagged and their spectra equal their lone runs bitwise."""
+    from fdtd_pic_amd import sweep
+    ps = _small_instances(count=3, steps=2400, seed=23, beams=(1,), n_particles=2000)
+    sch = sweep.Scheduler("f32", K=2, S=48, freqs=FREQS)
+    jobs = [sweep.Job(i, p) for i, p in enumerate(ps)]
+    try:
+        for j in jobs:
+            sch.admit(j)
+        sch.step()
+        sim = jobs[0].sims[1]
+        E = sim.read("E")
+        E[len(E) // 2] = np.nan
+        sim.write("E", E)
+        while sch.active:
+            sch.step()
+            sch.collect()
+    finally:
+        sch.close()
+    print("failed", [j.failed for j in jobs], flush=True)
+    assert jobs[0].failed and not np.isfinite(jobs[0].X[1]).all()
+    assert not jobs[1].failed and not jobs[2].failed
+    for p, j in zip(ps[1:], jobs[1:]):
+        a = sweep.run_alone(p, "f32", K=2, S=48, freqs=FREQS)
+        for k in range(4):
+            assert np.array_equal(a.X[k], j.X[k]), (j.id, k)
+
+
 def test_jobs_of_different_dt_finishing_together():
     """Two jobs with different cell sizes (so different dt) that finish in the
     same slice each get the DFT at their own dt: bitwise their spectra alone."""
