@@ -361,13 +361,13 @@ __device__ __forceinline__ void g1_push(const PushArgs<R>& pa, const CellFields<
     const int r = d < 0 ? n0 + m0 : (d == 0 ? n1 + m1 : n2 + m2);
     s.dr = (r << 2) | (d + 1);
     if (SUMS) {
-        // the weights onto nodes c-1+off and c+off (exact conversions, so the
-        // same sums as converting each node's share)
-        const double w0 = (double)a0, w1 = (double)a1;
-        acc[0] += off == 0 ? w0 : 0.0;
-        acc[1] += off == 0 ? w1 : (off == 1 ? w0 : 0.0);
-        acc[2] += off == 1 ? w1 : (off == 2 ? w0 : 0.0);
-        acc[3] += off == 2 ? w1 : 0.0;
+        // the weights onto nodes c-1+off and c+off, each node's share chosen
+        // in R and then converted (exact, so the same sums as choosing among
+        // converted weights, with 32-bit selects in FP32)
+        acc[0] += (double)(off == 0 ? a0 : R(0));
+        acc[1] += (double)(off == 0 ? a1 : (off == 1 ? a0 : R(0)));
+        acc[2] += (double)(off == 1 ? a1 : (off == 2 ? a0 : R(0)));
+        acc[3] += (double)(off == 2 ? a1 : R(0));
     }
     n0 += __popcll(bd0);
     n1 += __popcll(bd1);
