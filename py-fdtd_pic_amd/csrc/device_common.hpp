@@ -153,6 +153,25 @@ __device__ __forceinline__ void kahan_add(R& s, R& c, R inc) {
     s = t;
 }
 
+// a * b + c.  FP64: rounded once (an explicit FMA); the FP64 translation
+// units are compiled without FMA contraction (-ffp-contract=off), so every
+// fused product is one written out here and a cell's update is the same
+// sequence of roundings in every tile class and every (K, S) tiling.  FP32:
+// the plain expression, fused as the compiler chooses (the FP32 field units
+// keep contraction on; their results are held to a tolerance, and the
+// compensated sums absorb the difference).
+__device__ __forceinline__ double fmr(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fmr(float a, float b, float c) { return a * b + c; }
+
+// Kahan-compensated s += a * b (FP64: a * b - c rounded once).
+template <typename R>
+__device__ __forceinline__ void kahan_fma(R& s, R& c, R a, R b) {
+    R y = fmr(a, b, -c);
+    R t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
 // ---- Kerr, reference path: Re(roots[0]) of a x^3 + b x^2 + c x + d (FP64) ----
 // Restates the 1728.org closed form vendored at CubicEquationSolver.py:29-105
 // (branch order and tests kept), as consumed by Nonlin_Cubic_Solver

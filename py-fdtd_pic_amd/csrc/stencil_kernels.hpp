@@ -132,12 +132,12 @@ __device__ __forceinline__ KerrZ<R> slab_u(R kD, R jD, R dH, R Jv, R E, R& D, R&
         Pm = Vn;
         if (KAHAN) kahan_add(P, cP, Vn); else P = P + Vn;
     } else {
-        const R Pn = sA * P + sB * Pm + sC * E;
+        const R Pn = fmr(sA, P, fmr(sB, Pm, sC * E));
         Pm = P;
         P = Pn;
     }
-    const R inc = (HASJ ? dH - jD * Jv : dH) * kD;
-    if (KAHAN) kahan_add(D, cD, inc); else D = D + inc;
+    const R g = HASJ ? fmr(-jD, Jv, dH) : dH;
+    if (KAHAN) kahan_fma(D, cD, g, kD); else D = fmr(g, kD, D);
     // FP32: u = (D - cD) - (P - cP) from the compensated values, in FP32.
     // D - P is exact when D and P are within a factor 2 (the cancelling case)
     // and otherwise rounded relative to a u of their size, so the rounded sum
@@ -315,11 +315,11 @@ __device__ __forceinline__ void e_mixed(TILE_PARAMS, const TileCtx<R>& c, const 
 #pragma unroll
     for (int j = 0; j < K; ++j) {
         Ev[j] = E[j]; cEv[j] = cE[j];
-        const R inc = (dH[j] - Jv[j]) * ke[j];
-        if (KAHAN) kahan_add(Ev[j], cEv[j], inc); else Ev[j] = Ev[j] + inc;
+        const R a = dH[j] - Jv[j];
+        if (KAHAN) kahan_fma(Ev[j], cEv[j], a, ke[j]); else Ev[j] = fmr(a, ke[j], Ev[j]);
         if (STRIP) {
-            psE[j] = bE[j] * psE[j] + cEc[j] * dH[j];
-            Ev[j] = Ev[j] - CbE[j] * psE[j];
+            psE[j] = fmr(bE[j], psE[j], cEc[j] * dH[j]);
+            Ev[j] = fmr(-CbE[j], psE[j], Ev[j]);
         }
     }
     KerrZ<R> x[K], z[K];
@@ -424,22 +424,22 @@ __device__ __forceinline__ void run_steps(TILE_PARAMS, const TileCtx<R>& c, cons
         } else {
 #pragma unroll
             for (int j = 0; j < K; ++j) {
-                const R inc = (dH[j] - Jv[j]) * ke[j];
-                if (KAHAN) kahan_add(E[j], cE[j], inc); else E[j] = E[j] + inc;
+                const R a = dH[j] - Jv[j];
+                if (KAHAN) kahan_fma(E[j], cE[j], a, ke[j]); else E[j] = fmr(a, ke[j], E[j]);
                 if (STRIP) {
-                    psE[j] = bE[j] * psE[j] + cEc[j] * dH[j];
-                    E[j] = E[j] - CbE[j] * psE[j];
+                    psE[j] = fmr(bE[j], psE[j], cEc[j] * dH[j]);
+                    E[j] = fmr(-CbE[j], psE[j], E[j]);
                 }
             }
         }
         if constexpr (SRC) {
             const R sv = lane_read(srcEc[0], ix & 63);
 #pragma unroll
-            for (int j = 0; j < K; ++j) E[j] = E[j] + ms[j] * sv;
+            for (int j = 0; j < K; ++j) E[j] = fmr(ms[j], sv, E[j]);
             // ---------------- H half (TFSF correction first) ----------------
             const R hv = lane_read(srcHc[0], ix & 63);
 #pragma unroll
-            for (int j = 0; j < K; ++j) H[j] = H[j] - mt[j] * hv;
+            for (int j = 0; j < K; ++j) H[j] = fmr(-mt[j], hv, H[j]);
         }
         if constexpr (GEN) {
             // energy: the interior's H' before the launch's last H update
@@ -454,11 +454,10 @@ __device__ __forceinline__ void run_steps(TILE_PARAMS, const TileCtx<R>& c, cons
 #pragma unroll
         for (int j = 0; j < K; ++j) {
             const R dE = ((j == K - 1) ? Er : E[j + 1]) - E[j];
-            const R inc = dE * kh[j];
-            if (KAHAN) kahan_add(H[j], cH[j], inc); else H[j] = H[j] + inc;
+            if (KAHAN) kahan_fma(H[j], cH[j], dE, kh[j]); else H[j] = fmr(dE, kh[j], H[j]);
             if (STRIP) {
-                psH[j] = bH[j] * psH[j] + cHc[j] * dE;
-                H[j] = H[j] + C2H[j] * psH[j];
+                psH[j] = fmr(bH[j], psH[j], cHc[j] * dE);
+                H[j] = fmr(C2H[j], psH[j], H[j]);
             }
         }
         // ---------------- probes / snapshots ----------------

@@ -4,7 +4,9 @@
 
 Every HIP translation unit under csrc/ is compiled (in parallel) with
 `--offload-arch=gfx950 -O3` and `-target-feature -packed-fp32-ops` (no packed
-FP32 VALU ops are generated) and linked into fdtd_pic_amd/libfdtdpic.so.
+FP32 VALU ops are generated), all but the FP32 field units with
+`-ffp-contract=off` (only the fused products written out in the source), and
+linked into fdtd_pic_amd/libfdtdpic.so.
 """
 import glob
 import os
@@ -20,6 +22,18 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wno-unused-command-line-argument"]
+# -ffp-contract=off: the compiler fuses no product on its own; the FP64 field
+# update's fused products are written out (fmr), so an FP64 cell's result does
+# not depend on which tile-class loop or (K, S) tiling computed it.  The FP32
+# field units keep contraction (fewer register moves in their step loop).
+NO_CONTRACT = ["-ffp-contract=off"]
+
+
+def flags_for(src):
+    """Compile flags of one translation unit."""
+    if os.path.basename(src).startswith("stencil_f32_"):
+        return FLAGS
+    return FLAGS + NO_CONTRACT
 
 
 def sources():
@@ -42,7 +56,7 @@ def build(verbose=False, jobs=None):
     jobs = jobs or min(len(srcs), max(1, os.cpu_count() or 1))
     with tempfile.TemporaryDirectory(prefix="fdtdpic_build_") as tmpd:
         objs = [os.path.join(tmpd, os.path.basename(src)[:-4] + ".o") for src in srcs]
-        cmds = [[HIPCC] + FLAGS + ["-c", src, "-o", obj] for src, obj in zip(srcs, objs)]
+        cmds = [[HIPCC] + flags_for(src) + ["-c", src, "-o", obj] for src, obj in zip(srcs, objs)]
         if verbose:
             for c in cmds:
                 print(" ".join(c), flush=True)

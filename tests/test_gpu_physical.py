@@ -4,8 +4,9 @@ FP64 device runs must match oracle/physical.py to relative L2 <= 1e-10; FP32
 runs (Kahan-compensated fields, FP64 Kerr root) to <= 1e-5, and the
 reflection/transmission spectra to <= 1e-4 where the incident spectrum is
 above 1% of its peak.  Different temporal tilings (K, S) must give bitwise
-identical FP64 results up to FMA contraction order (the compiler may fuse a
-different product in different K instantiations): relative L2 <= 1e-13.
+identical FP64 results: the FP64 units are compiled without FMA contraction
+and fuse only the products the source writes out, so every tile class and K
+instantiation rounds a cell's update the same way.
 """
 import math
 
@@ -92,14 +93,16 @@ def test_fp32_probes_are_the_compensated_field(K, S, Ks):
 @pytest.mark.parametrize("K,S,Ks", [(1, 3, 1), (1, 31, 1), (2, 16, 2), (4, 8, 4), (4, 100, 4), (8, 64, 8),
                                     (8, 255, 8), (4, 48, 2), (2, 20, 1), (8, 100, 4)])
 def test_tilings_agree(K, S, Ks):
-    """Uniform and mixed (narrow slab tiles) tilings give the same fields."""
+    """Uniform and mixed (narrow slab tiles) tilings give the same FP64
+    fields, bitwise."""
     spec = _spec(chi3=1e-3, src_amp=2.0, steps=900)
     base = _device(spec, "f64", K=2, S=1)
     d = _device(spec, "f64", K=K, S=S, K_slab=Ks)
     errs = {k: rel_l2(d[k], base[k]) for k in ("E", "H", "D", "P", "PSIE", "PSIH", "probes")}
     print(K, S, Ks, {k: f"{v:.1e}" for k, v in errs.items()}, flush=True)
-    for k, v in errs.items():
-        assert v <= 1e-13, (K, S, k, v)
+    for k in errs:
+        assert np.array_equal(d[k], base[k]), (K, S, k, errs[k])
+    assert np.abs(base["P"]).max() > 0 and np.abs(base["probes"]).max() > 0
 
 
 def test_spectra_fp32_vs_fp64():
