@@ -309,6 +309,14 @@ int fdtd_probe_spectra_noalloc(fdtd_ctx *ctx, int nsims, const int *sim_ids, int
  *   6   ADE_PolarisationCurrent_Ex :609-633  P*, P_prev, Ex                  A, B, C
  *   7   AcubicFinder :803-853           Acubic*, Dx                           cub, qua, one, eps0
  *   8   NonLinExUpdate :858-877         Ex*, Dx, Acubic                       eps0, chi3, sqrt(1.2)
+ * and the per-step functions of the older BaseFDTD.py that differ in form:
+ *   op  function (BaseFDTD.py)          arrays                                scalars
+ *   9   CPML_ExUpdate :355-358,         Ex*, Hy, UpExSelf, c1, c2             -
+ *       ADE_ExUpdate :401-408           (Ex = Ex*Self + dH*c1*c2)
+ *   10  CPML_Psi_e_Update :331-336      Ex*, psi_Ex*, Hy, beX, ceX, Cb        -   (Ex += Cb psi)
+ *   11  CPML_Psi_m_Update :339-344      Hy*, psi_Hy*, Ex, bmY, cmY, C2        -   (Ex[n] - Ex[n+1])
+ *   12  ADE_PolarisationCurrent_Ex :378-390  P*, Pprev, Pprev2, Ex            A, B, C
+ *   13  ADE_DxUpdate :417-422           Dx*, Ex*, Hy, P                       dt, dz, eps0
  * (* = written).  Needs an FP64 context. */
 int fdtd_step_op(fdtd_ctx *ctx, int op, int narr, double *const *arrays, const int64_t *lens, int nscal,
                  const double *scalars, const int32_t *ranges);

@@ -1763,14 +1763,17 @@ struct OpSpec {
     int narr, nscal, nout;
     int prev, next;        // reads index n-1 / n+1
 };
-// indexed by op: EX_UPDATE HY_UPDATE PSI_E PSI_M DX_UPDATE EX_CREATE POLARISE ACUBIC NONLIN_EX
-const OpSpec kOps[9] = {{5, 0, 1, 1, 0}, {5, 0, 1, 0, 1}, {6, 0, 2, 1, 0}, {6, 0, 2, 0, 1}, {3, 1, 1, 1, 0},
-                        {3, 1, 1, 0, 0}, {3, 3, 1, 0, 0}, {2, 4, 1, 0, 0}, {3, 3, 1, 0, 0}};
+// indexed by op: EX_UPDATE HY_UPDATE PSI_E PSI_M DX_UPDATE EX_CREATE POLARISE ACUBIC NONLIN_EX,
+// then BaseFDTD's B_EX B_PSI_E B_PSI_M B_POLARISE B_DX
+constexpr int kNumOps = 14;
+const OpSpec kOps[kNumOps] = {{5, 0, 1, 1, 0}, {5, 0, 1, 0, 1}, {6, 0, 2, 1, 0}, {6, 0, 2, 0, 1}, {3, 1, 1, 1, 0},
+                              {3, 1, 1, 0, 0}, {3, 3, 1, 0, 0}, {2, 4, 1, 0, 0}, {3, 3, 1, 0, 0}, {5, 0, 1, 1, 0},
+                              {6, 0, 2, 1, 0}, {6, 0, 2, 0, 1}, {4, 3, 1, 0, 0}, {4, 3, 2, 1, 0}};
 }  // namespace
 
 extern "C" int fdtd_step_op(fdtd_ctx* c, int op, int narr, double* const* arrays, const int64_t* lens, int nscal,
                             const double* scalars, const int32_t* ranges) {
-    if (!c || op < 0 || op > 8 || !arrays || !lens || !ranges) return fdtd_fail(FDTD_E_ARG, "bad argument");
+    if (!c || op < 0 || op >= kNumOps || !arrays || !lens || !ranges) return fdtd_fail(FDTD_E_ARG, "bad argument");
     if (c->prec != FDTD_F64) return fdtd_fail(FDTD_E_ARG, "the single-op path runs in an FP64 context");
     const OpSpec& o = kOps[op];
     if (narr != o.narr || nscal != o.nscal || (nscal > 0 && !scalars))

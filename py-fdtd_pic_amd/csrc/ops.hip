@@ -12,6 +12,15 @@
 //   OP_POLARISE    ADE_PolarisationCurrent_Ex :609-633
 //   OP_ACUBIC      AcubicFinder + Nonlin_Cubic_Solver :803-853 (root 0, 1728.org form)
 //   OP_NONLIN_EX   NonLinExUpdate      :858-877
+// and the per-step functions of BaseFDTD.py (the older module: other index
+// ranges, the psi_E correction added, the psi_H difference reversed):
+//   OP_B_EX        CPML_ExUpdate :355-358, ADE_ExUpdate :401-408
+//   OP_B_PSI_E     CPML_Psi_e_Update   :331-336
+//   OP_B_PSI_M     CPML_Psi_m_Update   :339-344
+//   OP_B_POLARISE  ADE_PolarisationCurrent_Ex :378-390
+//   OP_B_DX        ADE_DxUpdate        :417-422
+// (BaseFDTD's CPML_HyUpdate and ADE_HyUpdate are OP_HY_UPDATE's form, and its
+// ADE_ExCreate is OP_EX_CREATE's.)
 #include <hip/hip_runtime.h>
 
 #include "device_common.hpp"
@@ -63,6 +72,24 @@ __global__ __launch_bounds__(256) void step_op(int op, OpArgs g) {
         }
         case 8:   // Ex[n] = Dx[n] / (eps0 * sqrt(1.2) + eps0 * chi3 * Acubic[n])
             a[0][n] = a[1][n] / (g.s[0] * g.s[2] + g.s[0] * g.s[1] * a[2][n]);
+            break;
+        case 9:   // Ex[n] = Ex[n]*Self[n] + (Hy[n] - Hy[n-1]) * c1[n] * c2[n]
+            a[0][n] = a[0][n] * a[2][n] + (a[1][n] - a[1][n - 1]) * a[3][n] * a[4][n];
+            break;
+        case 10:  // psi_Ex = beX psi_Ex + ceX (Hy[n] - Hy[n-1]); Ex += Cb psi_Ex
+            a[1][n] = a[3][n] * a[1][n] + a[4][n] * (a[2][n] - a[2][n - 1]);
+            a[0][n] = a[0][n] + a[5][n] * a[1][n];
+            break;
+        case 11:  // psi_Hy = bmY psi_Hy + cmY (Ex[n] - Ex[n+1]); Hy += C2 psi_Hy
+            a[1][n] = a[3][n] * a[1][n] + a[4][n] * (a[2][n] - a[2][n + 1]);
+            a[0][n] = a[0][n] + a[5][n] * a[1][n];
+            break;
+        case 12:  // P[n] = A Pprev[n] + B Pprev2[n] + C Ex[n]
+            a[0][n] = g.s[0] * a[1][n] + g.s[1] * a[2][n] + g.s[2] * a[3][n];
+            break;
+        case 13:  // Dx[n] += (Hy[n] - Hy[n-1]) * dt / dz; Ex[n] = (Dx[n] - P[n]) / eps0
+            a[0][n] = a[0][n] + (a[2][n] - a[2][n - 1]) * g.s[0] / g.s[1];
+            a[1][n] = (a[0][n] - a[3][n]) / g.s[2];
             break;
         default:
             break;
