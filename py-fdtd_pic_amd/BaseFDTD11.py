@@ -76,11 +76,11 @@ def Gaussian(V, P):
 
 
 def SineCont(V, P, rep, interv):
-    pass
+    """Empty in the reference (BaseFDTD11.py:98-99)."""
 
 
 def Ricker(V, P):
-    pass
+    """Empty in the reference (BaseFDTD11.py:101-102)."""
 
 
 def SmoothTurnOn(V, P, tempfreq=0):
