@@ -206,3 +206,14 @@ def test_history_shift_interior_only():
     B.ADE_TempPolCurr(V, P)
     assert V.tempTempVarPol[0] == -1 and V.tempTempVarPol[9] == -1 and V.tempTempVarPol[5] == 5
     assert V.tempVarPol[0] == 0 and V.tempVarPol[10] == 10 and V.tempVarPol[4] == 104
+
+
+def test_per_step_ranges_that_wrap_are_refused():
+    """A range that starts below cell 0 (front edge 0) or an ADE_ExUpdate
+    tail from cell 0 (rear edge 1) would wrap to the array's end in the
+    reference; the drop-in refuses them before any device call."""
+    V = types.SimpleNamespace(Ex=np.zeros(11), Dx=np.zeros(11), Hy=np.zeros(11), polarisationCurr=np.zeros(11))
+    with pytest.raises(ValueError):
+        B.ADE_ExCreate(V, _p(Nz=10, materialFrontEdge=0, materialRearEdge=5), None, None)
+    with pytest.raises(ValueError):
+        B.ADE_ExUpdate(V, _p(Nz=10, materialFrontEdge=1, materialRearEdge=1), None, None)
