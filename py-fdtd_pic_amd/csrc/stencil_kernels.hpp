@@ -50,11 +50,6 @@ __device__ __forceinline__ double lane_read(double v, int lane) {
 // multi-wave workgroup).
 __device__ __forceinline__ int wave_lane() { return (int)(threadIdx.x & 63u); }
 
-// dst with lane `lane` replaced by the uniform value v (a lane-id select)
-template <typename R>
-__device__ __forceinline__ R lane_write(R v, int lane, R dst) {
-    return (wave_lane() == lane) ? v : dst;
-}
 constexpr int kChunks = 4;    // S <= 255: a launch spans at most 4 x 64 steps
 
 // Kerr small-z series y(z) = sum_k (-1)^k C(3k,k)/(2k+1) z^k; FP64 keeps 14
@@ -208,6 +203,7 @@ struct TileCtx {
     double Jpre[2];
     double* probe_out;      // FP64 samples (FP32 fields: hi - c of the Kahan pair)
     R* snap;
+    R (*prl)[2][64];        // the wave's LDS rows of probe samples: [probe][E, cE][step in the chunk]
     // workgroup halo exchange (M_GROUP): this wave's place in its workgroup,
     // the exchange period in steps (= halo cells on an inner side) and the
     // emulated vacuum cells (E-form Kerr slab-boundary tiles)
@@ -360,16 +356,20 @@ __device__ __forceinline__ void run_steps(TILE_PARAMS, const TileCtx<R>& c, cons
     constexpr bool GROUP = (MODE & M_GROUP) != 0;
     int xleft = c.hx, xbuf = 0;      // steps to the next halo exchange, its buffer
     // source values of this launch, one step per lane (read back with
-    // v_readlane), and probe samples collected with lane selects and stored
-    // once after the loop: no memory access on the per-step path
+    // v_readlane): no global memory access on the per-step path
     // (chunk q of 64 steps sits in srcEc[0] while it runs; the arrays rotate
     // at chunk ends so every index is a compile-time constant)
     // The source and TFSF cells enter through 0/1 masks per lane and cell:
     // E + 1*v is E + v exactly and E + 0*v is E, so the injection needs no
     // branch on the lane or the cell.
-    // probe samples of the current 64 steps, lane j holding step j: the field
-    // and (FP32) its Kahan compensation, combined in FP64 when stored
-    R srcEc[kChunks], srcHc[kChunks], pacc[2], pcc[2], ms[K], mt[K];
+    // Probe samples of the current 64 steps: the lane holding the probe cell
+    // writes the field and (FP32) its Kahan compensation into the wave's LDS
+    // rows c.prl, step j of the chunk at index j (one masked LDS store per
+    // value, against a v_readlane, its wait states and a select per value
+    // when the samples were gathered lane by lane in registers: config 2
+    // launches 12.78 against 13.27 us); at the chunk's end lane j reads step
+    // j back, combines the pair in FP64 and stores it
+    R srcEc[kChunks], srcHc[kChunks], ms[K], mt[K];
     if constexpr (SRC) {
 #pragma unroll
         for (int j = 0; j < K; ++j) {
@@ -383,9 +383,6 @@ __device__ __forceinline__ void run_steps(TILE_PARAMS, const TileCtx<R>& c, cons
             srcEc[q] = (in && c.src_lane >= 0) ? c.srcE[c.t_first + ix] : R(0);
             srcHc[q] = (in && c.tf_lane >= 0) ? c.srcH[c.t_first + ix] : R(0);
         }
-    }
-    if constexpr (PROBE) {
-        pacc[0] = pacc[1] = pcc[0] = pcc[1] = R(0);
     }
     // the step body, called twice per loop trip: with two copies the
     // loop-carried registers alternate instead of being moved back each step
@@ -464,19 +461,18 @@ __device__ __forceinline__ void run_steps(TILE_PARAMS, const TileCtx<R>& c, cons
         if constexpr (PROBE && !GEN) {
 #pragma unroll
             for (int k = 0; k < 2; ++k)
-                if (k < c.npr) {
-                    const R v = lane_read(pick<R, K>(E, c.pr_rel[k] % K), c.pr_rel[k] / K);
-                    pacc[k] = lane_write(v, ix & 63, pacc[k]);
-                    if (KAHAN) {
-                        const R cv = lane_read(pick<R, K>(cE, c.pr_rel[k] % K), c.pr_rel[k] / K);
-                        pcc[k] = lane_write(cv, ix & 63, pcc[k]);
-                    }
+                if (k < c.npr && c.lane == c.pr_rel[k] / K) {
+                    c.prl[k][0][ix & 63] = pick<R, K>(E, c.pr_rel[k] % K);
+                    if (KAHAN) c.prl[k][1][ix & 63] = pick<R, K>(cE, c.pr_rel[k] % K);
                 }
         }
         // end of a 64-step chunk: flush probe samples, rotate source chunks
         if constexpr (PROBE || SRC) {
             if (((ix & 63) == 63) || s == c.s_hi - 1) {
                 if constexpr (PROBE && !GEN) {
+                    // the owning lanes' LDS stores come before the other lanes' reads
+                    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
                     const int tt = c.t_first + (ix & ~63) + c.lane;
 #pragma unroll
                     for (int k = 0; k < 2; ++k)
@@ -484,7 +480,11 @@ __device__ __forceinline__ void run_steps(TILE_PARAMS, const TileCtx<R>& c, cons
                             // FP64 in both precisions: an FP32 field as hi - c of
                             // its Kahan pair (the value fdtd_read_field returns)
                             c.probe_out[(long long)c.pr_id[k] * c.T + tt] =
-                                KAHAN ? (double)pacc[k] - (double)pcc[k] : (double)pacc[k];
+                                KAHAN ? (double)c.prl[k][0][c.lane] - (double)c.prl[k][1][c.lane]
+                                      : (double)c.prl[k][0][c.lane];
+                    // and the reads before the next chunk's stores
+                    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
                 }
                 if constexpr (SRC) {
 #pragma unroll
@@ -596,6 +596,9 @@ __device__ __forceinline__ void tile_body(const SimDev<R>& sd, int out0_, long l
     // registers: the loop's global stores (probes, snapshots) could alias the
     // descriptor, so the compiler would otherwise reload it every step.
     TileCtx<R> c;
+    // probe sample rows, one set per wave of the 256-thread workgroup
+    __shared__ R pr_rows[kTileWaves][2][2][64];
+    c.prl = pr_rows[__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6))];
     c.lane = lane;
     c.t0 = t0;
     c.wv = wv;
